@@ -1,0 +1,91 @@
+// Device-resident tables of one sparsity pattern (POD structs of device
+// pointers, passed by value to kernels or stored in device memory for the
+// fused multi-matrix launch).
+//
+// Layout in HBM (all 32-bit indices unless noted, one copy per pattern):
+//   factor: diag/lcnt/ufirst/ucnt/moff per column, upos (CSR -> CSC position,
+//           nnz_lu), umap (one target CSC position per scalar update,
+//           grouped per column k as |U_k| x |L_k| row-major), work units per
+//           level, columns to scale per level.
+//   solves: row lists per level for the four triangular sweeps.
+//   refine: CSR of the original A (row pointers, columns, source index into
+//           the caller's CSC values), per-LU-position source index and the
+//           frozen MC64 scale vectors.
+// Values (FP32 L\U, FP64 A, FP64 x/b) are separate buffers indexed by
+// matrix; a batch lays matrices out back to back (matrix-major).
+#pragma once
+
+#include <cstdint>
+
+namespace glu {
+
+struct Unit {          // one wave's share of a column update
+    uint32_t col;      // pivot column k
+    uint32_t jb;       // first U_k entry (index into the CSR row past the diagonal)
+    uint32_t jc;       // number of U_k entries
+    uint32_t pad;
+};
+
+struct FactorDev {
+    const uint32_t *col_ptr;   // sym_c_ptr [n+1]
+    const uint32_t *diag;      // l_col_ptr [n]
+    const uint32_t *lcnt;      // |L_k| [n]
+    const uint32_t *ufirst;    // csr_diag_ptr + 1 [n]
+    const uint64_t *moff;      // offset of column k's block in umap [n]
+    const uint32_t *upos;      // CSR position -> CSC position [nnz]
+    const uint32_t *umap;      // scatter targets [updates]
+    const Unit *units;         // [units]
+    const uint32_t *unit_ptr;  // per level [num_lev + 2]
+    const uint32_t *dcols;     // columns with |L_k| > 0, by level
+    const uint32_t *dcol_ptr;  // per level [num_lev + 1]
+    int32_t num_lev;
+    uint32_t n, nnz;
+};
+
+// One level-scheduled triangular sweep, row-pull form:
+//   x[r] = (x[r] - sum_{e in row r} V[vpos(e)] * x[idx[e]]) [/ V[dpos[r]]]
+// with row r's entries [beg[r] + begoff, end[r]) taken in ascending order,
+// or descending when desc != 0.
+struct TriDev {
+    const uint32_t *beg;
+    const uint32_t *end;
+    const uint32_t *idx;
+    const uint32_t *vpos;      // nullptr: identity
+    const uint32_t *dpos;      // nullptr: unit diagonal
+    const uint32_t *rows;      // rows grouped by level
+    const uint32_t *lvl_ptr;   // [nlev + 1]
+    uint32_t begoff;
+    int32_t desc;
+    int32_t nlev;
+    int32_t pad;
+};
+
+struct SolveDev {
+    TriDev lo, up;             // A x = b:   L then U
+    TriDev tup, tlo;           // A' x = b:  U' then L'
+    const uint32_t *row_perm, *row_perm_inv, *col_perm, *col_perm_inv;
+    const double *row_scale, *col_scale_perm;
+};
+
+struct RefineDev {
+    const uint32_t *a_rptr;    // CSR of A [n+1]
+    const uint32_t *a_cidx;    // [nnz_a]
+    const uint32_t *a_src;     // index into the caller's CSC values [nnz_a]
+    const uint32_t *a_cptr;    // caller CSC column pointers [n+1]
+    const uint32_t *a_ridx;    // caller CSC row indices [nnz_a]
+    const uint32_t *lu_src;    // LU position -> caller value index, or ~0u [nnz_lu]
+    const uint32_t *row_idx;   // sym_r_idx [nnz_lu]
+    const double *rs_col;      // row_scale[row_perm[c]] per GLU column c [n]
+    const double *csp;         // col_scale_perm [n]
+    uint32_t nnz_a;
+    uint32_t pad;
+};
+
+struct MatDesc {
+    FactorDev f;
+    SolveDev s;
+    RefineDev r;
+    uint32_t n, nnz;
+};
+
+}  // namespace glu

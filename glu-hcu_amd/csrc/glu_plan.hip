@@ -1,0 +1,1072 @@
+// Plan construction (host) and the C-ABI of include/glu_hip.h.
+//
+// A plan uploads one sparsity pattern once: the Symbolic_Matrix arrays that
+// LUonDevice copies on every call (src/numeric.cu:435-458), plus the tables
+// this build adds — the CSR->CSC map, the per-update scatter map, work
+// units and level schedules for the factorization and the four triangular
+// sweeps, and (from an analysis) the permutation/scales and the structure of
+// the original A for value refresh and FP64 refinement.
+//
+// Every index array is validated on the host before upload (pointer
+// monotonicity, ranges, sorted unique rows, diagonal positions, CSR/CSC
+// agreement, level permutation, fill closure); the kernels index without
+// bounds checks and rely on it.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/glu_hip.h"
+#include "glu_dev.h"
+#include "host/glu_host.h"
+
+namespace glu {
+namespace launch {
+void pert(const FactorDev &F, const float *val, unsigned int *norm_bits, float *d_pert, hipStream_t st);
+void factor_grid(const FactorDev &F, float *val, const float *d_pert, int lvl, uint32_t items, int perturb,
+                 int threads, hipStream_t st);
+void factor_fused(const FactorDev &F, float *val, const float *d_pert, int l0, int l1, int perturb,
+                  int threads, hipStream_t st);
+void tri_grid(const TriDev &T, const float *lu, float *x, int lvl, uint32_t rows, hipStream_t st);
+void tri_fused(const TriDev &T, const float *lu, float *x, int l0, int l1, int threads, hipStream_t st);
+void perm_scale_f32(const float *in, float *out, const uint32_t *P, const double *S, int by_perm, uint32_t n,
+                    hipStream_t st);
+void perm_scale_f64(const double *in, float *out, const uint32_t *P, const double *S, int by_perm,
+                    uint32_t n, hipStream_t st);
+void update_x(const float *t, double *x, const uint32_t *P, const double *S, int by_perm, uint32_t n,
+              hipStream_t st);
+void refresh(const RefineDev &R, const uint32_t *col_ptr, const double *ax, float *lu, uint32_t n,
+             uint32_t nnz_lu, int batch, hipStream_t st);
+void residual(const RefineDev &R, const double *ax, const double *b, const double *x, double *r, uint32_t n,
+              int transpose, int with_anorm, unsigned long long *norms, hipStream_t st);
+void batch_same(const MatDesc *desc, int batch, const double *ax, const double *b, double *x, float *lu,
+                float *t32, double *r64, int32_t *iters, double *eta, double tol, int maxit, int perturb,
+                int threads, hipStream_t st);
+void batch_group(const MatDesc *descs, const void *ios, const uint32_t *order, int count, double tol,
+                 int maxit, int perturb, int threads, hipStream_t st);
+void zero_u64(unsigned long long *p, int count, hipStream_t st);
+size_t matio_size();
+}  // namespace launch
+}  // namespace glu
+
+using namespace glu;
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+
+static thread_local std::string g_err;
+
+static glu_status fail(glu_status s, const std::string &msg)
+{
+    g_err = msg;
+    return s;
+}
+
+#define HIP_TRY(call)                                                                              \
+    do {                                                                                           \
+        hipError_t e_ = (call);                                                                    \
+        if (e_ != hipSuccess) return fail(GLU_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+extern "C" const char *glu_last_error(void) { return g_err.c_str(); }
+
+// ---------------------------------------------------------------------------
+// analysis object
+// ---------------------------------------------------------------------------
+
+struct glu_analysis {
+    Analysis an;
+    Symbolic sym;
+    std::vector<uint32_t> lu_from_dump;
+    int64_t updates = 0;
+};
+
+static int64_t count_updates(const Symbolic &s)
+{
+    int64_t u = 0;
+    for (uint32_t k = 0; k < s.n; ++k) {
+        int64_t L = (int64_t)s.sym_c_ptr[k + 1] - s.l_col_ptr[k] - 1;
+        int64_t U = (int64_t)s.csr_r_ptr[k + 1] - s.csr_diag_ptr[k] - 1;
+        u += L * U;
+    }
+    return u;
+}
+
+extern "C" glu_status glu_analyze(uint32_t n, uint32_t nnz, const double *ax, const uint32_t *ai,
+                                  const uint32_t *ap, glu_analysis **out)
+{
+    if (!out) return fail(GLU_ERR_ARGUMENT, "out is null");
+    *out = nullptr;
+    if (n == 0 || nnz == 0) return fail(GLU_ERR_EMPTY, "matrix is empty");
+    if (!ax || !ai || !ap) return fail(GLU_ERR_ARGUMENT, "null array");
+    std::unique_ptr<glu_analysis> a(new (std::nothrow) glu_analysis());
+    if (!a) return fail(GLU_ERR_MEMORY, "out of host memory");
+    std::string err;
+    int rc;
+    try {
+        rc = analyze(n, nnz, ax, ai, ap, a->an, err);
+        if (rc == kOk) {
+            // Duplicates the reference's adjacency check lets through
+            // (non-adjacent repeats in one column) are refused here.
+            std::vector<uint32_t> stamp(n, UINT32_MAX);
+            for (uint32_t c = 0; c < n && rc == kOk; ++c)
+                for (uint32_t k = ap[c]; k < ap[c + 1]; ++k) {
+                    if (stamp[ai[k]] == c) {
+                        rc = kErrDuplicate;
+                        err = "duplicate entry";
+                        break;
+                    }
+                    stamp[ai[k]] = c;
+                }
+        }
+        if (rc == kOk) {
+            symbolic(n, a->an.ai.data(), a->an.ap.data(), a->sym);
+            predict_lu(a->an.ai.data(), a->an.ap.data(), a->an.ax.data(), a->sym, &a->lu_from_dump);
+            a->updates = count_updates(a->sym);
+        }
+    } catch (const std::bad_alloc &) {
+        return fail(GLU_ERR_MEMORY, "out of host memory during analysis");
+    }
+    if (rc != kOk) return fail((glu_status)rc, err);
+    *out = a.release();
+    return GLU_OK;
+}
+
+extern "C" void glu_analysis_destroy(glu_analysis *a) { delete a; }
+
+extern "C" glu_status glu_analysis_info(const glu_analysis *a, int64_t info[8])
+{
+    if (!a || !info) return fail(GLU_ERR_ARGUMENT, "null argument");
+    info[0] = a->sym.n;
+    info[1] = a->an.nnz;
+    info[2] = a->sym.nnz;
+    info[3] = a->sym.num_lev;
+    info[4] = a->updates;
+    info[5] = info[6] = info[7] = 0;
+    return GLU_OK;
+}
+
+template <typename T>
+static glu_status copy_field(const std::vector<T> &v, void *dst, size_t bytes)
+{
+    if (bytes != v.size() * sizeof(T))
+        return fail(GLU_ERR_ARGUMENT, "size mismatch: expected " + std::to_string(v.size() * sizeof(T)) +
+                                          " bytes, got " + std::to_string(bytes));
+    if (bytes) std::memcpy(dst, v.data(), bytes);
+    return GLU_OK;
+}
+
+extern "C" glu_status glu_analysis_get(const glu_analysis *a, glu_field f, void *dst, size_t bytes)
+{
+    if (!a || (!dst && bytes)) return fail(GLU_ERR_ARGUMENT, "null argument");
+    const Symbolic &s = a->sym;
+    const Analysis &an = a->an;
+    switch (f) {
+    case GLU_F_SYM_C_PTR: return copy_field(s.sym_c_ptr, dst, bytes);
+    case GLU_F_SYM_R_IDX: return copy_field(s.sym_r_idx, dst, bytes);
+    case GLU_F_L_COL_PTR: return copy_field(s.l_col_ptr, dst, bytes);
+    case GLU_F_CSR_R_PTR: return copy_field(s.csr_r_ptr, dst, bytes);
+    case GLU_F_CSR_C_IDX: return copy_field(s.csr_c_idx, dst, bytes);
+    case GLU_F_CSR_DIAG_PTR: return copy_field(s.csr_diag_ptr, dst, bytes);
+    case GLU_F_VAL: return copy_field(s.val, dst, bytes);
+    case GLU_F_LEVEL_PTR: return copy_field(s.level_ptr, dst, bytes);
+    case GLU_F_LEVEL_IDX: return copy_field(s.level_idx, dst, bytes);
+    case GLU_F_ROW_PERM: return copy_field(an.row_perm, dst, bytes);
+    case GLU_F_ROW_PERM_INV: return copy_field(an.row_perm_inv, dst, bytes);
+    case GLU_F_COL_PERM: return copy_field(an.col_perm, dst, bytes);
+    case GLU_F_COL_PERM_INV: return copy_field(an.col_perm_inv, dst, bytes);
+    case GLU_F_ROW_SCALE: return copy_field(an.row_scale, dst, bytes);
+    case GLU_F_COL_SCALE_PERM: return copy_field(an.col_scale_perm, dst, bytes);
+    case GLU_F_DUMP_AX: return copy_field(an.ax, dst, bytes);
+    case GLU_F_DUMP_AI: return copy_field(an.ai, dst, bytes);
+    case GLU_F_DUMP_AP: return copy_field(an.ap, dst, bytes);
+    }
+    return fail(GLU_ERR_ARGUMENT, "unknown field");
+}
+
+// ---------------------------------------------------------------------------
+// plan
+// ---------------------------------------------------------------------------
+
+namespace {
+
+constexpr uint32_t kUnitTarget = 256;   // scalar updates per work unit
+constexpr int kGridThreads = 256;
+constexpr int kDefaultFusedThreads = 512;
+constexpr int kDefaultFuseUnits = 32;
+
+struct Segment {
+    int l0, l1;      // levels [l0, l1)
+    bool fused;
+    uint32_t items;  // grid segments: items (factor) or rows (solve)
+};
+
+struct TriHost {
+    std::vector<uint32_t> rows, lvl_ptr;
+    std::vector<Segment> segs;
+};
+
+// Device arena: one allocation, 256-byte aligned sub-buffers.
+struct Arena {
+    std::vector<std::pair<size_t, size_t>> parts;  // offset, bytes
+    size_t total = 0;
+    size_t add(size_t bytes)
+    {
+        size_t off = total;
+        total += (bytes + 255) & ~size_t(255);
+        parts.push_back({off, bytes});
+        return off;
+    }
+};
+
+}  // namespace
+
+struct glu_plan {
+    int device = 0;
+    uint32_t n = 0, nnz = 0, nnz_a = 0;
+    int num_lev = 0;
+    int64_t updates = 0;
+    int64_t units = 0;
+    int fused_threads = kDefaultFusedThreads;
+    int fuse_units = kDefaultFuseUnits;
+    bool has_scaling = false, has_refine = false;
+
+    std::vector<Segment> fsegs;
+    TriHost tri[4];          // lo, up, tup, tlo
+
+    char *d_arena = nullptr;
+    size_t arena_bytes = 0;
+    FactorDev F{};
+    SolveDev S{};
+    RefineDev R{};
+    MatDesc *d_desc = nullptr;   // lives in the arena
+
+    // scalars and single-matrix workspaces (also in the arena)
+    unsigned int *d_norm_bits = nullptr;
+    float *d_pert = nullptr;
+    unsigned long long *d_norms = nullptr;
+    float *d_t32 = nullptr;      // n
+    double *d_r64 = nullptr;     // n
+    float *d_val = nullptr;      // nnz (glu_factor_host)
+
+    // batch workspaces, grown on demand
+    float *d_bt32 = nullptr;
+    double *d_br64 = nullptr;
+    int batch_cap = 0;
+
+    ~glu_plan()
+    {
+        if (d_arena) (void)hipFree(d_arena);
+        if (d_val) (void)hipFree(d_val);
+        if (d_bt32) (void)hipFree(d_bt32);
+        if (d_br64) (void)hipFree(d_br64);
+    }
+};
+
+namespace {
+
+// ------------------------- validation -----------------------------------
+
+bool check_ptr(const uint32_t *p, uint32_t n, uint32_t total, std::string &err, const char *what)
+{
+    if (p[0] != 0 || p[n] != total) {
+        err = std::string(what) + ": pointer array must start at 0 and end at the entry count";
+        return false;
+    }
+    for (uint32_t i = 0; i < n; ++i)
+        if (p[i] > p[i + 1]) {
+            err = std::string(what) + ": pointer array not monotone";
+            return false;
+        }
+    return true;
+}
+
+bool validate_pattern(const glu_pattern &P, std::string &err)
+{
+    const uint32_t n = P.n, nnz = P.nnz;
+    if (!P.sym_c_ptr || !P.sym_r_idx || !P.l_col_ptr || !P.csr_r_ptr || !P.csr_c_idx || !P.csr_diag_ptr ||
+        !P.level_ptr || !P.level_idx) {
+        err = "null pattern array";
+        return false;
+    }
+    if (P.num_lev <= 0 || (uint32_t)P.num_lev > n) {
+        err = "num_lev out of range";
+        return false;
+    }
+    if (!check_ptr(P.sym_c_ptr, n, nnz, err, "sym_c_ptr") || !check_ptr(P.csr_r_ptr, n, nnz, err, "csr_r_ptr"))
+        return false;
+    for (uint32_t c = 0; c < n; ++c) {
+        for (uint32_t p = P.sym_c_ptr[c]; p < P.sym_c_ptr[c + 1]; ++p) {
+            if (P.sym_r_idx[p] >= n || (p > P.sym_c_ptr[c] && P.sym_r_idx[p] <= P.sym_r_idx[p - 1])) {
+                err = "sym_r_idx out of range or not strictly ascending in column " + std::to_string(c);
+                return false;
+            }
+        }
+        const uint32_t d = P.l_col_ptr[c];
+        if (d < P.sym_c_ptr[c] || d >= P.sym_c_ptr[c + 1] || P.sym_r_idx[d] != c) {
+            err = "l_col_ptr does not point at the diagonal of column " + std::to_string(c);
+            return false;
+        }
+    }
+    // CSR must be the exact transpose of the CSC pattern.
+    std::vector<uint32_t> cursor(P.csr_r_ptr, P.csr_r_ptr + n);
+    for (uint32_t c = 0; c < n; ++c)
+        for (uint32_t p = P.sym_c_ptr[c]; p < P.sym_c_ptr[c + 1]; ++p) {
+            const uint32_t r = P.sym_r_idx[p];
+            const uint32_t q = cursor[r]++;
+            if (q >= P.csr_r_ptr[r + 1] || P.csr_c_idx[q] != c) {
+                err = "CSR arrays are not the transpose of the CSC pattern";
+                return false;
+            }
+        }
+    for (uint32_t r = 0; r < n; ++r) {
+        const uint32_t q = P.csr_diag_ptr[r];
+        if (q < P.csr_r_ptr[r] || q >= P.csr_r_ptr[r + 1] || P.csr_c_idx[q] != r) {
+            err = "csr_diag_ptr does not point at the diagonal of row " + std::to_string(r);
+            return false;
+        }
+    }
+    if (P.level_ptr[0] != 0 || (uint32_t)P.level_ptr[P.num_lev] != n) {
+        err = "level_ptr must span 0..n";
+        return false;
+    }
+    for (int l = 0; l < P.num_lev; ++l)
+        if (P.level_ptr[l] > P.level_ptr[l + 1]) {
+            err = "level_ptr not monotone";
+            return false;
+        }
+    std::vector<int> level_of(n, -1);
+    for (int l = 0; l < P.num_lev; ++l)
+        for (int t = P.level_ptr[l]; t < P.level_ptr[l + 1]; ++t) {
+            const int k = P.level_idx[t];
+            if (k < 0 || (uint32_t)k >= n || level_of[k] != -1) {
+                err = "level_idx is not a permutation";
+                return false;
+            }
+            level_of[k] = l;
+        }
+    // Dependencies of the right-looking update must sit in earlier levels
+    // (src/symbolic.cc:115-130): otherwise two columns of one level race.
+    for (uint32_t i = 0; i < n; ++i) {
+        for (uint32_t p = P.sym_c_ptr[i]; p < P.l_col_ptr[i]; ++p) {
+            const uint32_t r = P.sym_r_idx[p];
+            if (P.l_col_ptr[r] + 1 != P.sym_c_ptr[r + 1] && level_of[r] >= level_of[i]) {
+                err = "levels violate a column dependency";
+                return false;
+            }
+        }
+        for (uint32_t q = P.csr_r_ptr[i]; q < P.csr_diag_ptr[i]; ++q)
+            if (level_of[P.csr_c_idx[q]] >= level_of[i]) {
+                err = "levels violate a row dependency";
+                return false;
+            }
+    }
+    return true;
+}
+
+// --------------------------- tables -------------------------------------
+
+struct Tables {
+    std::vector<uint32_t> lcnt, ufirst, upos, csc2csr, umap;
+    std::vector<uint64_t> moff;
+    std::vector<Unit> units;
+    std::vector<uint32_t> unit_ptr, dcols, dcol_ptr;
+};
+
+bool build_factor_tables(const glu_pattern &P, Tables &T, int64_t &updates, std::string &err)
+{
+    const uint32_t n = P.n, nnz = P.nnz;
+    T.lcnt.resize(n);
+    T.ufirst.resize(n);
+    T.moff.resize(n + 1);
+    T.upos.resize(nnz);
+    T.csc2csr.resize(nnz);
+    uint64_t off = 0;
+    for (uint32_t k = 0; k < n; ++k) {
+        T.lcnt[k] = P.sym_c_ptr[k + 1] - P.l_col_ptr[k] - 1;
+        T.ufirst[k] = P.csr_diag_ptr[k] + 1;
+        const uint32_t U = P.csr_r_ptr[k + 1] - T.ufirst[k];
+        T.moff[k] = off;
+        off += (uint64_t)T.lcnt[k] * U;
+    }
+    T.moff[n] = off;
+    updates = (int64_t)off;
+    {
+        std::vector<uint32_t> cursor(P.csr_r_ptr, P.csr_r_ptr + n);
+        for (uint32_t c = 0; c < n; ++c)
+            for (uint32_t p = P.sym_c_ptr[c]; p < P.sym_c_ptr[c + 1]; ++p) {
+                const uint32_t q = cursor[P.sym_r_idx[p]]++;
+                T.upos[q] = p;
+                T.csc2csr[p] = q;
+            }
+    }
+    // Scatter map, built per target column j: for every k with U(k,j) != 0
+    // and |L_k| > 0, the positions of L_k's rows inside column j.
+    try {
+        T.umap.assign(off, 0);
+    } catch (const std::bad_alloc &) {
+        err = "scatter map too large for host memory";
+        return false;
+    }
+    std::vector<uint32_t> stamp(n, UINT32_MAX), where(n);
+    for (uint32_t j = 0; j < n; ++j) {
+        for (uint32_t p = P.sym_c_ptr[j]; p < P.sym_c_ptr[j + 1]; ++p) {
+            stamp[P.sym_r_idx[p]] = j;
+            where[P.sym_r_idx[p]] = p;
+        }
+        for (uint32_t p = P.sym_c_ptr[j]; p < P.l_col_ptr[j]; ++p) {
+            const uint32_t k = P.sym_r_idx[p];
+            const uint32_t L = T.lcnt[k];
+            if (L == 0) continue;
+            const uint32_t jj = T.csc2csr[p] - T.ufirst[k];
+            uint64_t base = T.moff[k] + (uint64_t)jj * L;
+            const uint32_t l0 = P.l_col_ptr[k] + 1;
+            for (uint32_t ii = 0; ii < L; ++ii) {
+                const uint32_t i = P.sym_r_idx[l0 + ii];
+                if (stamp[i] != j) {
+                    err = "pattern is not closed under fill: (" + std::to_string(i) + "," + std::to_string(j) +
+                          ") missing";
+                    return false;
+                }
+                T.umap[base + ii] = where[i];
+            }
+        }
+    }
+    // Work units and deferred write-backs, level by level.
+    const int nl = P.num_lev;
+    T.unit_ptr.assign(nl + 2, 0);
+    T.dcol_ptr.assign(nl + 1, 0);
+    for (int l = 0; l < nl; ++l) {
+        T.unit_ptr[l] = (uint32_t)T.units.size();
+        T.dcol_ptr[l] = (uint32_t)T.dcols.size();
+        for (int t = P.level_ptr[l]; t < P.level_ptr[l + 1]; ++t) {
+            const uint32_t k = (uint32_t)P.level_idx[t];
+            const uint32_t L = T.lcnt[k];
+            if (L == 0) continue;
+            T.dcols.push_back(k);
+            const uint32_t U = P.csr_r_ptr[k + 1] - T.ufirst[k];
+            const uint32_t chunk = std::max<uint32_t>(1, kUnitTarget / L);
+            for (uint32_t jb = 0; jb < U; jb += chunk) T.units.push_back(Unit{k, jb, std::min(chunk, U - jb), 0});
+        }
+    }
+    T.unit_ptr[nl] = T.unit_ptr[nl + 1] = (uint32_t)T.units.size();
+    T.dcol_ptr[nl] = (uint32_t)T.dcols.size();
+    return true;
+}
+
+// Level sets of one triangular sweep.  dir > 0: rows ascending, dependencies
+// on smaller indices; dir < 0: descending.  deps(r, f) calls f(c) for each
+// dependency c of row r.
+template <typename Deps>
+void tri_levels(uint32_t n, int dir, Deps deps, TriHost &H)
+{
+    std::vector<int> lev(n, 0);
+    int nlev = 0;
+    for (uint32_t t = 0; t < n; ++t) {
+        const uint32_t r = dir > 0 ? t : n - 1 - t;
+        int m = -1;
+        deps(r, [&](uint32_t c) { m = std::max(m, lev[c]); });
+        lev[r] = m + 1;
+        nlev = std::max(nlev, m + 2);
+    }
+    H.lvl_ptr.assign(nlev + 1, 0);
+    for (uint32_t r = 0; r < n; ++r) ++H.lvl_ptr[lev[r] + 1];
+    for (int l = 0; l < nlev; ++l) H.lvl_ptr[l + 1] += H.lvl_ptr[l];
+    H.rows.assign(n, 0);
+    std::vector<uint32_t> cur(H.lvl_ptr.begin(), H.lvl_ptr.end() - 1);
+    for (uint32_t r = 0; r < n; ++r) H.rows[cur[lev[r]]++] = r;
+}
+
+void make_segments(const std::vector<uint32_t> &work, uint32_t narrow, std::vector<Segment> &segs)
+{
+    segs.clear();
+    const int nl = (int)work.size();
+    int l = 0;
+    while (l < nl) {
+        if (work[l] > narrow) {
+            segs.push_back(Segment{l, l + 1, false, work[l]});
+            ++l;
+            continue;
+        }
+        int e = l;
+        while (e < nl && work[e] <= narrow) ++e;
+        segs.push_back(Segment{l, e, true, 0});
+        l = e;
+    }
+}
+
+void factor_segments(glu_plan &p, const Tables &T)
+{
+    std::vector<uint32_t> work(p.num_lev + 1);
+    for (int l = 0; l <= p.num_lev; ++l) {
+        uint32_t items = T.unit_ptr[l + 1] - T.unit_ptr[l];
+        if (l > 0) items += T.dcol_ptr[l] - T.dcol_ptr[l - 1];
+        work[l] = items;
+    }
+    make_segments(work, (uint32_t)p.fuse_units, p.fsegs);
+}
+
+void tri_segments(glu_plan &p, TriHost &H)
+{
+    const int nlev = (int)H.lvl_ptr.size() - 1;
+    std::vector<uint32_t> work(nlev);
+    for (int l = 0; l < nlev; ++l) work[l] = H.lvl_ptr[l + 1] - H.lvl_ptr[l];
+    make_segments(work, (uint32_t)p.fused_threads * 4, H.segs);
+}
+
+struct Uploader {
+    Arena arena;
+    struct Item {
+        size_t off;
+        const void *src;
+        size_t bytes;
+    };
+    std::vector<Item> items;
+    template <typename T>
+    size_t put(const T *src, size_t count)
+    {
+        size_t off = arena.add(count * sizeof(T));
+        items.push_back(Item{off, src, count * sizeof(T)});
+        return off;
+    }
+    template <typename T>
+    size_t put(const std::vector<T> &v)
+    {
+        return put(v.data(), v.size());
+    }
+    size_t reserve(size_t bytes) { return arena.add(bytes); }
+};
+
+glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_plan_opts *opts, glu_plan **out)
+{
+    *out = nullptr;
+    std::string err;
+    if (P.n == 0 || P.nnz == 0) return fail(GLU_ERR_EMPTY, "matrix is empty");
+    if (!validate_pattern(P, err)) return fail(GLU_ERR_INVALID, err);
+
+    std::unique_ptr<glu_plan> p(new (std::nothrow) glu_plan());
+    if (!p) return fail(GLU_ERR_MEMORY, "out of host memory");
+    if (opts) {
+        if (opts->block_threads) {
+            if (opts->block_threads < 64 || opts->block_threads > 1024 || opts->block_threads % 64)
+                return fail(GLU_ERR_ARGUMENT, "block_threads must be a multiple of 64 in [64, 1024]");
+            p->fused_threads = opts->block_threads;
+        }
+        if (opts->fuse_units < 0) return fail(GLU_ERR_ARGUMENT, "fuse_units must be >= 0");
+        if (opts->fuse_units) p->fuse_units = opts->fuse_units;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(GLU_ERR_DEVICE, "no HIP device");
+    HIP_TRY(hipGetDevice(&p->device));
+
+    p->n = P.n;
+    p->nnz = P.nnz;
+    p->num_lev = P.num_lev;
+
+    Tables T;
+    try {
+        if (!build_factor_tables(P, T, p->updates, err)) return fail(GLU_ERR_INVALID, err);
+    } catch (const std::bad_alloc &) {
+        return fail(GLU_ERR_MEMORY, "out of host memory building tables");
+    }
+    factor_segments(*p, T);
+    p->units = (int64_t)T.units.size();
+
+    const uint32_t n = P.n;
+    // Solve schedules.
+    tri_levels(n, +1, [&](uint32_t r, auto f) {
+        for (uint32_t q = P.csr_r_ptr[r]; q < P.csr_diag_ptr[r]; ++q) f(P.csr_c_idx[q]);
+    }, p->tri[0]);
+    tri_levels(n, -1, [&](uint32_t r, auto f) {
+        for (uint32_t q = P.csr_diag_ptr[r] + 1; q < P.csr_r_ptr[r + 1]; ++q) f(P.csr_c_idx[q]);
+    }, p->tri[1]);
+    tri_levels(n, +1, [&](uint32_t r, auto f) {
+        for (uint32_t q = P.sym_c_ptr[r]; q < P.l_col_ptr[r]; ++q) f(P.sym_r_idx[q]);
+    }, p->tri[2]);
+    tri_levels(n, -1, [&](uint32_t r, auto f) {
+        for (uint32_t q = P.l_col_ptr[r] + 1; q < P.sym_c_ptr[r + 1]; ++q) f(P.sym_r_idx[q]);
+    }, p->tri[3]);
+    for (auto &h : p->tri) tri_segments(*p, h);
+
+    // Refinement / refresh tables from the analysis.
+    std::vector<uint32_t> a_rptr, a_cidx, a_src, lu_src;
+    std::vector<double> rs_col;
+    const Analysis *an = a ? &a->an : nullptr;
+    if (an) {
+        const uint32_t nnz_a = an->nnz;
+        p->nnz_a = nnz_a;
+        // CSR of the caller's A (rows = caller rows), remembering each
+        // entry's position in the caller's CSC value array.
+        a_rptr.assign(n + 1, 0);
+        for (uint32_t k = 0; k < nnz_a; ++k) ++a_rptr[an->in_ai[k] + 1];
+        for (uint32_t i = 0; i < n; ++i) a_rptr[i + 1] += a_rptr[i];
+        a_cidx.resize(nnz_a);
+        a_src.resize(nnz_a);
+        std::vector<uint32_t> cur(a_rptr.begin(), a_rptr.end() - 1);
+        for (uint32_t c = 0; c < n; ++c)
+            for (uint32_t k = an->in_ap[c]; k < an->in_ap[c + 1]; ++k) {
+                const uint32_t q = cur[an->in_ai[k]]++;
+                a_cidx[q] = c;
+                a_src[q] = k;
+            }
+        lu_src.resize(P.nnz);
+        for (uint32_t q = 0; q < P.nnz; ++q) {
+            const uint32_t d = a->lu_from_dump[q];
+            lu_src[q] = (d == UINT32_MAX) ? UINT32_MAX : an->src_pos[d];
+        }
+        rs_col.resize(n);
+        for (uint32_t c = 0; c < n; ++c) rs_col[c] = an->row_scale[an->row_perm[c]];
+        p->has_scaling = p->has_refine = true;
+    }
+
+    // ---------------- upload everything into one arena ------------------
+    Uploader U;
+    const size_t o_colptr = U.put(P.sym_c_ptr, n + 1);
+    const size_t o_rowidx = U.put(P.sym_r_idx, P.nnz);
+    const size_t o_diag = U.put(P.l_col_ptr, n);
+    const size_t o_crptr = U.put(P.csr_r_ptr, n + 1);
+    const size_t o_ccidx = U.put(P.csr_c_idx, P.nnz);
+    const size_t o_cdiag = U.put(P.csr_diag_ptr, n);
+    const size_t o_lcnt = U.put(T.lcnt);
+    const size_t o_ufirst = U.put(T.ufirst);
+    const size_t o_moff = U.put(T.moff);
+    const size_t o_upos = U.put(T.upos);
+    const size_t o_umap = U.put(T.umap);
+    const size_t o_units = U.put(T.units);
+    const size_t o_uptr = U.put(T.unit_ptr);
+    const size_t o_dcols = U.put(T.dcols);
+    const size_t o_dptr = U.put(T.dcol_ptr);
+    size_t o_trows[4], o_tptr[4];
+    for (int s = 0; s < 4; ++s) {
+        o_trows[s] = U.put(p->tri[s].rows);
+        o_tptr[s] = U.put(p->tri[s].lvl_ptr);
+    }
+    size_t o_rp = 0, o_rpi = 0, o_cp = 0, o_cpi = 0, o_rs = 0, o_csp = 0;
+    size_t o_arptr = 0, o_acidx = 0, o_asrc = 0, o_acptr = 0, o_aridx = 0, o_lusrc = 0, o_rscol = 0;
+    if (an) {
+        o_rp = U.put(an->row_perm);
+        o_rpi = U.put(an->row_perm_inv);
+        o_cp = U.put(an->col_perm);
+        o_cpi = U.put(an->col_perm_inv);
+        o_rs = U.put(an->row_scale);
+        o_csp = U.put(an->col_scale_perm);
+        o_arptr = U.put(a_rptr);
+        o_acidx = U.put(a_cidx);
+        o_asrc = U.put(a_src);
+        o_acptr = U.put(an->in_ap);
+        o_aridx = U.put(an->in_ai);
+        o_lusrc = U.put(lu_src);
+        o_rscol = U.put(rs_col);
+    }
+    const size_t o_desc = U.reserve(sizeof(MatDesc));
+    const size_t o_normbits = U.reserve(sizeof(unsigned int));
+    const size_t o_pert = U.reserve(sizeof(float));
+    const size_t o_norms = U.reserve(4 * sizeof(unsigned long long));
+    const size_t o_t32 = U.reserve(sizeof(float) * n);
+    const size_t o_r64 = U.reserve(sizeof(double) * n);
+
+    HIP_TRY(hipMalloc(&p->d_arena, U.arena.total));
+    p->arena_bytes = U.arena.total;
+    for (auto &it : U.items)
+        if (it.bytes) HIP_TRY(hipMemcpy(p->d_arena + it.off, it.src, it.bytes, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(p->d_arena + o_normbits, 0, sizeof(unsigned int)));
+
+    char *A = p->d_arena;
+    auto u32p = [&](size_t o) { return reinterpret_cast<const uint32_t *>(A + o); };
+    FactorDev &F = p->F;
+    F.col_ptr = u32p(o_colptr);
+    F.diag = u32p(o_diag);
+    F.lcnt = u32p(o_lcnt);
+    F.ufirst = u32p(o_ufirst);
+    F.moff = reinterpret_cast<const uint64_t *>(A + o_moff);
+    F.upos = u32p(o_upos);
+    F.umap = u32p(o_umap);
+    F.units = reinterpret_cast<const Unit *>(A + o_units);
+    F.unit_ptr = u32p(o_uptr);
+    F.dcols = u32p(o_dcols);
+    F.dcol_ptr = u32p(o_dptr);
+    F.num_lev = P.num_lev;
+    F.n = n;
+    F.nnz = P.nnz;
+
+    auto tri = [&](int s, const uint32_t *beg, uint32_t begoff, const uint32_t *end, const uint32_t *idx,
+                   const uint32_t *vpos, const uint32_t *dpos, int desc) {
+        TriDev t{};
+        t.beg = beg;
+        t.begoff = begoff;
+        t.end = end;
+        t.idx = idx;
+        t.vpos = vpos;
+        t.dpos = dpos;
+        t.desc = desc;
+        t.rows = u32p(o_trows[s]);
+        t.lvl_ptr = u32p(o_tptr[s]);
+        t.nlev = (int)p->tri[s].lvl_ptr.size() - 1;
+        return t;
+    };
+    const uint32_t *colptr = u32p(o_colptr), *rowidx = u32p(o_rowidx), *diag = u32p(o_diag);
+    const uint32_t *crptr = u32p(o_crptr), *ccidx = u32p(o_ccidx), *cdiag = u32p(o_cdiag);
+    SolveDev &S = p->S;
+    S.lo = tri(0, crptr, 0, cdiag, ccidx, F.upos, nullptr, 0);
+    S.up = tri(1, cdiag, 1, crptr + 1, ccidx, F.upos, diag, 1);
+    S.tup = tri(2, colptr, 0, diag, rowidx, nullptr, diag, 0);
+    S.tlo = tri(3, diag, 1, colptr + 1, rowidx, nullptr, nullptr, 0);
+    if (an) {
+        S.row_perm = u32p(o_rp);
+        S.row_perm_inv = u32p(o_rpi);
+        S.col_perm = u32p(o_cp);
+        S.col_perm_inv = u32p(o_cpi);
+        S.row_scale = reinterpret_cast<const double *>(A + o_rs);
+        S.col_scale_perm = reinterpret_cast<const double *>(A + o_csp);
+        RefineDev &R = p->R;
+        R.a_rptr = u32p(o_arptr);
+        R.a_cidx = u32p(o_acidx);
+        R.a_src = u32p(o_asrc);
+        R.a_cptr = u32p(o_acptr);
+        R.a_ridx = u32p(o_aridx);
+        R.lu_src = u32p(o_lusrc);
+        R.row_idx = rowidx;
+        R.rs_col = reinterpret_cast<const double *>(A + o_rscol);
+        R.csp = S.col_scale_perm;
+        R.nnz_a = p->nnz_a;
+    }
+    MatDesc md{};
+    md.f = F;
+    md.s = S;
+    md.r = p->R;
+    md.n = n;
+    md.nnz = P.nnz;
+    p->d_desc = reinterpret_cast<MatDesc *>(A + o_desc);
+    HIP_TRY(hipMemcpy(p->d_desc, &md, sizeof(MatDesc), hipMemcpyHostToDevice));
+    p->d_norm_bits = reinterpret_cast<unsigned int *>(A + o_normbits);
+    p->d_pert = reinterpret_cast<float *>(A + o_pert);
+    p->d_norms = reinterpret_cast<unsigned long long *>(A + o_norms);
+    p->d_t32 = reinterpret_cast<float *>(A + o_t32);
+    p->d_r64 = reinterpret_cast<double *>(A + o_r64);
+    *out = p.release();
+    return GLU_OK;
+}
+
+glu_pattern pattern_of(const Symbolic &s)
+{
+    glu_pattern P{};
+    P.n = s.n;
+    P.nnz = s.nnz;
+    P.num_lev = s.num_lev;
+    P.sym_c_ptr = s.sym_c_ptr.data();
+    P.sym_r_idx = s.sym_r_idx.data();
+    P.l_col_ptr = s.l_col_ptr.data();
+    P.csr_r_ptr = s.csr_r_ptr.data();
+    P.csr_c_idx = s.csr_c_idx.data();
+    P.csr_diag_ptr = s.csr_diag_ptr.data();
+    P.level_ptr = s.level_ptr.data();
+    P.level_idx = s.level_idx.data();
+    return P;
+}
+
+glu_status run_factor(glu_plan *p, float *d_val, int perturb, hipStream_t st)
+{
+    if (perturb) launch::pert(p->F, d_val, p->d_norm_bits, p->d_pert, st);
+    for (const Segment &s : p->fsegs) {
+        if (s.fused)
+            launch::factor_fused(p->F, d_val, p->d_pert, s.l0, s.l1, perturb, p->fused_threads, st);
+        else
+            launch::factor_grid(p->F, d_val, p->d_pert, s.l0, s.items, perturb, kGridThreads, st);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(GLU_ERR_DEVICE, std::string("factor launch: ") + hipGetErrorString(e));
+    return GLU_OK;
+}
+
+void run_tri(glu_plan *p, const TriDev &T, const TriHost &H, const float *lu, float *x, hipStream_t st)
+{
+    for (const Segment &s : H.segs) {
+        if (s.fused)
+            launch::tri_fused(T, lu, x, s.l0, s.l1, p->fused_threads, st);
+        else
+            launch::tri_grid(T, lu, x, s.l0, s.items, st);
+    }
+}
+
+// t (scaled, permuted) -> LU sweeps in place
+void run_sweeps(glu_plan *p, const float *lu, float *t, int transpose, hipStream_t st)
+{
+    if (!transpose) {
+        run_tri(p, p->S.lo, p->tri[0], lu, t, st);
+        run_tri(p, p->S.up, p->tri[1], lu, t, st);
+    } else {
+        run_tri(p, p->S.tup, p->tri[2], lu, t, st);
+        run_tri(p, p->S.tlo, p->tri[3], lu, t, st);
+    }
+}
+
+glu_status ensure_batch(glu_plan *p, int batch)
+{
+    if (batch <= p->batch_cap) return GLU_OK;
+    if (p->d_bt32) (void)hipFree(p->d_bt32);
+    if (p->d_br64) (void)hipFree(p->d_br64);
+    p->d_bt32 = nullptr;
+    p->d_br64 = nullptr;
+    p->batch_cap = 0;
+    HIP_TRY(hipMalloc(&p->d_bt32, sizeof(float) * (size_t)p->n * batch));
+    HIP_TRY(hipMalloc(&p->d_br64, sizeof(double) * (size_t)p->n * batch));
+    p->batch_cap = batch;
+    return GLU_OK;
+}
+
+}  // namespace
+
+extern "C" glu_status glu_plan_create(const glu_pattern *pat, const glu_plan_opts *opts, glu_plan **out)
+{
+    if (!pat || !out) return fail(GLU_ERR_ARGUMENT, "null argument");
+    try {
+        return build_plan(*pat, nullptr, opts, out);
+    } catch (const std::bad_alloc &) {
+        return fail(GLU_ERR_MEMORY, "out of host memory");
+    }
+}
+
+extern "C" glu_status glu_plan_create_from_analysis(const glu_analysis *a, const glu_plan_opts *opts,
+                                                    glu_plan **out)
+{
+    if (!a || !out) return fail(GLU_ERR_ARGUMENT, "null argument");
+    glu_pattern P = pattern_of(a->sym);
+    try {
+        return build_plan(P, a, opts, out);
+    } catch (const std::bad_alloc &) {
+        return fail(GLU_ERR_MEMORY, "out of host memory");
+    }
+}
+
+extern "C" void glu_plan_destroy(glu_plan *p) { delete p; }
+
+extern "C" glu_status glu_plan_info(const glu_plan *p, int64_t info[8])
+{
+    if (!p || !info) return fail(GLU_ERR_ARGUMENT, "null argument");
+    info[0] = p->n;
+    info[1] = p->nnz;
+    info[2] = p->nnz_a;
+    info[3] = p->num_lev;
+    info[4] = p->updates;
+    info[5] = (int64_t)p->fsegs.size();
+    info[6] = (int64_t)p->arena_bytes;
+    info[7] = p->units;
+    return GLU_OK;
+}
+
+extern "C" glu_status glu_factor(glu_plan *p, float *d_val, int perturb, void *stream)
+{
+    if (!p || !d_val) return fail(GLU_ERR_ARGUMENT, "null argument");
+    return run_factor(p, d_val, perturb, static_cast<hipStream_t>(stream));
+}
+
+extern "C" glu_status glu_factor_host(glu_plan *p, float *h_val, int perturb)
+{
+    if (!p || !h_val) return fail(GLU_ERR_ARGUMENT, "null argument");
+    if (!p->d_val) HIP_TRY(hipMalloc(&p->d_val, sizeof(float) * p->nnz));
+    HIP_TRY(hipMemcpy(p->d_val, h_val, sizeof(float) * p->nnz, hipMemcpyHostToDevice));
+    glu_status s = run_factor(p, p->d_val, perturb, nullptr);
+    if (s != GLU_OK) return s;
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(h_val, p->d_val, sizeof(float) * p->nnz, hipMemcpyDeviceToHost));
+    return GLU_OK;
+}
+
+extern "C" glu_status glu_solve(glu_plan *p, const float *d_lu, const float *d_b, float *d_x, int transpose,
+                                void *stream)
+{
+    if (!p || !d_lu || !d_b || !d_x) return fail(GLU_ERR_ARGUMENT, "null argument");
+    if (!p->has_scaling) return fail(GLU_ERR_ARGUMENT, "plan has no permutation/scaling (create it from an analysis)");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const SolveDev &S = p->S;
+    if (!transpose) {
+        launch::perm_scale_f32(d_b, p->d_t32, S.col_perm, S.col_scale_perm, 0, p->n, st);
+        run_sweeps(p, d_lu, p->d_t32, 0, st);
+        launch::perm_scale_f32(p->d_t32, d_x, S.row_perm_inv, S.row_scale, 0, p->n, st);
+    } else {
+        launch::perm_scale_f32(d_b, p->d_t32, S.row_perm, S.row_scale, 1, p->n, st);
+        run_sweeps(p, d_lu, p->d_t32, 1, st);
+        launch::perm_scale_f32(p->d_t32, d_x, S.col_perm_inv, S.col_scale_perm, 1, p->n, st);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(GLU_ERR_DEVICE, std::string("solve launch: ") + hipGetErrorString(e));
+    return GLU_OK;
+}
+
+extern "C" glu_status glu_refresh(glu_plan *p, const double *d_ax, float *d_lu, int batch, void *stream)
+{
+    if (!p || !d_ax || !d_lu || batch <= 0) return fail(GLU_ERR_ARGUMENT, "bad argument");
+    if (!p->has_refine) return fail(GLU_ERR_ARGUMENT, "plan has no analysis");
+    launch::refresh(p->R, p->F.col_ptr, d_ax, d_lu, p->n, p->nnz, batch, static_cast<hipStream_t>(stream));
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(GLU_ERR_DEVICE, std::string("refresh launch: ") + hipGetErrorString(e));
+    return GLU_OK;
+}
+
+static double bits_to_double(unsigned long long b)
+{
+    double d;
+    std::memcpy(&d, &b, sizeof d);
+    return d;
+}
+
+extern "C" glu_status glu_refine(glu_plan *p, const double *d_ax, const float *d_lu, const double *d_b,
+                                 double *d_x, double tol, int maxit, int transpose, int *h_iters, double *h_eta,
+                                 void *stream)
+{
+    if (!p || !d_ax || !d_lu || !d_b || !d_x || maxit <= 0) return fail(GLU_ERR_ARGUMENT, "bad argument");
+    if (!p->has_refine) return fail(GLU_ERR_ARGUMENT, "plan has no analysis");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const SolveDev &S = p->S;
+    const uint32_t n = p->n;
+    HIP_TRY(hipMemsetAsync(d_x, 0, sizeof(double) * n, st));
+    HIP_TRY(hipMemcpyAsync(p->d_r64, d_b, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    double anorm = 0.0, bnorm = 0.0, eta = 0.0;
+    int it = 0;
+    for (;;) {
+        if (!transpose)
+            launch::perm_scale_f64(p->d_r64, p->d_t32, S.col_perm, S.col_scale_perm, 0, n, st);
+        else
+            launch::perm_scale_f64(p->d_r64, p->d_t32, S.row_perm, S.row_scale, 1, n, st);
+        run_sweeps(p, d_lu, p->d_t32, transpose, st);
+        if (!transpose)
+            launch::update_x(p->d_t32, d_x, S.row_perm_inv, S.row_scale, 0, n, st);
+        else
+            launch::update_x(p->d_t32, d_x, S.col_perm_inv, S.col_scale_perm, 1, n, st);
+        launch::zero_u64(p->d_norms, 4, st);
+        launch::residual(p->R, d_ax, d_b, d_x, p->d_r64, n, transpose, it == 0, p->d_norms, st);
+        unsigned long long nb[4];
+        HIP_TRY(hipMemcpyAsync(nb, p->d_norms, sizeof nb, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        if (it == 0) {
+            anorm = bits_to_double(nb[2]);
+            bnorm = bits_to_double(nb[3]);
+        }
+        const double rn = bits_to_double(nb[0]), xn = bits_to_double(nb[1]);
+        eta = rn / (anorm * xn + bnorm);
+        ++it;
+        if (!(eta > tol) || it >= maxit) break;
+    }
+    if (h_iters) *h_iters = it;
+    if (h_eta) *h_eta = eta;
+    if (!(eta == eta)) return fail(GLU_ERR_NUMERIC, "refinement produced NaN");
+    return GLU_OK;
+}
+
+extern "C" glu_status glu_batch_solve(glu_plan *p, int batch, const double *d_ax, const double *d_b, double *d_x,
+                                      float *d_lu, double tol, int maxit, int32_t *d_iters, double *d_eta,
+                                      unsigned flags, void *stream)
+{
+    if (!p || batch <= 0 || !d_ax || !d_b || !d_x || !d_lu || !d_iters || !d_eta || maxit <= 0)
+        return fail(GLU_ERR_ARGUMENT, "bad argument");
+    if (!p->has_refine) return fail(GLU_ERR_ARGUMENT, "plan has no analysis");
+    glu_status s = ensure_batch(p, batch);
+    if (s != GLU_OK) return s;
+    launch::batch_same(p->d_desc, batch, d_ax, d_b, d_x, d_lu, p->d_bt32, p->d_br64, d_iters, d_eta, tol, maxit,
+                       (int)(flags & 1u), p->fused_threads, static_cast<hipStream_t>(stream));
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(GLU_ERR_DEVICE, std::string("batch launch: ") + hipGetErrorString(e));
+    return GLU_OK;
+}
+
+// ---------------------------------------------------------------------------
+// uneven groups
+// ---------------------------------------------------------------------------
+
+struct MatIOHost {            // mirrors the device-side MatIO layout
+    const double *ax;
+    const double *b;
+    double *x;
+    float *lu;
+    float *t32;
+    double *r64;
+    int32_t *iters;
+    double *eta;
+};
+
+struct glu_group {
+    std::vector<glu_plan *> plans;
+    MatDesc *d_descs = nullptr;
+    MatIOHost *d_ios = nullptr;
+    uint32_t *d_order = nullptr;
+    float *d_t32 = nullptr;
+    double *d_r64 = nullptr;
+    std::vector<size_t> voff;   // per-matrix offset into the scratch vectors
+    int threads = kDefaultFusedThreads;
+    ~glu_group()
+    {
+        if (d_descs) (void)hipFree(d_descs);
+        if (d_ios) (void)hipFree(d_ios);
+        if (d_order) (void)hipFree(d_order);
+        if (d_t32) (void)hipFree(d_t32);
+        if (d_r64) (void)hipFree(d_r64);
+    }
+};
+
+extern "C" glu_status glu_group_create(glu_plan *const *plans, int count, glu_group **out)
+{
+    if (!plans || count <= 0 || !out) return fail(GLU_ERR_ARGUMENT, "bad argument");
+    *out = nullptr;
+    if (launch::matio_size() != sizeof(MatIOHost)) return fail(GLU_ERR_GENERAL, "MatIO layout mismatch");
+    std::unique_ptr<glu_group> g(new glu_group());
+    std::vector<MatDesc> descs(count);
+    std::vector<std::pair<int64_t, uint32_t>> work(count);
+    size_t vtot = 0;
+    for (int i = 0; i < count; ++i) {
+        glu_plan *p = plans[i];
+        if (!p || !p->has_refine) return fail(GLU_ERR_ARGUMENT, "every plan must come from an analysis");
+        g->plans.push_back(p);
+        descs[i].f = p->F;
+        descs[i].s = p->S;
+        descs[i].r = p->R;
+        descs[i].n = p->n;
+        descs[i].nnz = p->nnz;
+        g->voff.push_back(vtot);
+        vtot += p->n;
+        g->threads = std::max(g->threads, p->fused_threads);
+        // heaviest first: levels dominate the latency, updates the work
+        work[i] = {-(int64_t)(p->num_lev + p->tri[0].lvl_ptr.size() + p->tri[1].lvl_ptr.size()) * 64 - p->updates,
+                   (uint32_t)i};
+    }
+    std::sort(work.begin(), work.end());
+    std::vector<uint32_t> order(count);
+    for (int i = 0; i < count; ++i) order[i] = work[i].second;
+    HIP_TRY(hipMalloc(&g->d_descs, sizeof(MatDesc) * count));
+    HIP_TRY(hipMalloc(&g->d_ios, sizeof(MatIOHost) * count));
+    HIP_TRY(hipMalloc(&g->d_order, sizeof(uint32_t) * count));
+    HIP_TRY(hipMalloc(&g->d_t32, sizeof(float) * vtot));
+    HIP_TRY(hipMalloc(&g->d_r64, sizeof(double) * vtot));
+    HIP_TRY(hipMemcpy(g->d_descs, descs.data(), sizeof(MatDesc) * count, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(g->d_order, order.data(), sizeof(uint32_t) * count, hipMemcpyHostToDevice));
+    *out = g.release();
+    return GLU_OK;
+}
+
+extern "C" void glu_group_destroy(glu_group *g) { delete g; }
+
+extern "C" glu_status glu_group_solve(glu_group *g, const double *const *d_ax, const double *const *d_b,
+                                      double *const *d_x, float *const *d_lu, double tol, int maxit,
+                                      int32_t *d_iters, double *d_eta, unsigned flags, void *stream)
+{
+    if (!g || !d_ax || !d_b || !d_x || !d_lu || !d_iters || !d_eta || maxit <= 0)
+        return fail(GLU_ERR_ARGUMENT, "bad argument");
+    const int count = (int)g->plans.size();
+    std::vector<MatIOHost> ios(count);
+    for (int i = 0; i < count; ++i) {
+        if (!d_ax[i] || !d_b[i] || !d_x[i] || !d_lu[i]) return fail(GLU_ERR_ARGUMENT, "null matrix pointer");
+        ios[i] = MatIOHost{d_ax[i], d_b[i], d_x[i], d_lu[i], g->d_t32 + g->voff[i], g->d_r64 + g->voff[i],
+                           d_iters + i, d_eta + i};
+    }
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    HIP_TRY(hipMemcpyAsync(g->d_ios, ios.data(), sizeof(MatIOHost) * count, hipMemcpyHostToDevice, st));
+    launch::batch_group(g->d_descs, g->d_ios, g->d_order, count, tol, maxit, (int)(flags & 1u), g->threads, st);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(GLU_ERR_DEVICE, std::string("group launch: ") + hipGetErrorString(e));
+    // ios is read by the kernel from device memory; the host copy may go.
+    HIP_TRY(hipStreamSynchronize(st));
+    return GLU_OK;
+}

@@ -1,0 +1,185 @@
+/*
+ * glu_hip.h — C-ABI of the MI355X (gfx950) numeric phase for GLU 3.0.
+ *
+ * Plain C types only (no HIP, torch or C++ types in the signatures); device
+ * buffers are passed as plain pointers obtained from hipMalloc (or any
+ * allocator of device memory), streams as an opaque `void *` holding a
+ * hipStream_t (NULL = the null stream).  Every entry point returns a
+ * glu_status; glu_last_error() gives the thread's last error text.
+ *
+ * Reference interfaces replaced (paths relative to the reference tree):
+ *   preprocess_from_arrays + Symbolic_Matrix::{fill_in,csr,predictLU,leveling}
+ *       src/preprocess/preprocess_arrays.c:18, src/symbolic.cc:11-151,
+ *       as called in sequence by src/glu_binding.cpp:87-109  -> glu_analyze
+ *   LUonDevice(Symbolic_Matrix&, ostream&, ostream&, bool PERTURB)
+ *       include/numeric.h:6, src/numeric.cu:347-639
+ *       -> glu_plan_create + glu_factor_host (+ glu_plan_destroy)
+ *   Symbolic_Matrix::solve(SNicsLU*, const vector<REAL>&)
+ *       src/symbolic.cc:200-251                             -> glu_solve
+ *   GLUFactorization(data, indices, indptr, n, perturb) / .solve(b)
+ *       src/glu_binding.cpp:44-155 (pybind11 module _pyglu) -> glu_analyze +
+ *       glu_plan_create_from_analysis + glu_factor + glu_solve
+ *   No counterpart in the reference (the refactor semantics follow NICSLU's
+ *   _I_NicsLU_MC64ScaleForRefact, src/nicslu/source/nicslu_mc64scale.c:58-85):
+ *       glu_refresh, glu_refine, glu_batch_solve, glu_group_*.
+ */
+#ifndef GLU_HIP_H
+#define GLU_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum glu_status {
+    GLU_OK = 0,
+    GLU_ERR_GENERAL = -1,
+    GLU_ERR_ARGUMENT = -2,      /* bad pointer / size / option */
+    GLU_ERR_MEMORY = -3,        /* host or device allocation failed */
+    GLU_ERR_DEVICE = -4,        /* HIP runtime error or no device */
+    GLU_ERR_STRUCT_SINGULAR = -5, /* MC64 found no perfect matching (nicslu.h:16) */
+    GLU_ERR_NUMERIC = -6,       /* non-finite factors / no convergence */
+    GLU_ERR_INVALID = -7,       /* malformed CSC (nicslu.h:18) */
+    GLU_ERR_DUPLICATE = -8,     /* duplicated entry (nicslu.h:19) */
+    GLU_ERR_EMPTY = -9          /* n == 0 or nnz == 0 (numeric.cu:349-352) */
+} glu_status;
+
+/* Thread-local text of the last error on this thread ("" if none). */
+const char *glu_last_error(void);
+
+/* ------------------------------------------------------------------------
+ * Host analysis: MC64 + AMD + permute/scale (NICSLU NicsLU_Analyze and
+ * my_DumpA), then fill-in, CSR, predictLU and leveling.  Bit-exact with the
+ * reference on the same input.  `ax`/`ai`/`ap` are the caller's CSC arrays
+ * (ap[n] == nnz); they are copied.
+ * ---------------------------------------------------------------------- */
+typedef struct glu_analysis glu_analysis;
+
+glu_status glu_analyze(uint32_t n, uint32_t nnz, const double *ax, const uint32_t *ai,
+                       const uint32_t *ap, glu_analysis **out);
+void glu_analysis_destroy(glu_analysis *a);
+
+/* info[0]=n info[1]=nnz(A) info[2]=nnz(L+U) info[3]=num_lev
+ * info[4]=scalar updates of one factorization  info[5]=L-solve levels
+ * info[6]=U-solve levels  info[7]=reserved */
+glu_status glu_analysis_info(const glu_analysis *a, int64_t info[8]);
+
+/* Copy one array of the analysis into `dst` (`bytes` must equal its size). */
+typedef enum glu_field {
+    GLU_F_SYM_C_PTR = 0,   /* u32[n+1] */
+    GLU_F_SYM_R_IDX,       /* u32[nnz_lu] */
+    GLU_F_L_COL_PTR,       /* u32[n]  (diagonal positions) */
+    GLU_F_CSR_R_PTR,       /* u32[n+1] */
+    GLU_F_CSR_C_IDX,       /* u32[nnz_lu] */
+    GLU_F_CSR_DIAG_PTR,    /* u32[n] */
+    GLU_F_VAL,             /* f32[nnz_lu]  predictLU values */
+    GLU_F_LEVEL_PTR,       /* i32[num_lev+1] */
+    GLU_F_LEVEL_IDX,       /* i32[n] */
+    GLU_F_ROW_PERM,        /* u32[n] */
+    GLU_F_ROW_PERM_INV,    /* u32[n] */
+    GLU_F_COL_PERM,        /* u32[n] */
+    GLU_F_COL_PERM_INV,    /* u32[n] */
+    GLU_F_ROW_SCALE,       /* f64[n] */
+    GLU_F_COL_SCALE_PERM,  /* f64[n] */
+    GLU_F_DUMP_AX,         /* f64[nnz]  permuted + scaled A (my_DumpA) */
+    GLU_F_DUMP_AI,         /* u32[nnz] */
+    GLU_F_DUMP_AP          /* u32[n+1] */
+} glu_field;
+
+glu_status glu_analysis_get(const glu_analysis *a, glu_field f, void *dst, size_t bytes);
+
+/* ------------------------------------------------------------------------
+ * Device plan: the pattern, levels, update maps and solve schedules,
+ * uploaded once and reused by every factorization/solve.
+ * ---------------------------------------------------------------------- */
+typedef struct glu_plan glu_plan;
+
+/* The Symbolic_Matrix fields LUonDevice reads (include/symbolic.h:12-23). */
+typedef struct glu_pattern {
+    uint32_t n, nnz;
+    int32_t num_lev;
+    const uint32_t *sym_c_ptr, *sym_r_idx, *l_col_ptr;
+    const uint32_t *csr_r_ptr, *csr_c_idx, *csr_diag_ptr;
+    const int32_t *level_ptr, *level_idx;
+} glu_pattern;
+
+typedef struct glu_plan_opts {
+    int32_t fuse_units;     /* level fusion: levels with <= this many update
+                               units run inside one workgroup; 0 = default */
+    int32_t block_threads;  /* threads per workgroup of the fused and batched
+                               kernels (64..1024, multiple of 64); 0 = default */
+    int32_t use_graph;      /* 1 = replay the level schedule from a hipGraph */
+    int32_t reserved[5];
+} glu_plan_opts;
+
+glu_status glu_plan_create(const glu_pattern *pat, const glu_plan_opts *opts, glu_plan **out);
+/* As glu_plan_create, plus the permutation/scaling needed by glu_solve and
+ * the original-A structure used by glu_refresh/glu_refine/glu_batch_solve. */
+glu_status glu_plan_create_from_analysis(const glu_analysis *a, const glu_plan_opts *opts,
+                                         glu_plan **out);
+void glu_plan_destroy(glu_plan *p);
+
+/* info[0]=n info[1]=nnz_lu info[2]=nnz_a info[3]=num_lev info[4]=updates
+ * info[5]=segments in the factor schedule info[6]=device bytes held
+ * info[7]=units */
+glu_status glu_plan_info(const glu_plan *p, int64_t info[8]);
+
+/* ------------------------------------------------------------------------
+ * Numeric factorization (FP32, in place, L unit-lower strictly below the
+ * diagonal, U including the diagonal — the layout of Symbolic_Matrix::val).
+ * perturb != 0: GESP as numeric.cu:490-506 and :139-140 — pert =
+ * 3.45e-4 * max column sum of |val| (FP32, before factorization); a pivot
+ * with |d| < pert becomes +pert, only in columns whose L part is non-empty.
+ * ---------------------------------------------------------------------- */
+glu_status glu_factor(glu_plan *p, float *d_val, int perturb, void *stream);
+/* Host buffer in/out, synchronous: the LUonDevice contract. */
+glu_status glu_factor_host(glu_plan *p, float *h_val, int perturb);
+
+/* FP32 solve with the factors `d_lu`, Symbolic_Matrix::solve semantics
+ * (permute+scale rhs, unit-L forward, U backward, unpermute+scale), bit-exact
+ * with the reference's host solve.  transpose != 0 solves A^T x = b.
+ * Needs a plan created from an analysis.  d_b may equal d_x. */
+glu_status glu_solve(glu_plan *p, const float *d_lu, const float *d_b, float *d_x,
+                     int transpose, void *stream);
+
+/* Scale + place `batch` FP64 value sets of A (caller CSC order, nnz(A) each)
+ * into the LU pattern with the frozen MC64 scales and permutation
+ * ((ax * rs) * cs, nicslu_mc64scale.c:82), cast to FP32, fill entries 0.
+ * d_lu receives batch * nnz_lu floats. */
+glu_status glu_refresh(glu_plan *p, const double *d_ax, float *d_lu, int batch, void *stream);
+
+/* Iterative refinement of A x = b (or A^T x = b): FP32 solves with d_lu,
+ * FP64 residual r = b - A x on the original A values d_ax and FP64 update,
+ * until eta = |r|_inf / (|A|_inf |x|_inf + |b|_inf) <= tol or maxit.
+ * x starts from 0.  h_iters/h_eta (host, may be NULL) receive the result. */
+glu_status glu_refine(glu_plan *p, const double *d_ax, const float *d_lu, const double *d_b,
+                      double *d_x, double tol, int maxit, int transpose, int *h_iters,
+                      double *h_eta, void *stream);
+
+/* Same-pattern batch, one persistent launch: for every b < batch,
+ * refresh the values d_ax + b*nnz_a into d_lu + b*nnz_lu, factor, solve and
+ * refine d_b + b*n into d_x + b*n.  d_iters[b], d_eta[b] (device) receive the
+ * per-matrix iteration count and backward error.  flags bit0: perturb. */
+glu_status glu_batch_solve(glu_plan *p, int batch, const double *d_ax, const double *d_b,
+                           double *d_x, float *d_lu, double tol, int maxit, int32_t *d_iters,
+                           double *d_eta, unsigned flags, void *stream);
+
+/* ------------------------------------------------------------------------
+ * Uneven batch: distinct patterns refactored + refined in one fused launch.
+ * ---------------------------------------------------------------------- */
+typedef struct glu_group glu_group;
+
+glu_status glu_group_create(glu_plan *const *plans, int count, glu_group **out);
+void glu_group_destroy(glu_group *g);
+/* Host arrays of `count` device pointers each, in plan order. */
+glu_status glu_group_solve(glu_group *g, const double *const *d_ax, const double *const *d_b,
+                           double *const *d_x, float *const *d_lu, double tol, int maxit,
+                           int32_t *d_iters, double *d_eta, unsigned flags, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GLU_HIP_H */
