@@ -26,6 +26,18 @@ struct Unit {          // one wave's share of a column update
     uint32_t pad;
 };
 
+// An entry updated by several pivot columns of one level: one lane owns it
+// and applies the updates in increasing column order (deterministic).
+struct MergeSrc {
+    uint32_t lpos;     // CSC position of L(i,k) (not yet divided)
+    uint32_t upos;     // CSC position of U(k,j)
+    uint32_t dpos;     // CSC position of the pivot U(k,k)
+    uint32_t pad;
+};
+
+// Scatter-map entries whose target is owned by a merge lane carry this bit.
+constexpr uint32_t kMergedBit = 0x80000000u;
+
 struct FactorDev {
     const uint32_t *col_ptr;   // sym_c_ptr [n+1]
     const uint32_t *diag;      // l_col_ptr [n]
@@ -38,6 +50,10 @@ struct FactorDev {
     const uint32_t *unit_ptr;  // per level [num_lev + 2]
     const uint32_t *dcols;     // columns with |L_k| > 0, by level
     const uint32_t *dcol_ptr;  // per level [num_lev + 1]
+    const uint32_t *ment;      // merge entries (target CSC positions), by level
+    const uint32_t *ment_ptr;  // per level [num_lev + 2]
+    const uint32_t *msrc_ptr;  // per merge entry [entries + 1]
+    const MergeSrc *msrc;      // sources in increasing column order
     int32_t num_lev;
     uint32_t n, nnz;
 };

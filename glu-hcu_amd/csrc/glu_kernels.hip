@@ -12,8 +12,12 @@
 // of a dense n-length scratch column; the L(:,k) /= U(k,k) write-back of a
 // level is deferred to the next level's launch so a level costs one barrier;
 // updates are fire-and-forget FP32 atomics (the reference's own update
-// operator, val += -(l*u)); solves pull along CSR/CSC rows so each unknown
-// is owned by one lane and accumulates in the reference's order.
+// operator, val += -(l*u)) where an entry gets one update per level, and
+// lane-owned sequential merges where several columns of a level update the
+// same entry, so the factors are deterministic and equal a sequential pass
+// over each level's columns in increasing order; solves pull along CSR/CSC
+// rows so each unknown is owned by one lane and accumulates in the
+// reference's order.
 //
 // Build flags (see the package Makefile): -ffp-contract=off so products are
 // rounded before they are subtracted, as the reference's FP32 kernels and
@@ -77,6 +81,10 @@ __device__ double wg_max(double v, double *sh)
 // One work unit: pivot column k, U_k entries [jb, jb + jc).
 //   l_i = A(i,k) / d          (d = A(k,k), GESP-perturbed if requested)
 //   A(i,j) += -(l_i * A(k,j)) for i in L_k, j in the slice of U_k
+// except targets that another column of the same level also updates; those
+// belong to a merge lane (factor_merge).  Every remaining target receives one
+// update per level, so the atomic is a plain fire-and-forget add and the
+// result does not depend on the order waves run in.
 // Lanes are split into groups of G = pow2 >= |L_k| (|L_k| <= 64), each group
 // owning one j at a time; longer columns loop over L_k in strides of 64.
 template <int Scope>
@@ -101,18 +109,43 @@ __device__ __forceinline__ void factor_unit(const FactorDev &F, float *__restric
         if (act) l = val[dpos + 1 + ii] / d;
         for (uint32_t jj = g; jj < u.jc; jj += step) {
             const float uv = val[up[jj]];
-            if (act) atomic_sub<Scope>(val + map[(uint64_t)jj * L + ii], l * uv);
+            if (act) {
+                const uint32_t t = map[(uint64_t)jj * L + ii];
+                if (!(t & kMergedBit)) atomic_sub<Scope>(val + t, l * uv);
+            }
         }
     } else {
         for (uint32_t jj = 0; jj < u.jc; ++jj) {
             const float uv = val[up[jj]];
             const uint32_t *__restrict__ mrow = map + (uint64_t)jj * L;
             for (uint32_t ii = (uint32_t)lane; ii < L; ii += kWave) {
+                const uint32_t t = mrow[ii];
+                if (t & kMergedBit) continue;
                 const float l = val[dpos + 1 + ii] / d;
-                atomic_sub<Scope>(val + mrow[ii], l * uv);
+                atomic_sub<Scope>(val + t, l * uv);
             }
         }
     }
+}
+
+// Merge entries e0 + lane: apply every source of this level in increasing
+// column order, t += -(l * u) with l = L(i,k) / U(k,k) — the order in which
+// a sequential pass over the level's columns would apply them.
+__device__ __forceinline__ void factor_merge(const FactorDev &F, float *__restrict__ val, uint32_t e,
+                                             uint32_t e_end, float pert, int perturb)
+{
+    if (e >= e_end) return;
+    const uint32_t t = F.ment[e];
+    float v = val[t];
+    const uint32_t s1 = F.msrc_ptr[e + 1];
+    for (uint32_t s = F.msrc_ptr[e]; s < s1; ++s) {
+        const MergeSrc m = F.msrc[s];
+        float d = val[m.dpos];
+        if (perturb && fabsf(d) < pert) d = pert;
+        const float l = val[m.lpos] / d;
+        v = v + (-(l * val[m.upos]));
+    }
+    val[t] = v;
 }
 
 // Deferred write-back of column k: L(:,k) /= d, and d := pert when GESP fired.
@@ -128,15 +161,21 @@ __device__ __forceinline__ void factor_div(const FactorDev &F, float *__restrict
     if (pt && lane == 0) val[dpos] = pert;
 }
 
+__device__ __forceinline__ uint32_t merge_waves(const FactorDev &F, int lvl)
+{
+    return (F.ment_ptr[lvl + 1] - F.ment_ptr[lvl] + kWave - 1) / kWave;
+}
+
 __device__ __forceinline__ uint32_t level_items(const FactorDev &F, int lvl)
 {
-    uint32_t items = F.unit_ptr[lvl + 1] - F.unit_ptr[lvl];
+    uint32_t items = F.unit_ptr[lvl + 1] - F.unit_ptr[lvl] + merge_waves(F, lvl);
     if (lvl > 0) items += F.dcol_ptr[lvl] - F.dcol_ptr[lvl - 1];
     return items;
 }
 
-// Item t of level lvl: update units of lvl first, then the write-backs of
-// the columns of level lvl-1.  lvl == num_lev has write-backs only.
+// Item t of level lvl: update units of lvl, then merge waves (64 entries
+// each), then the write-backs of the columns of level lvl-1.  lvl == num_lev
+// has write-backs only.
 template <int Scope>
 __device__ __forceinline__ void factor_item(const FactorDev &F, float *__restrict__ val, int lvl,
                                             uint32_t item, int lane, float pert, int perturb)
@@ -148,6 +187,12 @@ __device__ __forceinline__ void factor_item(const FactorDev &F, float *__restric
         return;
     }
     item -= nu;
+    const uint32_t nm = merge_waves(F, lvl);
+    if (item < nm) {
+        factor_merge(F, val, F.ment_ptr[lvl] + item * kWave + (uint32_t)lane, F.ment_ptr[lvl + 1], pert, perturb);
+        return;
+    }
+    item -= nm;
     if (lvl > 0) {
         const uint32_t d0 = F.dcol_ptr[lvl - 1];
         if (item < F.dcol_ptr[lvl] - d0) factor_div(F, val, F.dcols[d0 + item], lane, pert, perturb);
@@ -347,6 +392,16 @@ __global__ void k_residual(RefineDev R, const double *__restrict__ ax, const dou
             umax64(norms + 3, bmax);
         }
     }
+}
+
+// grad[k] = -lam[row(k)] * x[col(k)] over the caller's CSC entries.
+__global__ void k_adjoint_values(RefineDev R, const double *__restrict__ lam, const double *__restrict__ x,
+                                 double *__restrict__ grad, uint32_t n)
+{
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const double xc = x[c];
+    for (uint32_t k = R.a_cptr[c]; k < R.a_cptr[c + 1]; ++k) grad[k] = -(lam[R.a_ridx[k]] * xc);
 }
 
 // ---------------------------------------------------------------------------
@@ -561,6 +616,12 @@ void batch_group(const MatDesc *descs, const void *ios, const uint32_t *order, i
 {
     k_batch_group<<<count, threads, 0, st>>>(descs, static_cast<const MatIO *>(ios), order, tol, maxit,
                                             perturb);
+}
+
+void adjoint_values(const RefineDev &R, const double *lam, const double *x, double *grad, uint32_t n,
+                    hipStream_t st)
+{
+    k_adjoint_values<<<blocks_for(n, 256), 256, 0, st>>>(R, lam, x, grad, n);
 }
 
 void zero_u64(unsigned long long *p, int count, hipStream_t st)

@@ -48,6 +48,8 @@ void batch_same(const MatDesc *desc, int batch, const double *ax, const double *
 void batch_group(const MatDesc *descs, const void *ios, const uint32_t *order, int count, double tol,
                  int maxit, int perturb, int threads, hipStream_t st);
 void zero_u64(unsigned long long *p, int count, hipStream_t st);
+void adjoint_values(const RefineDev &R, const double *lam, const double *x, double *grad, uint32_t n,
+                    hipStream_t st);
 size_t matio_size();
 }  // namespace launch
 }  // namespace glu
@@ -293,6 +295,10 @@ bool validate_pattern(const glu_pattern &P, std::string &err)
         err = "null pattern array";
         return false;
     }
+    if (nnz >= kMergedBit) {
+        err = "nnz(L+U) must be below 2^31";
+        return false;
+    }
     if (P.num_lev <= 0 || (uint32_t)P.num_lev > n) {
         err = "num_lev out of range";
         return false;
@@ -375,7 +381,70 @@ struct Tables {
     std::vector<uint64_t> moff;
     std::vector<Unit> units;
     std::vector<uint32_t> unit_ptr, dcols, dcol_ptr;
+    std::vector<uint32_t> ment, ment_ptr, msrc_ptr;
+    std::vector<MergeSrc> msrc;
 };
+
+// Entries that two or more columns of one level update become merge
+// entries: their sources are listed in increasing column order (the order
+// level_idx holds them in) and their scatter-map slots are flagged so the
+// work units skip them.
+void build_merges(const glu_pattern &P, Tables &T)
+{
+    const uint32_t n = P.n, nnz = P.nnz;
+    const int nl = P.num_lev;
+    std::vector<uint32_t> stamp(nnz, UINT32_MAX), cnt(nnz, 0), slot(nnz, 0);
+    T.ment_ptr.assign(nl + 2, 0);
+    T.msrc_ptr.assign(1, 0);
+    std::vector<uint32_t> touched;
+    for (int l = 0; l < nl; ++l) {
+        T.ment_ptr[l] = (uint32_t)T.ment.size();
+        touched.clear();
+        auto each_update = [&](auto f) {
+            for (int t = P.level_ptr[l]; t < P.level_ptr[l + 1]; ++t) {
+                const uint32_t k = (uint32_t)P.level_idx[t];
+                const uint32_t L = T.lcnt[k];
+                if (L == 0) continue;
+                const uint32_t U = P.csr_r_ptr[k + 1] - T.ufirst[k];
+                for (uint32_t jj = 0; jj < U; ++jj)
+                    for (uint32_t ii = 0; ii < L; ++ii) f(k, jj, ii, T.moff[k] + (uint64_t)jj * L + ii);
+            }
+        };
+        each_update([&](uint32_t, uint32_t, uint32_t, uint64_t m) {
+            const uint32_t tgt = T.umap[m];
+            if (stamp[tgt] != (uint32_t)l) {
+                stamp[tgt] = (uint32_t)l;
+                cnt[tgt] = 0;
+                touched.push_back(tgt);
+            }
+            ++cnt[tgt];
+        });
+        std::sort(touched.begin(), touched.end());
+        for (uint32_t tgt : touched) {
+            if (cnt[tgt] < 2) continue;
+            slot[tgt] = (uint32_t)T.ment.size();
+            T.ment.push_back(tgt);
+        }
+        const uint32_t first = T.ment_ptr[l];
+        const uint32_t count = (uint32_t)T.ment.size() - first;
+        // per-entry source ranges, filled in column order
+        std::vector<uint32_t> fillpos(count);
+        for (uint32_t e = 0; e < count; ++e) {
+            T.msrc_ptr.push_back(T.msrc_ptr.back() + cnt[T.ment[first + e]]);
+            fillpos[e] = T.msrc_ptr[first + e];
+        }
+        T.msrc.resize(T.msrc_ptr.back());
+        each_update([&](uint32_t k, uint32_t jj, uint32_t ii, uint64_t m) {
+            const uint32_t tgt = T.umap[m] & ~kMergedBit;
+            if (cnt[tgt] < 2) return;
+            const uint32_t dpos = P.l_col_ptr[k];
+            T.msrc[fillpos[slot[tgt] - first]++] = MergeSrc{dpos + 1 + ii, T.upos[T.ufirst[k] + jj], dpos, 0};
+            T.umap[m] |= kMergedBit;
+        });
+        (void)n;
+    }
+    T.ment_ptr[nl] = T.ment_ptr[nl + 1] = (uint32_t)T.ment.size();
+}
 
 bool build_factor_tables(const glu_pattern &P, Tables &T, int64_t &updates, std::string &err)
 {
@@ -455,6 +524,7 @@ bool build_factor_tables(const glu_pattern &P, Tables &T, int64_t &updates, std:
     }
     T.unit_ptr[nl] = T.unit_ptr[nl + 1] = (uint32_t)T.units.size();
     T.dcol_ptr[nl] = (uint32_t)T.dcols.size();
+    build_merges(P, T);
     return true;
 }
 
@@ -504,6 +574,7 @@ void factor_segments(glu_plan &p, const Tables &T)
     std::vector<uint32_t> work(p.num_lev + 1);
     for (int l = 0; l <= p.num_lev; ++l) {
         uint32_t items = T.unit_ptr[l + 1] - T.unit_ptr[l];
+        items += (T.ment_ptr[l + 1] - T.ment_ptr[l] + 63) / 64;
         if (l > 0) items += T.dcol_ptr[l] - T.dcol_ptr[l - 1];
         work[l] = items;
     }
@@ -559,10 +630,6 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
         if (opts->fuse_units < 0) return fail(GLU_ERR_ARGUMENT, "fuse_units must be >= 0");
         if (opts->fuse_units) p->fuse_units = opts->fuse_units;
     }
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(GLU_ERR_DEVICE, "no HIP device");
-    HIP_TRY(hipGetDevice(&p->device));
-
     p->n = P.n;
     p->nnz = P.nnz;
     p->num_lev = P.num_lev;
@@ -573,6 +640,9 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
     } catch (const std::bad_alloc &) {
         return fail(GLU_ERR_MEMORY, "out of host memory building tables");
     }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(GLU_ERR_DEVICE, "no HIP device");
+    HIP_TRY(hipGetDevice(&p->device));
     factor_segments(*p, T);
     p->units = (int64_t)T.units.size();
 
@@ -640,6 +710,10 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
     const size_t o_uptr = U.put(T.unit_ptr);
     const size_t o_dcols = U.put(T.dcols);
     const size_t o_dptr = U.put(T.dcol_ptr);
+    const size_t o_ment = U.put(T.ment);
+    const size_t o_mptr = U.put(T.ment_ptr);
+    const size_t o_msptr = U.put(T.msrc_ptr);
+    const size_t o_msrc = U.put(T.msrc);
     size_t o_trows[4], o_tptr[4];
     for (int s = 0; s < 4; ++s) {
         o_trows[s] = U.put(p->tri[s].rows);
@@ -689,6 +763,10 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
     F.unit_ptr = u32p(o_uptr);
     F.dcols = u32p(o_dcols);
     F.dcol_ptr = u32p(o_dptr);
+    F.ment = u32p(o_ment);
+    F.ment_ptr = u32p(o_mptr);
+    F.msrc_ptr = u32p(o_msptr);
+    F.msrc = reinterpret_cast<const MergeSrc *>(A + o_msrc);
     F.num_lev = P.num_lev;
     F.n = n;
     F.nnz = P.nnz;
@@ -954,6 +1032,17 @@ extern "C" glu_status glu_refine(glu_plan *p, const double *d_ax, const float *d
     if (h_iters) *h_iters = it;
     if (h_eta) *h_eta = eta;
     if (!(eta == eta)) return fail(GLU_ERR_NUMERIC, "refinement produced NaN");
+    return GLU_OK;
+}
+
+extern "C" glu_status glu_adjoint_values(glu_plan *p, const double *d_lam, const double *d_x, double *d_grad_ax,
+                                         void *stream)
+{
+    if (!p || !d_lam || !d_x || !d_grad_ax) return fail(GLU_ERR_ARGUMENT, "null argument");
+    if (!p->has_refine) return fail(GLU_ERR_ARGUMENT, "plan has no analysis");
+    launch::adjoint_values(p->R, d_lam, d_x, d_grad_ax, p->n, static_cast<hipStream_t>(stream));
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(GLU_ERR_DEVICE, std::string("adjoint launch: ") + hipGetErrorString(e));
     return GLU_OK;
 }
 

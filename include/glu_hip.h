@@ -21,7 +21,8 @@
  *       glu_plan_create_from_analysis + glu_factor + glu_solve
  *   No counterpart in the reference (the refactor semantics follow NICSLU's
  *   _I_NicsLU_MC64ScaleForRefact, src/nicslu/source/nicslu_mc64scale.c:58-85):
- *       glu_refresh, glu_refine, glu_batch_solve, glu_group_*.
+ *       glu_refresh, glu_refine, glu_adjoint_values, glu_batch_solve,
+ *       glu_group_*.
  */
 #ifndef GLU_HIP_H
 #define GLU_HIP_H
@@ -157,6 +158,12 @@ glu_status glu_refresh(glu_plan *p, const double *d_ax, float *d_lu, int batch, 
 glu_status glu_refine(glu_plan *p, const double *d_ax, const float *d_lu, const double *d_b,
                       double *d_x, double tol, int maxit, int transpose, int *h_iters,
                       double *h_eta, void *stream);
+
+/* Adjoint of x = A^-1 b with respect to the stored values of A:
+ * grad_ax[k] = -lam[row(k)] * x[col(k)] for every entry k of the caller's
+ * CSC (lam = A^-T g).  All pointers are device buffers. */
+glu_status glu_adjoint_values(glu_plan *p, const double *d_lam, const double *d_x, double *d_grad_ax,
+                              void *stream);
 
 /* Same-pattern batch, one persistent launch: for every b < batch,
  * refresh the values d_ax + b*nnz_a into d_lu + b*nnz_lu, factor, solve and

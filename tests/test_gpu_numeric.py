@@ -1,0 +1,222 @@
+"""GPU parity of the numeric phase against the oracle (C restatement of the
+reference's FP32 arithmetic) — factorization, GESP, solves, refinement."""
+
+import numpy as np
+import pytest
+
+from helpers import backward_error, entry_rel_err, mna_case
+
+pytestmark = pytest.mark.gpu
+
+CASES = ["mna600", "mna2k", "small", "add32", "batch"]
+
+
+@pytest.fixture(scope="module")
+def torch_mod():
+    import torch
+    return torch
+
+
+def _setup(name):
+    from oracle import oracle as O
+    from pyglu.analysis import Analysis
+    from pyglu.device import Plan
+    data, indices, indptr, n = mna_case(name)
+    an = Analysis(data, indices, indptr, n)
+    return data, indices, indptr, n, an, Plan(an), O.SymPattern(an)
+
+
+@pytest.mark.parametrize("name", CASES)
+@pytest.mark.parametrize("perturb", [False, True])
+def test_factor_matches_oracle(torch_mod, name, perturb):
+    """FP32 factors bit-identical to the oracle's sequential pass over each
+    level's columns (a valid order of the reference's atomics), hence within
+    the 1e-4-per-entry bar."""
+    from oracle import oracle as O
+    torch = torch_mod
+    data, indices, indptr, n, an, plan, pat = _setup(name)
+    print(f"[{name}] n={n} nnz_lu={an.nnz_lu} levels={an.num_lev} perturb={perturb}", flush=True)
+    lu = torch.from_numpy(an.val.copy()).cuda()
+    plan.factor(lu, perturb=perturb)
+    torch.cuda.synchronize()
+    got = lu.cpu().numpy()
+    want, pert = O.factor_f32(pat, an.val, perturb=perturb)
+    assert np.isfinite(got).all()
+    err = entry_rel_err(got, want, pat.sym_c_ptr)
+    exact = np.mean(got.view(np.uint32) == want.view(np.uint32))
+    print(f"  max rel err {err.max():.3e}  exact fraction {exact:.6f}")
+    assert err.max() <= 1e-4
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+@pytest.mark.parametrize("name", ["mna2k", "small", "add32"])
+def test_factor_host_matches_device(torch_mod, name):
+    """glu_factor_host (the LUonDevice contract: host buffer in/out) and
+    repeated glu_factor calls on device memory give identical bits."""
+    torch = torch_mod
+    data, indices, indptr, n, an, plan, pat = _setup(name)
+    host = an.val.copy()
+    plan.factor_host(host)
+    for _ in range(3):
+        lu = torch.from_numpy(an.val.copy()).cuda()
+        plan.factor(lu)
+        torch.cuda.synchronize()
+        assert np.array_equal(host.view(np.uint32), lu.cpu().numpy().view(np.uint32))
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_solve_bit_exact(torch_mod, name):
+    """With the same factors, the device solve equals the reference's FP32
+    host solve (restated in the oracle) bit for bit."""
+    from oracle import oracle as O
+    from pyglu import mna
+    torch = torch_mod
+    data, indices, indptr, n, an, plan, pat = _setup(name)
+    lu = torch.from_numpy(an.val.copy()).cuda()
+    plan.factor(lu)
+    b = mna.rhs(n, 101).astype(np.float32)
+    x = plan.solve(lu, torch.from_numpy(b).cuda())
+    torch.cuda.synchronize()
+    want = O.solve_f32(pat, lu.cpu().numpy(), an.col_perm, an.row_perm_inv, an.col_scale_perm, an.row_scale, b)
+    assert np.array_equal(x.cpu().numpy().view(np.uint32), want.view(np.uint32))
+
+
+@pytest.mark.parametrize("name", ["mna2k", "small", "add32"])
+def test_transposed_solve(torch_mod, name):
+    """A' x = b with the FP32 factors: close to the FP64 oracle solve."""
+    from oracle import oracle as O
+    torch = torch_mod
+    data, indices, indptr, n, an, plan, pat = _setup(name)
+    lu = torch.from_numpy(an.val.copy()).cuda()
+    plan.factor(lu)
+    b = np.random.default_rng(3).uniform(-1, 1, n)
+    x = plan.solve(lu, torch.from_numpy(b.astype(np.float32)).cuda(), transpose=True).cpu().numpy()
+    lu64 = O.factor_f64(pat, O.lu_values_f64(pat, an.dump_ax, an.dump_ai, an.dump_ap))
+    x64 = O.solve_f64(pat, lu64, an, b.astype(np.float32).astype(np.float64), transpose=True)
+    rel = np.abs(x - x64).max() / np.abs(x64).max()
+    print(f"[{name}] transposed solve rel err vs FP64 {rel:.2e}")
+    assert rel < 1e-3
+
+
+@pytest.mark.parametrize("name", ["mna2k", "small", "add32", "batch"])
+@pytest.mark.parametrize("transpose", [False, True])
+def test_refine_backward_error(torch_mod, name, transpose):
+    """FP32 LU + FP64 residual refinement reaches eta <= 1e-10 (A and A')."""
+    torch = torch_mod
+    data, indices, indptr, n, an, plan, pat = _setup(name)
+    lu = torch.from_numpy(an.val.copy()).cuda()
+    plan.factor(lu)
+    b = np.random.default_rng(5).uniform(-1, 1, n)
+    x, it, eta = plan.refine(torch.from_numpy(data).cuda(), lu, torch.from_numpy(b).cuda(), tol=1e-10,
+                             maxit=20, transpose=transpose)
+    xh = x.cpu().numpy()
+    if transpose:
+        import scipy.sparse as sp
+        At = sp.csc_matrix((data, indices, indptr), shape=(n, n)).T.tocsc()
+        eta_h = backward_error(At.data, At.indices, At.indptr, n, xh, b)
+    else:
+        eta_h = backward_error(data, indices, indptr, n, xh, b)
+    print(f"[{name}] transpose={transpose} iters={it} eta={eta:.2e} host eta={eta_h:.2e}")
+    assert eta <= 1e-10 and eta_h <= 2e-10
+
+
+def test_batch_matches_single(torch_mod):
+    """The persistent same-pattern batch gives, per matrix, what the single
+    path gives: refreshed values, factors within the atomics' freedom, and
+    solutions refined to the tolerance."""
+    from pyglu import mna
+    torch = torch_mod
+    data, indices, indptr, n, an, plan, pat = _setup("mna2k")
+    B = 6
+    ax = np.stack([mna.perturb_values(data, 3000 + b) for b in range(B)])
+    bb = np.stack([mna.rhs(n, 3500 + b) for b in range(B)])
+    axd = torch.from_numpy(ax).cuda()
+    x, lu, iters, eta = plan.batch_solve(axd, torch.from_numpy(bb).cuda(), tol=1e-10, maxit=20)
+    torch.cuda.synchronize()
+    lu_ref = plan.refresh(axd)
+    for b in range(B):
+        one = lu_ref[b].clone()
+        plan.factor(one)
+        torch.cuda.synchronize()
+        assert np.array_equal(lu[b].cpu().numpy().view(np.uint32), one.cpu().numpy().view(np.uint32))
+        e = backward_error(ax[b], indices, indptr, n, x[b].cpu().numpy(), bb[b])
+        print(f"  matrix {b}: iters={int(iters[b])} eta={float(eta[b]):.2e} host={e:.2e}")
+        assert float(eta[b]) <= 1e-10 and e <= 2e-10
+
+
+def test_refresh_matches_analysis(torch_mod):
+    """Refreshing the analysed values reproduces predictLU's FP32 values up to
+    the product order of the two MC64 scaling passes (<= 1 ulp)."""
+    torch = torch_mod
+    data, indices, indptr, n, an, plan, pat = _setup("small")
+    lu = plan.refresh(torch.from_numpy(data).cuda()).cpu().numpy().reshape(-1)
+    ulps = np.abs(lu.view(np.int32).astype(np.int64) - an.val.view(np.int32).astype(np.int64))
+    assert ulps.max() <= 1 and (lu == 0).sum() == (an.val == 0).sum()
+
+
+def test_group_matches_single(torch_mod):
+    """An uneven group of distinct patterns, one fused launch."""
+    from pyglu import mna
+    from pyglu.analysis import Analysis
+    from pyglu.device import Group, Plan
+    torch = torch_mod
+    mats, plans, axs, bs = [], [], [], []
+    for spec, seed in mna.uneven_specs(6, seed0=4100):
+        spec = mna.MNASpec(n_nodes=min(spec.n_nodes, 3000), n_vsrc=min(spec.n_vsrc, 20),
+                           extra_hi=spec.extra_hi, n_rails=spec.n_rails, rail_deg=min(spec.rail_deg, 100))
+        d, i, p, n = mna.mna_csc(spec, seed)
+        an = Analysis(d, i, p, n)
+        mats.append((d, i, p, n))
+        plans.append(Plan(an))
+        axs.append(torch.from_numpy(d).cuda())
+        bs.append(torch.from_numpy(mna.rhs(n, seed + 1)).cuda())
+    g = Group(plans)
+    xs, lus, iters, eta = g.solve(axs, bs, tol=1e-10, maxit=20)
+    torch.cuda.synchronize()
+    for k, (d, i, p, n) in enumerate(mats):
+        e = backward_error(d, i, p, n, xs[k].cpu().numpy(), bs[k].cpu().numpy())
+        print(f"  group member {k}: n={n} iters={int(iters[k])} eta={float(eta[k]):.2e} host={e:.2e}")
+        assert float(eta[k]) <= 1e-10 and e <= 2e-10
+
+
+def test_autograd_against_dense_fp64(torch_mod):
+    """dL/dA (on A's pattern) and dL/db of L = |x - x*|^2 against the dense
+    FP64 reference gradient and central finite differences."""
+    import scipy.sparse as sp
+    from pyglu import mna
+    from pyglu.analysis import Analysis
+    from pyglu.device import Plan, spsolve_autograd
+    torch = torch_mod
+    d, i, p, n = mna.mna_csc(mna.MNASpec(n_nodes=300, n_vsrc=6, n_rails=2, rail_deg=30), 51)
+    an = Analysis(d, i, p, n)
+    plan = Plan(an)
+    rng = np.random.default_rng(5000)
+    xstar = rng.standard_normal(n)
+    b = rng.uniform(-1, 1, n)
+    vals = torch.from_numpy(d.copy()).cuda().requires_grad_(True)
+    bt = torch.from_numpy(b).cuda().requires_grad_(True)
+    x = spsolve_autograd(vals, bt, plan)
+    loss = ((x - torch.from_numpy(xstar).cuda()) ** 2).sum()
+    loss.backward()
+    gA = vals.grad.cpu().numpy()
+    gb = bt.grad.cpu().numpy()
+    # dense FP64 reference
+    A = sp.csc_matrix((d, i, p), shape=(n, n)).toarray()
+    xd = np.linalg.solve(A, b)
+    lam = np.linalg.solve(A.T, 2 * (xd - xstar))
+    cols = np.repeat(np.arange(n), np.diff(p))
+    gA_ref = -lam[i] * xd[cols]
+    relA = np.abs(gA - gA_ref).max() / np.abs(gA_ref).max()
+    relb = np.abs(gb - lam).max() / np.abs(lam).max()
+    print(f"grad rel err: A {relA:.2e}  b {relb:.2e}")
+    assert relA <= 1e-6 and relb <= 1e-6
+    # central finite differences on a few entries (FP64 dense solves)
+    h = 1e-6
+    for k in rng.choice(d.size, 10, replace=False):
+        Ap, Am = A.copy(), A.copy()
+        Ap[i[k], cols[k]] += h
+        Am[i[k], cols[k]] -= h
+        fp = ((np.linalg.solve(Ap, b) - xstar) ** 2).sum()
+        fm = ((np.linalg.solve(Am, b) - xstar) ** 2).sum()
+        fd = (fp - fm) / (2 * h)
+        assert abs(fd - gA[k]) <= 1e-5 * max(abs(fd), 1e-3 * np.abs(gA).max())
