@@ -121,7 +121,7 @@ def main():
         xb = x[b].cpu().numpy()
         r = b_h[b] - A @ xb
         e = np.abs(r).max() / (np.abs(A).sum(axis=1).max() * np.abs(xb).max() + np.abs(b_h[b]).max())
-        ok = ok and e <= 2 * args.tol
+        ok = bool(ok and e <= 2 * args.tol)
     value = B * K / wall
     log(f"[bench] batch: {wall / K * 1e3:.3f} ms/step (events {kernel_ms:.3f} ms) -> {value:.1f} refactors/s; "
         f"iters {iters_h.min()}..{iters_h.max()} max eta {eta_h.max():.2e} ok={ok}")

@@ -2,11 +2,10 @@
 // pointers, passed by value to kernels or stored in device memory for the
 // fused multi-matrix launch).
 //
-// Layout in HBM (all 32-bit indices unless noted, one copy per pattern):
-//   factor: diag/lcnt/ufirst/ucnt/moff per column, upos (CSR -> CSC position,
-//           nnz_lu), umap (one target CSC position per scalar update,
-//           grouped per column k as |U_k| x |L_k| row-major), work units per
-//           level, columns to scale per level.
+// Layout in HBM (all 32-bit indices, one copy per pattern):
+//   factor: per level, 16-byte update items (target, L, U and pivot
+//           positions of one scalar update), merge entries with their source
+//           lists, and 8-byte division items.
 //   solves: row lists per level for the four triangular sweeps.
 //   refine: CSR of the original A (row pointers, columns, source index into
 //           the caller's CSC values), per-LU-position source index and the
@@ -15,16 +14,11 @@
 // matrix; a batch lays matrices out back to back (matrix-major).
 #pragma once
 
+#include <hip/hip_runtime.h>
+
 #include <cstdint>
 
 namespace glu {
-
-struct Unit {          // one wave's share of a column update
-    uint32_t col;      // pivot column k
-    uint32_t jb;       // first U_k entry (index into the CSR row past the diagonal)
-    uint32_t jc;       // number of U_k entries
-    uint32_t pad;
-};
 
 // An entry updated by several pivot columns of one level: one lane owns it
 // and applies the updates in increasing column order (deterministic).
@@ -35,25 +29,27 @@ struct MergeSrc {
     uint32_t pad;
 };
 
-// Scatter-map entries whose target is owned by a merge lane carry this bit.
-constexpr uint32_t kMergedBit = 0x80000000u;
+// Flag of a division item: the first L entry of its column, whose lane also
+// stores the GESP pivot.
+constexpr uint32_t kFirstBit = 0x80000000u;
 
+// Per-level lane items of the right-looking factorization (one lane each):
+//   update  A(i,j) += -(A(i,k) / A(k,k) * A(k,j))   {tgt, lpos, upos, dpos}
+//           for entries exactly one column of the level updates;
+//   merge   all updates of an entry several columns of the level update,
+//           in increasing column order;
+//   divide  A(i,k) /= A(k,k) for the columns of the previous level
+//           {lpos, dpos | kFirstBit} (deferred so a level needs one barrier).
 struct FactorDev {
-    const uint32_t *col_ptr;   // sym_c_ptr [n+1]
-    const uint32_t *diag;      // l_col_ptr [n]
-    const uint32_t *lcnt;      // |L_k| [n]
-    const uint32_t *ufirst;    // csr_diag_ptr + 1 [n]
-    const uint64_t *moff;      // offset of column k's block in umap [n]
-    const uint32_t *upos;      // CSR position -> CSC position [nnz]
-    const uint32_t *umap;      // scatter targets [updates]
-    const Unit *units;         // [units]
-    const uint32_t *unit_ptr;  // per level [num_lev + 2]
-    const uint32_t *dcols;     // columns with |L_k| > 0, by level
-    const uint32_t *dcol_ptr;  // per level [num_lev + 1]
+    const uint32_t *col_ptr;   // sym_c_ptr [n+1] (GESP column sums)
+    const uint4 *upd;          // update items, by level
+    const uint32_t *upd_ptr;   // [num_lev + 2]
     const uint32_t *ment;      // merge entries (target CSC positions), by level
-    const uint32_t *ment_ptr;  // per level [num_lev + 2]
+    const uint32_t *ment_ptr;  // [num_lev + 2]
     const uint32_t *msrc_ptr;  // per merge entry [entries + 1]
     const MergeSrc *msrc;      // sources in increasing column order
+    const uint2 *dvd;          // division items, by level
+    const uint32_t *dvd_ptr;   // [num_lev + 1]
     int32_t num_lev;
     uint32_t n, nnz;
 };
@@ -68,8 +64,9 @@ struct TriDev {
     const uint32_t *idx;
     const uint32_t *vpos;      // nullptr: identity
     const uint32_t *dpos;      // nullptr: unit diagonal
-    const uint32_t *rows;      // rows grouped by level
+    const uint32_t *rows;      // rows grouped by level, long rows first
     const uint32_t *lvl_ptr;   // [nlev + 1]
+    const uint32_t *lvl_long;  // long rows at the head of each level [nlev]
     uint32_t begoff;
     int32_t desc;
     int32_t nlev;

@@ -78,125 +78,106 @@ __device__ double wg_max(double v, double *sh)
 // factorization
 // ---------------------------------------------------------------------------
 
-// One work unit: pivot column k, U_k entries [jb, jb + jc).
-//   l_i = A(i,k) / d          (d = A(k,k), GESP-perturbed if requested)
-//   A(i,j) += -(l_i * A(k,j)) for i in L_k, j in the slice of U_k
-// except targets that another column of the same level also updates; those
-// belong to a merge lane (factor_merge).  Every remaining target receives one
-// update per level, so the atomic is a plain fire-and-forget add and the
-// result does not depend on the order waves run in.
-// Lanes are split into groups of G = pow2 >= |L_k| (|L_k| <= 64), each group
-// owning one j at a time; longer columns loop over L_k in strides of 64.
-template <int Scope>
-__device__ __forceinline__ void factor_unit(const FactorDev &F, float *__restrict__ val,
-                                            const Unit u, int lane, float pert, int perturb)
+__device__ __forceinline__ float gesp(float d, float pert, int perturb)
 {
-    const uint32_t k = u.col;
-    const uint32_t dpos = F.diag[k];
-    const uint32_t L = F.lcnt[k];
-    const uint32_t *__restrict__ up = F.upos + F.ufirst[k] + u.jb;
-    const uint32_t *__restrict__ map = F.umap + (F.moff[k] + (uint64_t)u.jb * L);
-    float d = val[dpos];
-    if (perturb && fabsf(d) < pert) d = pert;
-    if (L <= (uint32_t)kWave) {
-        const uint32_t G = pow2_ceil(L);
-        const uint32_t sh = (uint32_t)(__ffs((int)G) - 1);
-        const uint32_t ii = (uint32_t)lane & (G - 1u);
-        const uint32_t g = (uint32_t)lane >> sh;
-        const uint32_t step = (uint32_t)kWave >> sh;
-        const bool act = ii < L;
-        float l = 0.f;
-        if (act) l = val[dpos + 1 + ii] / d;
-        for (uint32_t jj = g; jj < u.jc; jj += step) {
-            const float uv = val[up[jj]];
-            if (act) {
-                const uint32_t t = map[(uint64_t)jj * L + ii];
-                if (!(t & kMergedBit)) atomic_sub<Scope>(val + t, l * uv);
-            }
+    return (perturb && fabsf(d) < pert) ? pert : d;
+}
+
+// Update items [t, t + Kunroll*stride, ...) of one level, Kunroll at a time:
+// all loads of the group are issued before its atomics, so a lane keeps
+// several independent memory requests in flight.
+//   A(i,j) += -(l * A(k,j)),  l = A(i,k) / A(k,k)  (A(k,k) GESP-perturbed)
+// Each target here receives exactly this one update in this level, so the
+// fire-and-forget atomic add gives an order-independent result.
+template <int Scope>
+__device__ __forceinline__ void factor_updates(const uint4 *__restrict__ upd, uint32_t count, uint32_t t,
+                                               uint32_t stride, float *__restrict__ val, float pert, int perturb)
+{
+    constexpr int U = 4;
+    for (; t < count; t += U * stride) {
+        uint4 it[U];
+        bool on[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t q = t + u * stride;
+            on[u] = q < count;
+            if (on[u]) it[u] = upd[q];
         }
-    } else {
-        for (uint32_t jj = 0; jj < u.jc; ++jj) {
-            const float uv = val[up[jj]];
-            const uint32_t *__restrict__ mrow = map + (uint64_t)jj * L;
-            for (uint32_t ii = (uint32_t)lane; ii < L; ii += kWave) {
-                const uint32_t t = mrow[ii];
-                if (t & kMergedBit) continue;
-                const float l = val[dpos + 1 + ii] / d;
-                atomic_sub<Scope>(val + t, l * uv);
+        float d[U], l[U], v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (on[u]) {
+                d[u] = val[it[u].w];
+                l[u] = val[it[u].y];
+                v[u] = val[it[u].z];
             }
-        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (on[u]) {
+                const float lk = l[u] / gesp(d[u], pert, perturb);
+                atomic_sub<Scope>(val + it[u].x, lk * v[u]);
+            }
     }
 }
 
-// Merge entries e0 + lane: apply every source of this level in increasing
-// column order, t += -(l * u) with l = L(i,k) / U(k,k) — the order in which
-// a sequential pass over the level's columns would apply them.
+// Merge entry e: every update of this level in increasing column order,
+// t += -(l * u) — the order a sequential pass over the level applies them.
 __device__ __forceinline__ void factor_merge(const FactorDev &F, float *__restrict__ val, uint32_t e,
-                                             uint32_t e_end, float pert, int perturb)
+                                             float pert, int perturb)
 {
-    if (e >= e_end) return;
     const uint32_t t = F.ment[e];
     float v = val[t];
     const uint32_t s1 = F.msrc_ptr[e + 1];
     for (uint32_t s = F.msrc_ptr[e]; s < s1; ++s) {
         const MergeSrc m = F.msrc[s];
-        float d = val[m.dpos];
-        if (perturb && fabsf(d) < pert) d = pert;
-        const float l = val[m.lpos] / d;
+        const float l = val[m.lpos] / gesp(val[m.dpos], pert, perturb);
         v = v + (-(l * val[m.upos]));
     }
     val[t] = v;
 }
 
-// Deferred write-back of column k: L(:,k) /= d, and d := pert when GESP fired.
-__device__ __forceinline__ void factor_div(const FactorDev &F, float *__restrict__ val,
-                                           uint32_t k, int lane, float pert, int perturb)
+// Division item of a column of the previous level: L(i,k) /= U(k,k); the
+// lane of the column's first L entry stores the GESP pivot.  Other lanes of
+// the same column may read the pivot before or after that store: both
+// values perturb to the same pert.
+__device__ __forceinline__ void factor_divide(float *__restrict__ val, uint2 it, float pert, int perturb)
 {
-    const uint32_t dpos = F.diag[k];
-    const uint32_t L = F.lcnt[k];
-    float d = val[dpos];
-    const bool pt = perturb && fabsf(d) < pert;
-    if (pt) d = pert;
-    for (uint32_t ii = (uint32_t)lane; ii < L; ii += kWave) val[dpos + 1 + ii] = val[dpos + 1 + ii] / d;
-    if (pt && lane == 0) val[dpos] = pert;
+    const uint32_t dpos = it.y & ~kFirstBit;
+    const float d0 = val[dpos];
+    const bool pt = perturb && fabsf(d0) < pert;
+    val[it.x] = val[it.x] / (pt ? pert : d0);
+    if (pt && (it.y & kFirstBit)) val[dpos] = pert;
 }
 
-__device__ __forceinline__ uint32_t merge_waves(const FactorDev &F, int lvl)
-{
-    return (F.ment_ptr[lvl + 1] - F.ment_ptr[lvl] + kWave - 1) / kWave;
-}
+struct LevelSpan {
+    uint32_t u0, nu, m0, nm, d0, nd;
+    __device__ __forceinline__ uint32_t items() const { return nu + nm + nd; }
+};
 
-__device__ __forceinline__ uint32_t level_items(const FactorDev &F, int lvl)
+__device__ __forceinline__ LevelSpan level_span(const FactorDev &F, int lvl)
 {
-    uint32_t items = F.unit_ptr[lvl + 1] - F.unit_ptr[lvl] + merge_waves(F, lvl);
-    if (lvl > 0) items += F.dcol_ptr[lvl] - F.dcol_ptr[lvl - 1];
-    return items;
-}
-
-// Item t of level lvl: update units of lvl, then merge waves (64 entries
-// each), then the write-backs of the columns of level lvl-1.  lvl == num_lev
-// has write-backs only.
-template <int Scope>
-__device__ __forceinline__ void factor_item(const FactorDev &F, float *__restrict__ val, int lvl,
-                                            uint32_t item, int lane, float pert, int perturb)
-{
-    const uint32_t u0 = F.unit_ptr[lvl];
-    const uint32_t nu = F.unit_ptr[lvl + 1] - u0;
-    if (item < nu) {
-        factor_unit<Scope>(F, val, F.units[u0 + item], lane, pert, perturb);
-        return;
-    }
-    item -= nu;
-    const uint32_t nm = merge_waves(F, lvl);
-    if (item < nm) {
-        factor_merge(F, val, F.ment_ptr[lvl] + item * kWave + (uint32_t)lane, F.ment_ptr[lvl + 1], pert, perturb);
-        return;
-    }
-    item -= nm;
+    LevelSpan s;
+    s.u0 = F.upd_ptr[lvl];
+    s.nu = F.upd_ptr[lvl + 1] - s.u0;
+    s.m0 = F.ment_ptr[lvl];
+    s.nm = F.ment_ptr[lvl + 1] - s.m0;
     if (lvl > 0) {
-        const uint32_t d0 = F.dcol_ptr[lvl - 1];
-        if (item < F.dcol_ptr[lvl] - d0) factor_div(F, val, F.dcols[d0 + item], lane, pert, perturb);
+        s.d0 = F.dvd_ptr[lvl - 1];
+        s.nd = F.dvd_ptr[lvl] - s.d0;
+    } else {
+        s.d0 = s.nd = 0;
     }
+    return s;
+}
+
+// Lanes [first, first + stride*...) of one level: updates, merges, divisions.
+template <int Scope>
+__device__ __forceinline__ void factor_level(const FactorDev &F, float *__restrict__ val, const LevelSpan &s,
+                                             uint32_t first, uint32_t stride, float pert, int perturb)
+{
+    factor_updates<Scope>(F.upd + s.u0, s.nu, first, stride, val, pert, perturb);
+    for (uint32_t t = first; t < s.nm; t += stride) factor_merge(F, val, s.m0 + t, pert, perturb);
+    for (uint32_t t = first; t < s.nd; t += stride) factor_divide(val, F.dvd[s.d0 + t], pert, perturb);
 }
 
 // GESP: pert = 3.45e-4 * max_c sum_p |val[p]| (FP32 sums in column order).
@@ -222,27 +203,25 @@ __global__ void k_zero_u64(unsigned long long *__restrict__ p, int count)
     if ((int)threadIdx.x < count) p[threadIdx.x] = 0ull;
 }
 
-// One level across the grid: one wave per item.
+// One level across the grid.  The grid covers ceil(items / 4) lanes for the
+// update part (4 items per lane) plus one lane per merge and division item;
+// the host sizes it with level_lanes().
 __global__ void k_factor_grid(FactorDev F, float *__restrict__ val, const float *__restrict__ d_pert,
                               int lvl, int perturb)
 {
-    const uint32_t item = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
-    const int lane = threadIdx.x & (kWave - 1);
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t stride = gridDim.x * blockDim.x;
     const float pert = perturb ? *d_pert : 0.f;
-    factor_item<__HIP_MEMORY_SCOPE_AGENT>(F, val, lvl, item, lane, pert, perturb);
+    factor_level<__HIP_MEMORY_SCOPE_AGENT>(F, val, level_span(F, lvl), g, stride, pert, perturb);
 }
 
 // A run of narrow levels inside one workgroup, a barrier between levels.
 __global__ void k_factor_fused(FactorDev F, float *__restrict__ val, const float *__restrict__ d_pert,
                                int l0, int l1, int perturb)
 {
-    const int lane = threadIdx.x & (kWave - 1);
-    const uint32_t w = threadIdx.x / kWave, nw = blockDim.x / kWave;
     const float pert = perturb ? *d_pert : 0.f;
     for (int lvl = l0; lvl < l1; ++lvl) {
-        const uint32_t items = level_items(F, lvl);
-        for (uint32_t t = w; t < items; t += nw)
-            factor_item<__HIP_MEMORY_SCOPE_AGENT>(F, val, lvl, t, lane, pert, perturb);
+        factor_level<__HIP_MEMORY_SCOPE_AGENT>(F, val, level_span(F, lvl), threadIdx.x, blockDim.x, pert, perturb);
         __syncthreads();
     }
 }
@@ -419,22 +398,52 @@ struct MatIO {
     double *eta;
 };
 
-__device__ __forceinline__ void wg_tri(const TriDev &T, const float *__restrict__ lu, float *__restrict__ x)
+__device__ __forceinline__ float wave_sum(float v)
 {
+    for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+// One triangular sweep of the refinement inside a workgroup (x may be LDS).
+// Rows longer than kLongRow (listed first in each level) take a wave each
+// and sum their products with a wave reduction; the rest take a lane each
+// and accumulate sequentially.  This is the correction solve of iterative
+// refinement, where the summation order is free; glu_solve keeps the
+// reference's sequential order for every row.
+__device__ __forceinline__ void wg_tri_refine(const TriDev &T, const float *__restrict__ lu, float *x)
+{
+    const uint32_t tid = threadIdx.x, nt = blockDim.x;
+    const uint32_t lane = tid & (kWave - 1), w = tid / kWave, nw = nt / kWave;
     for (int lvl = 0; lvl < T.nlev; ++lvl) {
-        for (uint32_t t = T.lvl_ptr[lvl] + threadIdx.x; t < T.lvl_ptr[lvl + 1]; t += blockDim.x)
-            tri_row(T, lu, x, T.rows[t]);
+        const uint32_t r0 = T.lvl_ptr[lvl], r1 = T.lvl_ptr[lvl + 1], nl = T.lvl_long[lvl];
+        for (uint32_t t = w; t < nl; t += nw) {
+            const uint32_t r = T.rows[r0 + t];
+            const uint32_t b = T.beg[r] + T.begoff, e = T.end[r];
+            float sum = 0.f;
+            for (uint32_t q = b + lane; q < e; q += kWave) {
+                const uint32_t p = T.vpos ? T.vpos[q] : q;
+                sum += lu[p] * x[T.idx[q]];
+            }
+            sum = wave_sum(sum);
+            if (lane == 0) {
+                float acc = x[r] - sum;
+                if (T.dpos) acc = acc / lu[T.dpos[r]];
+                x[r] = acc;
+            }
+        }
+        for (uint32_t t = r0 + nl + tid; t < r1; t += nt) tri_row(T, lu, x, T.rows[t]);
         __syncthreads();
     }
 }
 
+// One workgroup runs one matrix end to end: value refresh, GESP value,
+// factorization, then refinement sweeps until eta <= tol or maxit.  xs is
+// the FP32 correction vector (LDS when it fits, else the global scratch).
 __device__ void wg_pipeline(const MatDesc &M, const MatIO &io, double tol, int maxit, int perturb,
-                            double *sh)
+                            double *sh, float *xs)
 {
     const uint32_t n = M.n;
     const uint32_t tid = threadIdx.x, nt = blockDim.x;
-    const int lane = tid & (kWave - 1);
-    const uint32_t w = tid / kWave, nw = nt / kWave;
     const FactorDev &F = M.f;
     const SolveDev &S = M.s;
     const RefineDev &R = M.r;
@@ -467,9 +476,7 @@ __device__ void wg_pipeline(const MatDesc &M, const MatIO &io, double tol, int m
 
     // 3. factorization, level by level
     for (int lvl = 0; lvl <= F.num_lev; ++lvl) {
-        const uint32_t items = level_items(F, lvl);
-        for (uint32_t t = w; t < items; t += nw)
-            factor_item<__HIP_MEMORY_SCOPE_WORKGROUP>(F, io.lu, lvl, t, lane, pert, perturb);
+        factor_level<__HIP_MEMORY_SCOPE_WORKGROUP>(F, io.lu, level_span(F, lvl), tid, nt, pert, perturb);
         __syncthreads();
     }
 
@@ -477,15 +484,13 @@ __device__ void wg_pipeline(const MatDesc &M, const MatIO &io, double tol, int m
     int it = 0;
     double eta = 0.0;
     for (;;) {
-        for (uint32_t j = tid; j < n; j += nt) {
-            const uint32_t pj = S.col_perm[j];
-            io.t32[j] = (float)(io.r64[pj] * S.col_scale_perm[j]);
-        }
-        __syncthreads();
-        wg_tri(S.lo, io.lu, io.t32);
-        wg_tri(S.up, io.lu, io.t32);
         for (uint32_t j = tid; j < n; j += nt)
-            io.x[j] = io.x[j] + (double)io.t32[S.row_perm_inv[j]] * S.row_scale[j];
+            xs[j] = (float)(io.r64[S.col_perm[j]] * S.col_scale_perm[j]);
+        __syncthreads();
+        wg_tri_refine(S.lo, io.lu, xs);
+        wg_tri_refine(S.up, io.lu, xs);
+        for (uint32_t j = tid; j < n; j += nt)
+            io.x[j] = io.x[j] + (double)xs[S.row_perm_inv[j]] * S.row_scale[j];
         __syncthreads();
         double rmax = 0.0, xmax = 0.0;
         for (uint32_t i = tid; i < n; i += nt) {
@@ -506,8 +511,12 @@ __device__ void wg_pipeline(const MatDesc &M, const MatIO &io, double tol, int m
     }
 }
 
+extern __shared__ float g_lds_vec[];
+
 // Same pattern: matrix b = blockIdx.x uses desc[0] and contiguous buffers.
-__global__ void k_batch_same(const MatDesc *__restrict__ desc, const double *__restrict__ ax,
+// kLds: the correction vector lives in dynamic LDS (n floats).
+template <bool kLds>
+__global__ void __launch_bounds__(1024) k_batch_same(const MatDesc *__restrict__ desc, const double *__restrict__ ax,
                              const double *__restrict__ b, double *__restrict__ x, float *__restrict__ lu,
                              float *__restrict__ t32, double *__restrict__ r64, int32_t *__restrict__ iters,
                              double *__restrict__ eta, double tol, int maxit, int perturb)
@@ -524,16 +533,17 @@ __global__ void k_batch_same(const MatDesc *__restrict__ desc, const double *__r
     io.r64 = r64 + m * M.n;
     io.iters = iters + m;
     io.eta = eta + m;
-    wg_pipeline(M, io, tol, maxit, perturb, sh);
+    wg_pipeline(M, io, tol, maxit, perturb, sh, kLds ? g_lds_vec : io.t32);
 }
 
 // Distinct patterns: workgroup g runs matrix order[g] with its own tables.
-__global__ void k_batch_group(const MatDesc *__restrict__ descs, const MatIO *__restrict__ ios,
+template <bool kLds>
+__global__ void __launch_bounds__(1024) k_batch_group(const MatDesc *__restrict__ descs, const MatIO *__restrict__ ios,
                               const uint32_t *__restrict__ order, double tol, int maxit, int perturb)
 {
     __shared__ double sh[1024 / kWave];
     const uint32_t m = order[blockIdx.x];
-    wg_pipeline(descs[m], ios[m], tol, maxit, perturb, sh);
+    wg_pipeline(descs[m], ios[m], tol, maxit, perturb, sh, kLds ? g_lds_vec : ios[m].t32);
 }
 
 // ---------------------------------------------------------------------------
@@ -551,11 +561,11 @@ void pert(const FactorDev &F, const float *val, unsigned int *norm_bits, float *
     k_pert_finish<<<1, 1, 0, st>>>(norm_bits, d_pert);
 }
 
-void factor_grid(const FactorDev &F, float *val, const float *d_pert, int lvl, uint32_t items,
+// lanes: the lanes the level needs (update items count 1/4 each)
+void factor_grid(const FactorDev &F, float *val, const float *d_pert, int lvl, uint32_t lanes,
                  int perturb, int threads, hipStream_t st)
 {
-    const unsigned wpb = (unsigned)threads / kWave;
-    k_factor_grid<<<blocks_for(items, wpb), threads, 0, st>>>(F, val, d_pert, lvl, perturb);
+    k_factor_grid<<<blocks_for(lanes, (unsigned)threads), threads, 0, st>>>(F, val, d_pert, lvl, perturb);
 }
 
 void factor_fused(const FactorDev &F, float *val, const float *d_pert, int l0, int l1, int perturb,
@@ -604,18 +614,43 @@ void residual(const RefineDev &R, const double *ax, const double *b, const doubl
     k_residual<<<blocks_for(n, 256), 256, 0, st>>>(R, ax, b, x, r, n, transpose, with_anorm, norms);
 }
 
+// lds_bytes > 0: the correction vector is kept in that much dynamic LDS.
 void batch_same(const MatDesc *desc, int batch, const double *ax, const double *b, double *x, float *lu,
                 float *t32, double *r64, int32_t *iters, double *eta, double tol, int maxit, int perturb,
-                int threads, hipStream_t st)
+                int threads, size_t lds_bytes, hipStream_t st)
 {
-    k_batch_same<<<batch, threads, 0, st>>>(desc, ax, b, x, lu, t32, r64, iters, eta, tol, maxit, perturb);
+    if (lds_bytes)
+        k_batch_same<true><<<batch, threads, lds_bytes, st>>>(desc, ax, b, x, lu, t32, r64, iters, eta, tol, maxit,
+                                                             perturb);
+    else
+        k_batch_same<false><<<batch, threads, 0, st>>>(desc, ax, b, x, lu, t32, r64, iters, eta, tol, maxit,
+                                                      perturb);
 }
 
 void batch_group(const MatDesc *descs, const void *ios, const uint32_t *order, int count, double tol,
-                 int maxit, int perturb, int threads, hipStream_t st)
+                 int maxit, int perturb, int threads, size_t lds_bytes, hipStream_t st)
 {
-    k_batch_group<<<count, threads, 0, st>>>(descs, static_cast<const MatIO *>(ios), order, tol, maxit,
-                                            perturb);
+    const MatIO *io = static_cast<const MatIO *>(ios);
+    if (lds_bytes)
+        k_batch_group<true><<<count, threads, lds_bytes, st>>>(descs, io, order, tol, maxit, perturb);
+    else
+        k_batch_group<false><<<count, threads, 0, st>>>(descs, io, order, tol, maxit, perturb);
+}
+
+// Allows dynamic LDS above the default cap for the batch kernels; returns
+// the per-workgroup LDS limit in bytes.
+size_t enable_lds(size_t bytes)
+{
+    int dev = 0, lim = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return 0;
+    const size_t avail = (size_t)lim > 1024 ? (size_t)lim - 1024 : 0;  // static LDS of the kernels
+    if (bytes > avail) return avail;
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_batch_same<true>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_batch_group<true>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    return avail;
 }
 
 void adjoint_values(const RefineDev &R, const double *lam, const double *x, double *grad, uint32_t n,

@@ -44,9 +44,10 @@ void residual(const RefineDev &R, const double *ax, const double *b, const doubl
               int transpose, int with_anorm, unsigned long long *norms, hipStream_t st);
 void batch_same(const MatDesc *desc, int batch, const double *ax, const double *b, double *x, float *lu,
                 float *t32, double *r64, int32_t *iters, double *eta, double tol, int maxit, int perturb,
-                int threads, hipStream_t st);
+                int threads, size_t lds_bytes, hipStream_t st);
 void batch_group(const MatDesc *descs, const void *ios, const uint32_t *order, int count, double tol,
-                 int maxit, int perturb, int threads, hipStream_t st);
+                 int maxit, int perturb, int threads, size_t lds_bytes, hipStream_t st);
+size_t enable_lds(size_t bytes);
 void zero_u64(unsigned long long *p, int count, hipStream_t st);
 void adjoint_values(const RefineDev &R, const double *lam, const double *x, double *grad, uint32_t n,
                     hipStream_t st);
@@ -196,10 +197,9 @@ extern "C" glu_status glu_analysis_get(const glu_analysis *a, glu_field f, void 
 
 namespace {
 
-constexpr uint32_t kUnitTarget = 256;   // scalar updates per work unit
 constexpr int kGridThreads = 256;
-constexpr int kDefaultFusedThreads = 512;
-constexpr int kDefaultFuseUnits = 32;
+constexpr int kDefaultFusedThreads = 1024;
+constexpr int kDefaultFuseUnits = 2048;   // lanes
 
 struct Segment {
     int l0, l1;      // levels [l0, l1)
@@ -208,7 +208,7 @@ struct Segment {
 };
 
 struct TriHost {
-    std::vector<uint32_t> rows, lvl_ptr;
+    std::vector<uint32_t> rows, lvl_ptr, lvl_long;
     std::vector<Segment> segs;
 };
 
@@ -295,7 +295,7 @@ bool validate_pattern(const glu_pattern &P, std::string &err)
         err = "null pattern array";
         return false;
     }
-    if (nnz >= kMergedBit) {
+    if (nnz >= 0x80000000u) {
         err = "nnz(L+U) must be below 2^31";
         return false;
     }
@@ -377,41 +377,107 @@ bool validate_pattern(const glu_pattern &P, std::string &err)
 // --------------------------- tables -------------------------------------
 
 struct Tables {
-    std::vector<uint32_t> lcnt, ufirst, upos, csc2csr, umap;
-    std::vector<uint64_t> moff;
-    std::vector<Unit> units;
-    std::vector<uint32_t> unit_ptr, dcols, dcol_ptr;
+    std::vector<uint32_t> upos;        // CSR position -> CSC position
+    std::vector<uint4> upd;            // update items, by level
+    std::vector<uint32_t> upd_ptr;     // [num_lev + 2]
     std::vector<uint32_t> ment, ment_ptr, msrc_ptr;
     std::vector<MergeSrc> msrc;
+    std::vector<uint2> dvd;            // division items, by level
+    std::vector<uint32_t> dvd_ptr;     // [num_lev + 1]
 };
 
-// Entries that two or more columns of one level update become merge
-// entries: their sources are listed in increasing column order (the order
-// level_idx holds them in) and their scatter-map slots are flagged so the
-// work units skip them.
-void build_merges(const glu_pattern &P, Tables &T)
+// Lane items of the factorization, level by level.  For pivot column k of
+// level l, every (i in L_k, j in U_k) update targets the CSC position of
+// (i,j) in column j; the fill closure of the pattern guarantees it exists
+// (checked).  Targets that two or more columns of the level update become
+// merge entries whose sources keep the level's column order (level_idx lists
+// a level's columns in increasing order); all other updates are items.
+bool build_factor_tables(const glu_pattern &P, Tables &T, int64_t &updates, std::string &err)
 {
     const uint32_t n = P.n, nnz = P.nnz;
     const int nl = P.num_lev;
-    std::vector<uint32_t> stamp(nnz, UINT32_MAX), cnt(nnz, 0), slot(nnz, 0);
+    T.upos.resize(nnz);
+    std::vector<uint32_t> csc2csr(nnz);
+    {
+        std::vector<uint32_t> cursor(P.csr_r_ptr, P.csr_r_ptr + n);
+        for (uint32_t c = 0; c < n; ++c)
+            for (uint32_t p = P.sym_c_ptr[c]; p < P.sym_c_ptr[c + 1]; ++p) {
+                const uint32_t q = cursor[P.sym_r_idx[p]]++;
+                T.upos[q] = p;
+                csc2csr[p] = q;
+            }
+    }
+    // Scatter targets per (k, jj, ii), built per target column j.
+    std::vector<uint64_t> moff(n + 1);
+    uint64_t off = 0;
+    for (uint32_t k = 0; k < n; ++k) {
+        moff[k] = off;
+        const uint64_t L = P.sym_c_ptr[k + 1] - P.l_col_ptr[k] - 1;
+        const uint64_t U = P.csr_r_ptr[k + 1] - P.csr_diag_ptr[k] - 1;
+        off += L * U;
+    }
+    moff[n] = off;
+    updates = (int64_t)off;
+    std::vector<uint32_t> umap;
+    try {
+        umap.assign(off, 0);
+    } catch (const std::bad_alloc &) {
+        err = "update tables too large for host memory";
+        return false;
+    }
+    {
+        std::vector<uint32_t> stamp(n, UINT32_MAX), where(n);
+        for (uint32_t j = 0; j < n; ++j) {
+            for (uint32_t p = P.sym_c_ptr[j]; p < P.sym_c_ptr[j + 1]; ++p) {
+                stamp[P.sym_r_idx[p]] = j;
+                where[P.sym_r_idx[p]] = p;
+            }
+            for (uint32_t p = P.sym_c_ptr[j]; p < P.l_col_ptr[j]; ++p) {
+                const uint32_t k = P.sym_r_idx[p];
+                const uint32_t L = P.sym_c_ptr[k + 1] - P.l_col_ptr[k] - 1;
+                if (L == 0) continue;
+                const uint32_t jj = csc2csr[p] - P.csr_diag_ptr[k] - 1;
+                const uint64_t base = moff[k] + (uint64_t)jj * L;
+                const uint32_t l0 = P.l_col_ptr[k] + 1;
+                for (uint32_t ii = 0; ii < L; ++ii) {
+                    const uint32_t i = P.sym_r_idx[l0 + ii];
+                    if (stamp[i] != j) {
+                        err = "pattern is not closed under fill: (" + std::to_string(i) + "," + std::to_string(j) +
+                              ") missing";
+                        return false;
+                    }
+                    umap[base + ii] = where[i];
+                }
+            }
+        }
+    }
+    // Items per level.
+    T.upd_ptr.assign(nl + 2, 0);
     T.ment_ptr.assign(nl + 2, 0);
+    T.dvd_ptr.assign(nl + 1, 0);
     T.msrc_ptr.assign(1, 0);
-    std::vector<uint32_t> touched;
+    std::vector<uint32_t> stamp(nnz, UINT32_MAX), cnt(nnz, 0), slot(nnz, 0), touched;
     for (int l = 0; l < nl; ++l) {
+        T.upd_ptr[l] = (uint32_t)T.upd.size();
         T.ment_ptr[l] = (uint32_t)T.ment.size();
-        touched.clear();
+        T.dvd_ptr[l] = (uint32_t)T.dvd.size();
         auto each_update = [&](auto f) {
             for (int t = P.level_ptr[l]; t < P.level_ptr[l + 1]; ++t) {
                 const uint32_t k = (uint32_t)P.level_idx[t];
-                const uint32_t L = T.lcnt[k];
+                const uint32_t dpos = P.l_col_ptr[k];
+                const uint32_t L = P.sym_c_ptr[k + 1] - dpos - 1;
                 if (L == 0) continue;
-                const uint32_t U = P.csr_r_ptr[k + 1] - T.ufirst[k];
-                for (uint32_t jj = 0; jj < U; ++jj)
-                    for (uint32_t ii = 0; ii < L; ++ii) f(k, jj, ii, T.moff[k] + (uint64_t)jj * L + ii);
+                const uint32_t uf = P.csr_diag_ptr[k] + 1;
+                const uint32_t U = P.csr_r_ptr[k + 1] - uf;
+                for (uint32_t jj = 0; jj < U; ++jj) {
+                    const uint32_t up = T.upos[uf + jj];
+                    const uint64_t base = moff[k] + (uint64_t)jj * L;
+                    for (uint32_t ii = 0; ii < L; ++ii) f(umap[base + ii], dpos + 1 + ii, up, dpos);
+                }
             }
         };
-        each_update([&](uint32_t, uint32_t, uint32_t, uint64_t m) {
-            const uint32_t tgt = T.umap[m];
+        touched.clear();
+        each_update([&](uint32_t tgt, uint32_t, uint32_t, uint32_t) {
             if (stamp[tgt] != (uint32_t)l) {
                 stamp[tgt] = (uint32_t)l;
                 cnt[tgt] = 0;
@@ -420,119 +486,50 @@ void build_merges(const glu_pattern &P, Tables &T)
             ++cnt[tgt];
         });
         std::sort(touched.begin(), touched.end());
+        const uint32_t first = (uint32_t)T.ment.size();
         for (uint32_t tgt : touched) {
             if (cnt[tgt] < 2) continue;
             slot[tgt] = (uint32_t)T.ment.size();
             T.ment.push_back(tgt);
+            T.msrc_ptr.push_back(T.msrc_ptr.back() + cnt[tgt]);
         }
-        const uint32_t first = T.ment_ptr[l];
-        const uint32_t count = (uint32_t)T.ment.size() - first;
-        // per-entry source ranges, filled in column order
-        std::vector<uint32_t> fillpos(count);
-        for (uint32_t e = 0; e < count; ++e) {
-            T.msrc_ptr.push_back(T.msrc_ptr.back() + cnt[T.ment[first + e]]);
-            fillpos[e] = T.msrc_ptr[first + e];
-        }
+        std::vector<uint32_t> fillpos(T.msrc_ptr.begin() + first, T.msrc_ptr.end() - 1);
         T.msrc.resize(T.msrc_ptr.back());
-        each_update([&](uint32_t k, uint32_t jj, uint32_t ii, uint64_t m) {
-            const uint32_t tgt = T.umap[m] & ~kMergedBit;
-            if (cnt[tgt] < 2) return;
-            const uint32_t dpos = P.l_col_ptr[k];
-            T.msrc[fillpos[slot[tgt] - first]++] = MergeSrc{dpos + 1 + ii, T.upos[T.ufirst[k] + jj], dpos, 0};
-            T.umap[m] |= kMergedBit;
+        each_update([&](uint32_t tgt, uint32_t lpos, uint32_t up, uint32_t dpos) {
+            if (cnt[tgt] >= 2)
+                T.msrc[fillpos[slot[tgt] - first]++] = MergeSrc{lpos, up, dpos, 0};
+            else
+                T.upd.push_back(make_uint4(tgt, lpos, up, dpos));
         });
-        (void)n;
-    }
-    T.ment_ptr[nl] = T.ment_ptr[nl + 1] = (uint32_t)T.ment.size();
-}
-
-bool build_factor_tables(const glu_pattern &P, Tables &T, int64_t &updates, std::string &err)
-{
-    const uint32_t n = P.n, nnz = P.nnz;
-    T.lcnt.resize(n);
-    T.ufirst.resize(n);
-    T.moff.resize(n + 1);
-    T.upos.resize(nnz);
-    T.csc2csr.resize(nnz);
-    uint64_t off = 0;
-    for (uint32_t k = 0; k < n; ++k) {
-        T.lcnt[k] = P.sym_c_ptr[k + 1] - P.l_col_ptr[k] - 1;
-        T.ufirst[k] = P.csr_diag_ptr[k] + 1;
-        const uint32_t U = P.csr_r_ptr[k + 1] - T.ufirst[k];
-        T.moff[k] = off;
-        off += (uint64_t)T.lcnt[k] * U;
-    }
-    T.moff[n] = off;
-    updates = (int64_t)off;
-    {
-        std::vector<uint32_t> cursor(P.csr_r_ptr, P.csr_r_ptr + n);
-        for (uint32_t c = 0; c < n; ++c)
-            for (uint32_t p = P.sym_c_ptr[c]; p < P.sym_c_ptr[c + 1]; ++p) {
-                const uint32_t q = cursor[P.sym_r_idx[p]]++;
-                T.upos[q] = p;
-                T.csc2csr[p] = q;
-            }
-    }
-    // Scatter map, built per target column j: for every k with U(k,j) != 0
-    // and |L_k| > 0, the positions of L_k's rows inside column j.
-    try {
-        T.umap.assign(off, 0);
-    } catch (const std::bad_alloc &) {
-        err = "scatter map too large for host memory";
-        return false;
-    }
-    std::vector<uint32_t> stamp(n, UINT32_MAX), where(n);
-    for (uint32_t j = 0; j < n; ++j) {
-        for (uint32_t p = P.sym_c_ptr[j]; p < P.sym_c_ptr[j + 1]; ++p) {
-            stamp[P.sym_r_idx[p]] = j;
-            where[P.sym_r_idx[p]] = p;
-        }
-        for (uint32_t p = P.sym_c_ptr[j]; p < P.l_col_ptr[j]; ++p) {
-            const uint32_t k = P.sym_r_idx[p];
-            const uint32_t L = T.lcnt[k];
-            if (L == 0) continue;
-            const uint32_t jj = T.csc2csr[p] - T.ufirst[k];
-            uint64_t base = T.moff[k] + (uint64_t)jj * L;
-            const uint32_t l0 = P.l_col_ptr[k] + 1;
-            for (uint32_t ii = 0; ii < L; ++ii) {
-                const uint32_t i = P.sym_r_idx[l0 + ii];
-                if (stamp[i] != j) {
-                    err = "pattern is not closed under fill: (" + std::to_string(i) + "," + std::to_string(j) +
-                          ") missing";
-                    return false;
-                }
-                T.umap[base + ii] = where[i];
-            }
-        }
-    }
-    // Work units and deferred write-backs, level by level.
-    const int nl = P.num_lev;
-    T.unit_ptr.assign(nl + 2, 0);
-    T.dcol_ptr.assign(nl + 1, 0);
-    for (int l = 0; l < nl; ++l) {
-        T.unit_ptr[l] = (uint32_t)T.units.size();
-        T.dcol_ptr[l] = (uint32_t)T.dcols.size();
         for (int t = P.level_ptr[l]; t < P.level_ptr[l + 1]; ++t) {
             const uint32_t k = (uint32_t)P.level_idx[t];
-            const uint32_t L = T.lcnt[k];
-            if (L == 0) continue;
-            T.dcols.push_back(k);
-            const uint32_t U = P.csr_r_ptr[k + 1] - T.ufirst[k];
-            const uint32_t chunk = std::max<uint32_t>(1, kUnitTarget / L);
-            for (uint32_t jb = 0; jb < U; jb += chunk) T.units.push_back(Unit{k, jb, std::min(chunk, U - jb), 0});
+            const uint32_t dpos = P.l_col_ptr[k];
+            for (uint32_t p = dpos + 1; p < P.sym_c_ptr[k + 1]; ++p)
+                T.dvd.push_back(make_uint2(p, dpos | (p == dpos + 1 ? kFirstBit : 0u)));
         }
     }
-    T.unit_ptr[nl] = T.unit_ptr[nl + 1] = (uint32_t)T.units.size();
-    T.dcol_ptr[nl] = (uint32_t)T.dcols.size();
-    build_merges(P, T);
+    T.upd_ptr[nl] = T.upd_ptr[nl + 1] = (uint32_t)T.upd.size();
+    T.ment_ptr[nl] = T.ment_ptr[nl + 1] = (uint32_t)T.ment.size();
+    T.dvd_ptr[nl] = (uint32_t)T.dvd.size();
+    if (T.upd.size() >= UINT32_MAX || T.msrc.size() >= UINT32_MAX) {
+        err = "more than 2^32 update items";
+        return false;
+    }
     return true;
 }
 
 // Level sets of one triangular sweep.  dir > 0: rows ascending, dependencies
 // on smaller indices; dir < 0: descending.  deps(r, f) calls f(c) for each
 // dependency c of row r.
-template <typename Deps>
-void tri_levels(uint32_t n, int dir, Deps deps, TriHost &H)
+// Level sets of one triangular sweep.  dir > 0: rows ascending, dependencies
+// on smaller indices; dir < 0: descending.  deps(r, f) calls f(c) for each
+// dependency c of row r; len(r) is the row's entry count.  Within a level,
+// rows longer than kLongRow come first (lvl_long of them), then the rest,
+// each group in increasing row order.
+constexpr uint32_t kLongRow = 64;
+
+template <typename Deps, typename Len>
+void tri_levels(uint32_t n, int dir, Deps deps, Len len, TriHost &H)
 {
     std::vector<int> lev(n, 0);
     int nlev = 0;
@@ -544,11 +541,19 @@ void tri_levels(uint32_t n, int dir, Deps deps, TriHost &H)
         nlev = std::max(nlev, m + 2);
     }
     H.lvl_ptr.assign(nlev + 1, 0);
-    for (uint32_t r = 0; r < n; ++r) ++H.lvl_ptr[lev[r] + 1];
+    H.lvl_long.assign(nlev, 0);
+    for (uint32_t r = 0; r < n; ++r) {
+        ++H.lvl_ptr[lev[r] + 1];
+        if (len(r) > kLongRow) ++H.lvl_long[lev[r]];
+    }
     for (int l = 0; l < nlev; ++l) H.lvl_ptr[l + 1] += H.lvl_ptr[l];
     H.rows.assign(n, 0);
-    std::vector<uint32_t> cur(H.lvl_ptr.begin(), H.lvl_ptr.end() - 1);
-    for (uint32_t r = 0; r < n; ++r) H.rows[cur[lev[r]]++] = r;
+    std::vector<uint32_t> cur_long(H.lvl_ptr.begin(), H.lvl_ptr.end() - 1), cur_short(nlev);
+    for (int l = 0; l < nlev; ++l) cur_short[l] = H.lvl_ptr[l] + H.lvl_long[l];
+    for (uint32_t r = 0; r < n; ++r) {
+        if (len(r) > kLongRow) H.rows[cur_long[lev[r]]++] = r;
+        else H.rows[cur_short[lev[r]]++] = r;
+    }
 }
 
 void make_segments(const std::vector<uint32_t> &work, uint32_t narrow, std::vector<Segment> &segs)
@@ -569,15 +574,19 @@ void make_segments(const std::vector<uint32_t> &work, uint32_t narrow, std::vect
     }
 }
 
+// Lanes a level needs: update items count a quarter each (4 per lane).
+uint32_t level_lanes(const Tables &T, int l)
+{
+    const uint32_t nu = T.upd_ptr[l + 1] - T.upd_ptr[l];
+    const uint32_t nm = T.ment_ptr[l + 1] - T.ment_ptr[l];
+    const uint32_t nd = l > 0 ? T.dvd_ptr[l] - T.dvd_ptr[l - 1] : 0;
+    return std::max({(nu + 3) / 4, nm, nd});
+}
+
 void factor_segments(glu_plan &p, const Tables &T)
 {
     std::vector<uint32_t> work(p.num_lev + 1);
-    for (int l = 0; l <= p.num_lev; ++l) {
-        uint32_t items = T.unit_ptr[l + 1] - T.unit_ptr[l];
-        items += (T.ment_ptr[l + 1] - T.ment_ptr[l] + 63) / 64;
-        if (l > 0) items += T.dcol_ptr[l] - T.dcol_ptr[l - 1];
-        work[l] = items;
-    }
+    for (int l = 0; l <= p.num_lev; ++l) work[l] = level_lanes(T, l);
     make_segments(work, (uint32_t)p.fuse_units, p.fsegs);
 }
 
@@ -644,22 +653,22 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(GLU_ERR_DEVICE, "no HIP device");
     HIP_TRY(hipGetDevice(&p->device));
     factor_segments(*p, T);
-    p->units = (int64_t)T.units.size();
+    p->units = (int64_t)T.upd.size();
 
     const uint32_t n = P.n;
     // Solve schedules.
     tri_levels(n, +1, [&](uint32_t r, auto f) {
         for (uint32_t q = P.csr_r_ptr[r]; q < P.csr_diag_ptr[r]; ++q) f(P.csr_c_idx[q]);
-    }, p->tri[0]);
+    }, [&](uint32_t r) { return P.csr_diag_ptr[r] - P.csr_r_ptr[r]; }, p->tri[0]);
     tri_levels(n, -1, [&](uint32_t r, auto f) {
         for (uint32_t q = P.csr_diag_ptr[r] + 1; q < P.csr_r_ptr[r + 1]; ++q) f(P.csr_c_idx[q]);
-    }, p->tri[1]);
+    }, [&](uint32_t r) { return P.csr_r_ptr[r + 1] - P.csr_diag_ptr[r] - 1; }, p->tri[1]);
     tri_levels(n, +1, [&](uint32_t r, auto f) {
         for (uint32_t q = P.sym_c_ptr[r]; q < P.l_col_ptr[r]; ++q) f(P.sym_r_idx[q]);
-    }, p->tri[2]);
+    }, [&](uint32_t r) { return P.l_col_ptr[r] - P.sym_c_ptr[r]; }, p->tri[2]);
     tri_levels(n, -1, [&](uint32_t r, auto f) {
         for (uint32_t q = P.l_col_ptr[r] + 1; q < P.sym_c_ptr[r + 1]; ++q) f(P.sym_r_idx[q]);
-    }, p->tri[3]);
+    }, [&](uint32_t r) { return P.sym_c_ptr[r + 1] - P.l_col_ptr[r] - 1; }, p->tri[3]);
     for (auto &h : p->tri) tri_segments(*p, h);
 
     // Refinement / refresh tables from the analysis.
@@ -701,23 +710,20 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
     const size_t o_crptr = U.put(P.csr_r_ptr, n + 1);
     const size_t o_ccidx = U.put(P.csr_c_idx, P.nnz);
     const size_t o_cdiag = U.put(P.csr_diag_ptr, n);
-    const size_t o_lcnt = U.put(T.lcnt);
-    const size_t o_ufirst = U.put(T.ufirst);
-    const size_t o_moff = U.put(T.moff);
     const size_t o_upos = U.put(T.upos);
-    const size_t o_umap = U.put(T.umap);
-    const size_t o_units = U.put(T.units);
-    const size_t o_uptr = U.put(T.unit_ptr);
-    const size_t o_dcols = U.put(T.dcols);
-    const size_t o_dptr = U.put(T.dcol_ptr);
+    const size_t o_upd = U.put(T.upd);
+    const size_t o_uptr = U.put(T.upd_ptr);
     const size_t o_ment = U.put(T.ment);
     const size_t o_mptr = U.put(T.ment_ptr);
     const size_t o_msptr = U.put(T.msrc_ptr);
     const size_t o_msrc = U.put(T.msrc);
-    size_t o_trows[4], o_tptr[4];
+    const size_t o_dvd = U.put(T.dvd);
+    const size_t o_dptr = U.put(T.dvd_ptr);
+    size_t o_trows[4], o_tptr[4], o_tlong[4];
     for (int s = 0; s < 4; ++s) {
         o_trows[s] = U.put(p->tri[s].rows);
         o_tptr[s] = U.put(p->tri[s].lvl_ptr);
+        o_tlong[s] = U.put(p->tri[s].lvl_long);
     }
     size_t o_rp = 0, o_rpi = 0, o_cp = 0, o_cpi = 0, o_rs = 0, o_csp = 0;
     size_t o_arptr = 0, o_acidx = 0, o_asrc = 0, o_acptr = 0, o_aridx = 0, o_lusrc = 0, o_rscol = 0;
@@ -753,20 +759,15 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
     auto u32p = [&](size_t o) { return reinterpret_cast<const uint32_t *>(A + o); };
     FactorDev &F = p->F;
     F.col_ptr = u32p(o_colptr);
-    F.diag = u32p(o_diag);
-    F.lcnt = u32p(o_lcnt);
-    F.ufirst = u32p(o_ufirst);
-    F.moff = reinterpret_cast<const uint64_t *>(A + o_moff);
-    F.upos = u32p(o_upos);
-    F.umap = u32p(o_umap);
-    F.units = reinterpret_cast<const Unit *>(A + o_units);
-    F.unit_ptr = u32p(o_uptr);
-    F.dcols = u32p(o_dcols);
-    F.dcol_ptr = u32p(o_dptr);
+    F.upd = reinterpret_cast<const uint4 *>(A + o_upd);
+    F.upd_ptr = u32p(o_uptr);
     F.ment = u32p(o_ment);
     F.ment_ptr = u32p(o_mptr);
     F.msrc_ptr = u32p(o_msptr);
     F.msrc = reinterpret_cast<const MergeSrc *>(A + o_msrc);
+    F.dvd = reinterpret_cast<const uint2 *>(A + o_dvd);
+    F.dvd_ptr = u32p(o_dptr);
+    const uint32_t *upos_d = u32p(o_upos);
     F.num_lev = P.num_lev;
     F.n = n;
     F.nnz = P.nnz;
@@ -783,14 +784,15 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
         t.desc = desc;
         t.rows = u32p(o_trows[s]);
         t.lvl_ptr = u32p(o_tptr[s]);
+        t.lvl_long = u32p(o_tlong[s]);
         t.nlev = (int)p->tri[s].lvl_ptr.size() - 1;
         return t;
     };
     const uint32_t *colptr = u32p(o_colptr), *rowidx = u32p(o_rowidx), *diag = u32p(o_diag);
     const uint32_t *crptr = u32p(o_crptr), *ccidx = u32p(o_ccidx), *cdiag = u32p(o_cdiag);
     SolveDev &S = p->S;
-    S.lo = tri(0, crptr, 0, cdiag, ccidx, F.upos, nullptr, 0);
-    S.up = tri(1, cdiag, 1, crptr + 1, ccidx, F.upos, diag, 1);
+    S.lo = tri(0, crptr, 0, cdiag, ccidx, upos_d, nullptr, 0);
+    S.up = tri(1, cdiag, 1, crptr + 1, ccidx, upos_d, diag, 1);
     S.tup = tri(2, colptr, 0, diag, rowidx, nullptr, diag, 0);
     S.tlo = tri(3, diag, 1, colptr + 1, rowidx, nullptr, nullptr, 0);
     if (an) {
@@ -1055,8 +1057,10 @@ extern "C" glu_status glu_batch_solve(glu_plan *p, int batch, const double *d_ax
     if (!p->has_refine) return fail(GLU_ERR_ARGUMENT, "plan has no analysis");
     glu_status s = ensure_batch(p, batch);
     if (s != GLU_OK) return s;
+    const size_t lds = sizeof(float) * p->n;
+    const size_t use = (launch::enable_lds(lds) >= lds) ? lds : 0;
     launch::batch_same(p->d_desc, batch, d_ax, d_b, d_x, d_lu, p->d_bt32, p->d_br64, d_iters, d_eta, tol, maxit,
-                       (int)(flags & 1u), p->fused_threads, static_cast<hipStream_t>(stream));
+                       (int)(flags & 1u), p->fused_threads, use, static_cast<hipStream_t>(stream));
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(GLU_ERR_DEVICE, std::string("batch launch: ") + hipGetErrorString(e));
     return GLU_OK;
@@ -1152,7 +1156,11 @@ extern "C" glu_status glu_group_solve(glu_group *g, const double *const *d_ax, c
     }
     hipStream_t st = static_cast<hipStream_t>(stream);
     HIP_TRY(hipMemcpyAsync(g->d_ios, ios.data(), sizeof(MatIOHost) * count, hipMemcpyHostToDevice, st));
-    launch::batch_group(g->d_descs, g->d_ios, g->d_order, count, tol, maxit, (int)(flags & 1u), g->threads, st);
+    size_t nmax = 0;
+    for (glu_plan *p : g->plans) nmax = std::max<size_t>(nmax, p->n);
+    const size_t lds = sizeof(float) * nmax;
+    const size_t use = (launch::enable_lds(lds) >= lds) ? lds : 0;
+    launch::batch_group(g->d_descs, g->d_ios, g->d_order, count, tol, maxit, (int)(flags & 1u), g->threads, use, st);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(GLU_ERR_DEVICE, std::string("group launch: ") + hipGetErrorString(e));
     // ios is read by the kernel from device memory; the host copy may go.

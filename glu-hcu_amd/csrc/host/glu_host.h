@@ -64,6 +64,10 @@ struct Symbolic {
 
 // fill_in + csr + leveling (src/symbolic.cc:11-79,106-151).
 void symbolic(uint32_t n, const uint32_t *ai, const uint32_t *ap, Symbolic &s);
+// The three steps separately (s.n set by the caller for fill_in).
+void symbolic_fill_in(uint32_t n, const uint32_t *ai, const uint32_t *ap, Symbolic &s);
+void symbolic_csr(Symbolic &s);
+void symbolic_leveling(Symbolic &s);
 // predictLU (src/symbolic.cc:82-103): FP64 -> FP32 placement of A into the
 // LU pattern, fill entries zero.  Also returns, for every LU position, the
 // dumped-entry index it came from or UINT32_MAX for fill.

@@ -23,6 +23,13 @@ namespace glu {
 
 void symbolic(uint32_t n, const uint32_t *ai, const uint32_t *ap, Symbolic &s)
 {
+    symbolic_fill_in(n, ai, ap, s);
+    symbolic_csr(s);
+    symbolic_leveling(s);
+}
+
+void symbolic_fill_in(uint32_t n, const uint32_t *ai, const uint32_t *ap, Symbolic &s)
+{
     s.n = n;
     s.sym_c_ptr.assign(1, 0);
     s.sym_r_idx.clear();
@@ -66,8 +73,11 @@ void symbolic(uint32_t n, const uint32_t *ai, const uint32_t *ap, Symbolic &s)
         s.sym_r_idx.insert(s.sym_r_idx.end(), col.begin(), col.end());
     }
     s.nnz = s.sym_c_ptr.back();
+}
 
-    // CSR transpose.
+void symbolic_csr(Symbolic &s)
+{
+    const uint32_t n = s.n;
     s.csr_r_ptr.assign(n + 1, 0);
     for (uint32_t p = 0; p < s.nnz; ++p) ++s.csr_r_ptr[s.sym_r_idx[p] + 1];
     for (uint32_t i = 0; i < n; ++i) s.csr_r_ptr[i + 1] += s.csr_r_ptr[i];
@@ -84,7 +94,11 @@ void symbolic(uint32_t n, const uint32_t *ai, const uint32_t *ap, Symbolic &s)
             }
     }
 
-    // Level sets.
+}
+
+void symbolic_leveling(Symbolic &s)
+{
+    const uint32_t n = s.n;
     std::vector<int> inlevel(n, 0), level_size(n, 0);
     int num_lev = 0;
     for (uint32_t i = 0; i < n; ++i) {
