@@ -29,6 +29,10 @@ struct MergeSrc {
     uint32_t pad;
 };
 
+// Rows / merge entries longer than this are processed by a whole wave.
+constexpr uint32_t kLongRow = 64;
+constexpr uint32_t kLongMerge = 32;
+
 // Flag of a division item: the first L entry of its column, whose lane also
 // stores the GESP pivot.
 constexpr uint32_t kFirstBit = 0x80000000u;
@@ -44,8 +48,10 @@ struct FactorDev {
     const uint32_t *col_ptr;   // sym_c_ptr [n+1] (GESP column sums)
     const uint4 *upd;          // update items, by level
     const uint32_t *upd_ptr;   // [num_lev + 2]
-    const uint32_t *ment;      // merge entries (target CSC positions), by level
+    const uint32_t *ment;      // merge entries (target CSC positions), by level,
+                               // entries with > kLongMerge sources first
     const uint32_t *ment_ptr;  // [num_lev + 2]
+    const uint32_t *ment_long; // long merge entries at the head of each level [num_lev + 1]
     const uint32_t *msrc_ptr;  // per merge entry [entries + 1]
     const MergeSrc *msrc;      // sources in increasing column order
     const uint2 *dvd;          // division items, by level
@@ -88,10 +94,14 @@ struct RefineDev {
     const uint32_t *a_ridx;    // caller CSC row indices [nnz_a]
     const uint32_t *lu_src;    // LU position -> caller value index, or ~0u [nnz_lu]
     const uint32_t *row_idx;   // sym_r_idx [nnz_lu]
+    const uint32_t *lu_col;    // column of each LU position [nnz_lu]
+    const uint32_t *a_rows;    // rows of A, rows with > kLongRow entries first [n]
+    const uint32_t *a_cols;    // columns of A, long ones first [n]
+    uint32_t a_nlong;          // number of long rows
+    uint32_t a_nlong_t;        // number of long columns
     const double *rs_col;      // row_scale[row_perm[c]] per GLU column c [n]
     const double *csp;         // col_scale_perm [n]
     uint32_t nnz_a;
-    uint32_t pad;
 };
 
 struct MatDesc {

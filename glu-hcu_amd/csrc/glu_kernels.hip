@@ -122,18 +122,59 @@ __device__ __forceinline__ void factor_updates(const uint4 *__restrict__ upd, ui
 
 // Merge entry e: every update of this level in increasing column order,
 // t += -(l * u) — the order a sequential pass over the level applies them.
+// Sources are fetched four at a time so the loads overlap; the additions
+// stay sequential.
 __device__ __forceinline__ void factor_merge(const FactorDev &F, float *__restrict__ val, uint32_t e,
                                              float pert, int perturb)
 {
     const uint32_t t = F.ment[e];
     float v = val[t];
     const uint32_t s1 = F.msrc_ptr[e + 1];
-    for (uint32_t s = F.msrc_ptr[e]; s < s1; ++s) {
+    uint32_t s = F.msrc_ptr[e];
+    for (; s + 4 <= s1; s += 4) {
+        float p[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const MergeSrc m = F.msrc[s + u];
+            p[u] = (val[m.lpos] / gesp(val[m.dpos], pert, perturb)) * val[m.upos];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v = v + (-p[u]);
+    }
+    for (; s < s1; ++s) {
         const MergeSrc m = F.msrc[s];
-        const float l = val[m.lpos] / gesp(val[m.dpos], pert, perturb);
-        v = v + (-(l * val[m.upos]));
+        v = v + (-((val[m.lpos] / gesp(val[m.dpos], pert, perturb)) * val[m.upos]));
     }
     val[t] = v;
+}
+
+// The same for an entry with many sources, by one wave: the products of 64
+// sources at a time in parallel, then lane 0 adds them in source order from
+// the wave's LDS slice (64 floats) — the same arithmetic, sequence included.
+__device__ __forceinline__ void factor_merge_wave(const FactorDev &F, float *__restrict__ val, uint32_t e,
+                                                  float pert, int perturb, uint32_t lane, float *lds)
+{
+    const uint32_t t = F.ment[e];
+    const uint32_t s0 = F.msrc_ptr[e], s1 = F.msrc_ptr[e + 1];
+    float v = val[t];
+    for (uint32_t base = s0; base < s1; base += kWave) {
+        const uint32_t s = base + lane;
+        float p = 0.f;
+        if (s < s1) {
+            const MergeSrc m = F.msrc[s];
+            p = (val[m.lpos] / gesp(val[m.dpos], pert, perturb)) * val[m.upos];
+        }
+        lds[lane] = p;
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        if (lane == 0) {
+            const uint32_t cnt = min((uint32_t)kWave, s1 - base);
+            for (uint32_t i = 0; i < cnt; ++i) v = v + (-lds[i]);
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+    if (lane == 0) val[t] = v;
 }
 
 // Division item of a column of the previous level: L(i,k) /= U(k,k); the
@@ -150,8 +191,7 @@ __device__ __forceinline__ void factor_divide(float *__restrict__ val, uint2 it,
 }
 
 struct LevelSpan {
-    uint32_t u0, nu, m0, nm, d0, nd;
-    __device__ __forceinline__ uint32_t items() const { return nu + nm + nd; }
+    uint32_t u0, nu, m0, nm, ml, d0, nd;
 };
 
 __device__ __forceinline__ LevelSpan level_span(const FactorDev &F, int lvl)
@@ -161,6 +201,7 @@ __device__ __forceinline__ LevelSpan level_span(const FactorDev &F, int lvl)
     s.nu = F.upd_ptr[lvl + 1] - s.u0;
     s.m0 = F.ment_ptr[lvl];
     s.nm = F.ment_ptr[lvl + 1] - s.m0;
+    s.ml = F.ment_long[lvl];
     if (lvl > 0) {
         s.d0 = F.dvd_ptr[lvl - 1];
         s.nd = F.dvd_ptr[lvl] - s.d0;
@@ -170,14 +211,19 @@ __device__ __forceinline__ LevelSpan level_span(const FactorDev &F, int lvl)
     return s;
 }
 
-// Lanes [first, first + stride*...) of one level: updates, merges, divisions.
+// All items of one level for a group of `nthreads` cooperating lanes (a
+// workgroup or the whole grid), `tid` = this lane's index in the group:
+// long merges one wave each, then update items, short merges and divisions
+// one lane each.  lds: this wave's 64-float slice.
 template <int Scope>
 __device__ __forceinline__ void factor_level(const FactorDev &F, float *__restrict__ val, const LevelSpan &s,
-                                             uint32_t first, uint32_t stride, float pert, int perturb)
+                                             uint32_t tid, uint32_t nthreads, float pert, int perturb, float *lds)
 {
-    factor_updates<Scope>(F.upd + s.u0, s.nu, first, stride, val, pert, perturb);
-    for (uint32_t t = first; t < s.nm; t += stride) factor_merge(F, val, s.m0 + t, pert, perturb);
-    for (uint32_t t = first; t < s.nd; t += stride) factor_divide(val, F.dvd[s.d0 + t], pert, perturb);
+    const uint32_t lane = tid & (kWave - 1), w = tid / kWave, nw = nthreads / kWave;
+    for (uint32_t e = w; e < s.ml; e += nw) factor_merge_wave(F, val, s.m0 + e, pert, perturb, lane, lds);
+    factor_updates<Scope>(F.upd + s.u0, s.nu, tid, nthreads, val, pert, perturb);
+    for (uint32_t t = s.ml + tid; t < s.nm; t += nthreads) factor_merge(F, val, s.m0 + t, pert, perturb);
+    for (uint32_t t = tid; t < s.nd; t += nthreads) factor_divide(val, F.dvd[s.d0 + t], pert, perturb);
 }
 
 // GESP: pert = 3.45e-4 * max_c sum_p |val[p]| (FP32 sums in column order).
@@ -203,25 +249,27 @@ __global__ void k_zero_u64(unsigned long long *__restrict__ p, int count)
     if ((int)threadIdx.x < count) p[threadIdx.x] = 0ull;
 }
 
-// One level across the grid.  The grid covers ceil(items / 4) lanes for the
-// update part (4 items per lane) plus one lane per merge and division item;
-// the host sizes it with level_lanes().
-__global__ void k_factor_grid(FactorDev F, float *__restrict__ val, const float *__restrict__ d_pert,
-                              int lvl, int perturb)
+// One level across the grid; the host sizes the grid with the lanes the
+// level needs (update items count a quarter, long merges a wave).
+__global__ void __launch_bounds__(256) k_factor_grid(FactorDev F, float *__restrict__ val,
+                                                     const float *__restrict__ d_pert, int lvl, int perturb)
 {
+    __shared__ float lds[256 / kWave][kWave];
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t stride = gridDim.x * blockDim.x;
     const float pert = perturb ? *d_pert : 0.f;
-    factor_level<__HIP_MEMORY_SCOPE_AGENT>(F, val, level_span(F, lvl), g, stride, pert, perturb);
+    factor_level<__HIP_MEMORY_SCOPE_AGENT>(F, val, level_span(F, lvl), g, gridDim.x * blockDim.x, pert, perturb,
+                                           lds[threadIdx.x / kWave]);
 }
 
 // A run of narrow levels inside one workgroup, a barrier between levels.
-__global__ void k_factor_fused(FactorDev F, float *__restrict__ val, const float *__restrict__ d_pert,
-                               int l0, int l1, int perturb)
+__global__ void __launch_bounds__(1024) k_factor_fused(FactorDev F, float *__restrict__ val,
+                                                       const float *__restrict__ d_pert, int l0, int l1, int perturb)
 {
+    __shared__ float lds[1024 / kWave][kWave];
     const float pert = perturb ? *d_pert : 0.f;
     for (int lvl = l0; lvl < l1; ++lvl) {
-        factor_level<__HIP_MEMORY_SCOPE_AGENT>(F, val, level_span(F, lvl), threadIdx.x, blockDim.x, pert, perturb);
+        factor_level<__HIP_MEMORY_SCOPE_AGENT>(F, val, level_span(F, lvl), threadIdx.x, blockDim.x, pert, perturb,
+                                               lds[threadIdx.x / kWave]);
         __syncthreads();
     }
 }
@@ -250,16 +298,68 @@ __device__ __forceinline__ void tri_row(const TriDev &T, const float *__restrict
     x[r] = acc;
 }
 
-__global__ void k_tri_grid(TriDev T, const float *__restrict__ lu, float *__restrict__ x, int lvl)
+// A long row by one wave, with the reference's arithmetic: products of 64
+// entries at a time in parallel (each rounded, as in the sequential loop),
+// then lane 0 subtracts them in the row's order (ascending, or descending
+// for the U sweep) from the wave's LDS slice.
+__device__ __forceinline__ void tri_row_wave(const TriDev &T, const float *__restrict__ lu,
+                                             float *__restrict__ x, uint32_t r, uint32_t lane, float *lds)
 {
-    const uint32_t t = T.lvl_ptr[lvl] + blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t b = T.beg[r] + T.begoff, e = T.end[r];
+    float acc = x[r];
+    const uint32_t nchunk = (e - b + kWave - 1) / kWave;
+    for (uint32_t c = 0; c < nchunk; ++c) {
+        // ascending: chunk c covers [b + 64c, ...); descending: from the top
+        const uint32_t lo = T.desc ? (e - b > (c + 1) * kWave ? e - (c + 1) * kWave : b) : b + c * kWave;
+        const uint32_t hi = T.desc ? e - c * kWave : min(e, lo + kWave);
+        const uint32_t q = lo + lane;
+        float p = 0.f;
+        if (q < hi) p = lu[T.vpos ? T.vpos[q] : q] * x[T.idx[q]];
+        lds[lane] = p;
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        if (lane == 0) {
+            const uint32_t cnt = hi - lo;
+            if (!T.desc)
+                for (uint32_t i = 0; i < cnt; ++i) acc = acc - lds[i];
+            else
+                for (uint32_t i = cnt; i > 0; --i) acc = acc - lds[i - 1];
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+    if (lane == 0) {
+        if (T.dpos) acc = acc / lu[T.dpos[r]];
+        x[r] = acc;
+    }
+}
+
+// One level across the grid: waves [0, nlong) take the level's long rows,
+// the remaining lanes one short row each.
+__global__ void __launch_bounds__(256) k_tri_grid(TriDev T, const float *__restrict__ lu, float *__restrict__ x,
+                                                  int lvl)
+{
+    __shared__ float lds[256 / kWave][kWave];
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t r0 = T.lvl_ptr[lvl], nlong = T.lvl_long[lvl];
+    const uint32_t w = g / kWave;
+    if (w < nlong) {
+        tri_row_wave(T, lu, x, T.rows[r0 + w], g & (kWave - 1), lds[threadIdx.x / kWave]);
+        return;
+    }
+    const uint32_t t = r0 + nlong + (g - nlong * kWave);
     if (t < T.lvl_ptr[lvl + 1]) tri_row(T, lu, x, T.rows[t]);
 }
 
-__global__ void k_tri_fused(TriDev T, const float *__restrict__ lu, float *__restrict__ x, int l0, int l1)
+__global__ void __launch_bounds__(1024) k_tri_fused(TriDev T, const float *__restrict__ lu, float *__restrict__ x,
+                                                    int l0, int l1)
 {
+    __shared__ float lds[1024 / kWave][kWave];
+    const uint32_t lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave, nw = blockDim.x / kWave;
     for (int lvl = l0; lvl < l1; ++lvl) {
-        for (uint32_t t = T.lvl_ptr[lvl] + threadIdx.x; t < T.lvl_ptr[lvl + 1]; t += blockDim.x)
+        const uint32_t r0 = T.lvl_ptr[lvl], nlong = T.lvl_long[lvl];
+        for (uint32_t i = w; i < nlong; i += nw) tri_row_wave(T, lu, x, T.rows[r0 + i], lane, lds[w]);
+        for (uint32_t t = r0 + nlong + threadIdx.x; t < T.lvl_ptr[lvl + 1]; t += blockDim.x)
             tri_row(T, lu, x, T.rows[t]);
         __syncthreads();
     }
@@ -292,48 +392,79 @@ __global__ void k_update_x(const float *__restrict__ t, double *__restrict__ x,
 // refresh, residual, norms
 // ---------------------------------------------------------------------------
 
-// lu[p] = float((ax[src[p]] * rs_col[c]) * csp[row[p]]), 0 for fill.
-__device__ __forceinline__ void refresh_col(const RefineDev &R, const uint32_t *__restrict__ col_ptr,
-                                            const double *__restrict__ ax, float *__restrict__ lu,
-                                            uint32_t c)
+// lu[p] = float((ax[src[p]] * rs_col[col[p]]) * csp[row[p]]), 0 for fill.
+__device__ __forceinline__ void refresh_entry(const RefineDev &R, const double *__restrict__ ax,
+                                              float *__restrict__ lu, uint32_t p)
 {
-    const double rsc = R.rs_col[c];
-    for (uint32_t p = col_ptr[c]; p < col_ptr[c + 1]; ++p) {
-        const uint32_t s = R.lu_src[p];
-        lu[p] = (s == 0xffffffffu) ? 0.f : (float)((ax[s] * rsc) * R.csp[R.row_idx[p]]);
-    }
+    const uint32_t s = R.lu_src[p];
+    lu[p] = (s == 0xffffffffu) ? 0.f : (float)((ax[s] * R.rs_col[R.lu_col[p]]) * R.csp[R.row_idx[p]]);
 }
 
-__global__ void k_refresh(RefineDev R, const uint32_t *__restrict__ col_ptr, const double *__restrict__ ax,
-                          float *__restrict__ lu, uint32_t n, uint32_t nnz_lu, int batch)
+__global__ void k_refresh(RefineDev R, const double *__restrict__ ax, float *__restrict__ lu, uint32_t nnz_lu,
+                          int batch)
 {
     const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= (uint64_t)n * batch) return;
-    const uint32_t b = (uint32_t)(g / n), c = (uint32_t)(g % n);
-    refresh_col(R, col_ptr, ax + (uint64_t)b * R.nnz_a, lu + (uint64_t)b * nnz_lu, c);
+    if (g >= (uint64_t)nnz_lu * batch) return;
+    const uint32_t b = (uint32_t)(g / nnz_lu), p = (uint32_t)(g % nnz_lu);
+    refresh_entry(R, ax + (uint64_t)b * R.nnz_a, lu + (uint64_t)b * nnz_lu, p);
 }
 
-// Row i of A (transpose=0, CSR) or of A' (transpose=1, the caller's CSC).
-__device__ __forceinline__ double a_row_dot(const RefineDev &R, const double *__restrict__ ax,
-                                            const double *__restrict__ x, uint32_t i, int transpose,
-                                            double acc, double *abs_sum)
+__device__ __forceinline__ double wave_sum64(double v)
 {
-    double s = 0.0;
-    if (!transpose) {
-        for (uint32_t q = R.a_rptr[i]; q < R.a_rptr[i + 1]; ++q) {
-            const double a = ax[R.a_src[q]];
-            acc = fma(-a, x[R.a_cidx[q]], acc);
-            s += fabs(a);
-        }
-    } else {
-        for (uint32_t q = R.a_cptr[i]; q < R.a_cptr[i + 1]; ++q) {
-            const double a = ax[q];
-            acc = fma(-a, x[R.a_ridx[q]], acc);
-            s += fabs(a);
-        }
+    for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+// Entries [q0, q1) of row i of A (transpose = 0: the CSR of A) or of A'
+// (transpose = 1: column i of the caller's CSC).
+struct ARow {
+    uint32_t q0, q1;
+};
+
+__device__ __forceinline__ ARow a_row(const RefineDev &R, uint32_t i, int transpose)
+{
+    return transpose ? ARow{R.a_cptr[i], R.a_cptr[i + 1]} : ARow{R.a_rptr[i], R.a_rptr[i + 1]};
+}
+
+// sum_q a_q * x[col_q] and sum_q |a_q| over entries [q0, q0 + step, ...) of row i
+__device__ __forceinline__ void a_row_partial(const RefineDev &R, const double *__restrict__ ax,
+                                              const double *__restrict__ x, ARow row, uint32_t first,
+                                              uint32_t step, int transpose, double &dot, double &asum)
+{
+    dot = 0.0;
+    asum = 0.0;
+    for (uint32_t q = row.q0 + first; q < row.q1; q += step) {
+        const double a = transpose ? ax[q] : ax[R.a_src[q]];
+        if (x) dot = fma(a, x[transpose ? R.a_ridx[q] : R.a_cidx[q]], dot);
+        asum += fabs(a);
     }
-    if (abs_sum) *abs_sum = s;
-    return acc;
+}
+
+// Rows of A for one cooperating group: long rows one wave each (FP64 wave
+// reductions), the rest one lane each.  f(i, dot, asum) is called once per
+// row by one lane.
+template <typename F>
+__device__ __forceinline__ void for_rows_of_a(const RefineDev &R, const double *__restrict__ ax,
+                                              const double *__restrict__ x, uint32_t n, int transpose,
+                                              uint32_t tid, uint32_t nthreads, F f)
+{
+    const uint32_t lane = tid & (kWave - 1), w = tid / kWave, nw = nthreads / kWave;
+    const uint32_t *rows = transpose ? R.a_cols : R.a_rows;
+    const uint32_t nl = transpose ? R.a_nlong_t : R.a_nlong;
+    for (uint32_t t = w; t < nl; t += nw) {
+        const uint32_t i = rows[t];
+        double dot, asum;
+        a_row_partial(R, ax, x, a_row(R, i, transpose), lane, kWave, transpose, dot, asum);
+        dot = wave_sum64(dot);
+        asum = wave_sum64(asum);
+        if (lane == 0) f(i, dot, asum);
+    }
+    for (uint32_t t = nl + tid; t < n; t += nthreads) {
+        const uint32_t i = rows[t];
+        double dot, asum;
+        a_row_partial(R, ax, x, a_row(R, i, transpose), 0, 1, transpose, dot, asum);
+        f(i, dot, asum);
+    }
 }
 
 // norms[0] = |r|_inf, [1] = |x|_inf, [2] = |A|_inf, [3] = |b|_inf (as u64
@@ -344,20 +475,41 @@ __device__ __forceinline__ void umax64(unsigned long long *p, double v)
     atomicMax(p, bits);
 }
 
-__global__ void k_residual(RefineDev R, const double *__restrict__ ax, const double *__restrict__ b,
-                           const double *__restrict__ x, double *__restrict__ r, uint32_t n,
-                           int transpose, int with_anorm, unsigned long long *__restrict__ norms)
+// r = b - A x (or A' x); max norms.  Grid: one wave per long row plus one
+// lane per short row.
+__global__ void __launch_bounds__(256) k_residual(RefineDev R, const double *__restrict__ ax,
+                                                  const double *__restrict__ b, const double *__restrict__ x,
+                                                  double *__restrict__ r, uint32_t n, int transpose, int with_anorm,
+                                                  unsigned long long *__restrict__ norms)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t lane = g & (kWave - 1), w = g / kWave;
+    const uint32_t *rows = transpose ? R.a_cols : R.a_rows;
+    const uint32_t nl = transpose ? R.a_nlong_t : R.a_nlong;
     double rmax = 0.0, xmax = 0.0, amax = 0.0, bmax = 0.0;
-    if (i < n) {
-        double asum;
-        const double ri = a_row_dot(R, ax, x, i, transpose, b[i], with_anorm ? &asum : nullptr);
+    auto take = [&](uint32_t i, double dot, double asum) {
+        const double ri = b[i] - dot;
         r[i] = ri;
-        rmax = fabs(ri);
-        xmax = fabs(x[i]);
-        bmax = fabs(b[i]);
-        if (with_anorm) amax = asum;
+        rmax = nmax(rmax, fabs(ri));
+        xmax = nmax(xmax, fabs(x[i]));
+        bmax = nmax(bmax, fabs(b[i]));
+        amax = nmax(amax, asum);
+    };
+    if (w < nl) {
+        const uint32_t i = rows[w];
+        double dot, asum;
+        a_row_partial(R, ax, x, a_row(R, i, transpose), lane, kWave, transpose, dot, asum);
+        dot = wave_sum64(dot);
+        asum = wave_sum64(asum);
+        if (lane == 0) take(i, dot, asum);
+    } else {
+        const uint32_t t = nl + (g - nl * kWave);
+        if (t < n) {
+            const uint32_t i = rows[t];
+            double dot, asum;
+            a_row_partial(R, ax, x, a_row(R, i, transpose), 0, 1, transpose, dot, asum);
+            take(i, dot, asum);
+        }
     }
     rmax = wave_max(rmax);
     xmax = wave_max(xmax);
@@ -440,7 +592,7 @@ __device__ __forceinline__ void wg_tri_refine(const TriDev &T, const float *__re
 // factorization, then refinement sweeps until eta <= tol or maxit.  xs is
 // the FP32 correction vector (LDS when it fits, else the global scratch).
 __device__ void wg_pipeline(const MatDesc &M, const MatIO &io, double tol, int maxit, int perturb,
-                            double *sh, float *xs)
+                            double *sh, float *xs, float *wlds)
 {
     const uint32_t n = M.n;
     const uint32_t tid = threadIdx.x, nt = blockDim.x;
@@ -449,12 +601,10 @@ __device__ void wg_pipeline(const MatDesc &M, const MatIO &io, double tol, int m
     const RefineDev &R = M.r;
 
     // 1. values: scale + place A into the LU pattern; |A|_inf and |b|_inf
+    for (uint32_t p = tid; p < M.nnz; p += nt) refresh_entry(R, io.ax, io.lu, p);
     double amax = 0.0, bmax = 0.0;
+    for_rows_of_a(R, io.ax, nullptr, n, 0, tid, nt, [&](uint32_t, double, double asum) { amax = nmax(amax, asum); });
     for (uint32_t c = tid; c < n; c += nt) {
-        refresh_col(R, F.col_ptr, io.ax, io.lu, c);
-        double s = 0.0;
-        for (uint32_t q = R.a_rptr[c]; q < R.a_rptr[c + 1]; ++q) s += fabs(io.ax[R.a_src[q]]);
-        amax = nmax(amax, s);
         bmax = nmax(bmax, fabs(io.b[c]));
         io.x[c] = 0.0;
         io.r64[c] = io.b[c];
@@ -476,7 +626,7 @@ __device__ void wg_pipeline(const MatDesc &M, const MatIO &io, double tol, int m
 
     // 3. factorization, level by level
     for (int lvl = 0; lvl <= F.num_lev; ++lvl) {
-        factor_level<__HIP_MEMORY_SCOPE_WORKGROUP>(F, io.lu, level_span(F, lvl), tid, nt, pert, perturb);
+        factor_level<__HIP_MEMORY_SCOPE_WORKGROUP>(F, io.lu, level_span(F, lvl), tid, nt, pert, perturb, wlds);
         __syncthreads();
     }
 
@@ -493,12 +643,12 @@ __device__ void wg_pipeline(const MatDesc &M, const MatIO &io, double tol, int m
             io.x[j] = io.x[j] + (double)xs[S.row_perm_inv[j]] * S.row_scale[j];
         __syncthreads();
         double rmax = 0.0, xmax = 0.0;
-        for (uint32_t i = tid; i < n; i += nt) {
-            const double ri = a_row_dot(R, io.ax, io.x, i, 0, io.b[i], nullptr);
+        for_rows_of_a(R, io.ax, io.x, n, 0, tid, nt, [&](uint32_t i, double dot, double) {
+            const double ri = io.b[i] - dot;
             io.r64[i] = ri;
             rmax = nmax(rmax, fabs(ri));
-            xmax = nmax(xmax, fabs(io.x[i]));
-        }
+        });
+        for (uint32_t i = tid; i < n; i += nt) xmax = nmax(xmax, fabs(io.x[i]));
         const double rn = wg_max(rmax, sh);
         const double xn = wg_max(xmax, sh);
         eta = rn / (anorm * xn + bnorm);
@@ -522,6 +672,7 @@ __global__ void __launch_bounds__(1024) k_batch_same(const MatDesc *__restrict__
                              double *__restrict__ eta, double tol, int maxit, int perturb)
 {
     __shared__ double sh[1024 / kWave];
+    __shared__ float wl[1024 / kWave][kWave];
     const MatDesc &M = desc[0];
     const uint64_t m = blockIdx.x;
     MatIO io;
@@ -533,7 +684,7 @@ __global__ void __launch_bounds__(1024) k_batch_same(const MatDesc *__restrict__
     io.r64 = r64 + m * M.n;
     io.iters = iters + m;
     io.eta = eta + m;
-    wg_pipeline(M, io, tol, maxit, perturb, sh, kLds ? g_lds_vec : io.t32);
+    wg_pipeline(M, io, tol, maxit, perturb, sh, kLds ? g_lds_vec : io.t32, wl[threadIdx.x / kWave]);
 }
 
 // Distinct patterns: workgroup g runs matrix order[g] with its own tables.
@@ -542,8 +693,9 @@ __global__ void __launch_bounds__(1024) k_batch_group(const MatDesc *__restrict_
                               const uint32_t *__restrict__ order, double tol, int maxit, int perturb)
 {
     __shared__ double sh[1024 / kWave];
+    __shared__ float wl[1024 / kWave][kWave];
     const uint32_t m = order[blockIdx.x];
-    wg_pipeline(descs[m], ios[m], tol, maxit, perturb, sh, kLds ? g_lds_vec : ios[m].t32);
+    wg_pipeline(descs[m], ios[m], tol, maxit, perturb, sh, kLds ? g_lds_vec : ios[m].t32, wl[threadIdx.x / kWave]);
 }
 
 // ---------------------------------------------------------------------------
@@ -574,9 +726,10 @@ void factor_fused(const FactorDev &F, float *val, const float *d_pert, int l0, i
     k_factor_fused<<<1, threads, 0, st>>>(F, val, d_pert, l0, l1, perturb);
 }
 
-void tri_grid(const TriDev &T, const float *lu, float *x, int lvl, uint32_t rows, hipStream_t st)
+// lanes: long rows x 64 + short rows of the level
+void tri_grid(const TriDev &T, const float *lu, float *x, int lvl, uint32_t lanes, hipStream_t st)
 {
-    k_tri_grid<<<blocks_for(rows, 256), 256, 0, st>>>(T, lu, x, lvl);
+    k_tri_grid<<<blocks_for(lanes, 256), 256, 0, st>>>(T, lu, x, lvl);
 }
 
 void tri_fused(const TriDev &T, const float *lu, float *x, int l0, int l1, int threads, hipStream_t st)
@@ -602,16 +755,16 @@ void update_x(const float *t, double *x, const uint32_t *P, const double *S, int
     k_update_x<<<blocks_for(n, 256), 256, 0, st>>>(t, x, P, S, by_perm, n);
 }
 
-void refresh(const RefineDev &R, const uint32_t *col_ptr, const double *ax, float *lu, uint32_t n,
-             uint32_t nnz_lu, int batch, hipStream_t st)
+void refresh(const RefineDev &R, const double *ax, float *lu, uint32_t nnz_lu, int batch, hipStream_t st)
 {
-    k_refresh<<<blocks_for((uint64_t)n * batch, 256), 256, 0, st>>>(R, col_ptr, ax, lu, n, nnz_lu, batch);
+    k_refresh<<<blocks_for((uint64_t)nnz_lu * batch, 256), 256, 0, st>>>(R, ax, lu, nnz_lu, batch);
 }
 
+// lanes: one wave per long row of A (or A') plus one lane per short row
 void residual(const RefineDev &R, const double *ax, const double *b, const double *x, double *r, uint32_t n,
-              int transpose, int with_anorm, unsigned long long *norms, hipStream_t st)
+              int transpose, int with_anorm, unsigned long long *norms, uint32_t lanes, hipStream_t st)
 {
-    k_residual<<<blocks_for(n, 256), 256, 0, st>>>(R, ax, b, x, r, n, transpose, with_anorm, norms);
+    k_residual<<<blocks_for(lanes, 256), 256, 0, st>>>(R, ax, b, x, r, n, transpose, with_anorm, norms);
 }
 
 // lds_bytes > 0: the correction vector is kept in that much dynamic LDS.
@@ -644,7 +797,7 @@ size_t enable_lds(size_t bytes)
     int dev = 0, lim = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 0;
     if (hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return 0;
-    const size_t avail = (size_t)lim > 1024 ? (size_t)lim - 1024 : 0;  // static LDS of the kernels
+    const size_t avail = (size_t)lim > 8192 ? (size_t)lim - 8192 : 0;  // static LDS of the kernels (< 8 KiB)
     if (bytes > avail) return avail;
     (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_batch_same<true>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);

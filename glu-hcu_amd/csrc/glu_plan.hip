@@ -38,10 +38,9 @@ void perm_scale_f64(const double *in, float *out, const uint32_t *P, const doubl
                     uint32_t n, hipStream_t st);
 void update_x(const float *t, double *x, const uint32_t *P, const double *S, int by_perm, uint32_t n,
               hipStream_t st);
-void refresh(const RefineDev &R, const uint32_t *col_ptr, const double *ax, float *lu, uint32_t n,
-             uint32_t nnz_lu, int batch, hipStream_t st);
+void refresh(const RefineDev &R, const double *ax, float *lu, uint32_t nnz_lu, int batch, hipStream_t st);
 void residual(const RefineDev &R, const double *ax, const double *b, const double *x, double *r, uint32_t n,
-              int transpose, int with_anorm, unsigned long long *norms, hipStream_t st);
+              int transpose, int with_anorm, unsigned long long *norms, uint32_t lanes, hipStream_t st);
 void batch_same(const MatDesc *desc, int batch, const double *ax, const double *b, double *x, float *lu,
                 float *t32, double *r64, int32_t *iters, double *eta, double tol, int maxit, int perturb,
                 int threads, size_t lds_bytes, hipStream_t st);
@@ -236,6 +235,7 @@ struct glu_plan {
     int fused_threads = kDefaultFusedThreads;
     int fuse_units = kDefaultFuseUnits;
     bool has_scaling = false, has_refine = false;
+    uint32_t a_nlong = 0, a_nlong_t = 0;   // long rows / columns of A
 
     std::vector<Segment> fsegs;
     TriHost tri[4];          // lo, up, tup, tlo
@@ -380,7 +380,7 @@ struct Tables {
     std::vector<uint32_t> upos;        // CSR position -> CSC position
     std::vector<uint4> upd;            // update items, by level
     std::vector<uint32_t> upd_ptr;     // [num_lev + 2]
-    std::vector<uint32_t> ment, ment_ptr, msrc_ptr;
+    std::vector<uint32_t> ment, ment_ptr, ment_long, msrc_ptr;
     std::vector<MergeSrc> msrc;
     std::vector<uint2> dvd;            // division items, by level
     std::vector<uint32_t> dvd_ptr;     // [num_lev + 1]
@@ -454,6 +454,7 @@ bool build_factor_tables(const glu_pattern &P, Tables &T, int64_t &updates, std:
     // Items per level.
     T.upd_ptr.assign(nl + 2, 0);
     T.ment_ptr.assign(nl + 2, 0);
+    T.ment_long.assign(nl + 1, 0);
     T.dvd_ptr.assign(nl + 1, 0);
     T.msrc_ptr.assign(1, 0);
     std::vector<uint32_t> stamp(nnz, UINT32_MAX), cnt(nnz, 0), slot(nnz, 0), touched;
@@ -487,12 +488,15 @@ bool build_factor_tables(const glu_pattern &P, Tables &T, int64_t &updates, std:
         });
         std::sort(touched.begin(), touched.end());
         const uint32_t first = (uint32_t)T.ment.size();
-        for (uint32_t tgt : touched) {
-            if (cnt[tgt] < 2) continue;
-            slot[tgt] = (uint32_t)T.ment.size();
-            T.ment.push_back(tgt);
-            T.msrc_ptr.push_back(T.msrc_ptr.back() + cnt[tgt]);
-        }
+        // long merge entries (a wave each) first, then the rest
+        for (int pass = 0; pass < 2; ++pass)
+            for (uint32_t tgt : touched) {
+                if (cnt[tgt] < 2 || (cnt[tgt] > kLongMerge) != (pass == 0)) continue;
+                slot[tgt] = (uint32_t)T.ment.size();
+                T.ment.push_back(tgt);
+                T.msrc_ptr.push_back(T.msrc_ptr.back() + cnt[tgt]);
+                if (pass == 0) ++T.ment_long[l];
+            }
         std::vector<uint32_t> fillpos(T.msrc_ptr.begin() + first, T.msrc_ptr.end() - 1);
         T.msrc.resize(T.msrc_ptr.back());
         each_update([&](uint32_t tgt, uint32_t lpos, uint32_t up, uint32_t dpos) {
@@ -526,7 +530,6 @@ bool build_factor_tables(const glu_pattern &P, Tables &T, int64_t &updates, std:
 // dependency c of row r; len(r) is the row's entry count.  Within a level,
 // rows longer than kLongRow come first (lvl_long of them), then the rest,
 // each group in increasing row order.
-constexpr uint32_t kLongRow = 64;
 
 template <typename Deps, typename Len>
 void tri_levels(uint32_t n, int dir, Deps deps, Len len, TriHost &H)
@@ -574,13 +577,15 @@ void make_segments(const std::vector<uint32_t> &work, uint32_t narrow, std::vect
     }
 }
 
-// Lanes a level needs: update items count a quarter each (4 per lane).
+// Lanes a level needs: update items count a quarter each (4 per lane),
+// long merge entries a wave each.
 uint32_t level_lanes(const Tables &T, int l)
 {
     const uint32_t nu = T.upd_ptr[l + 1] - T.upd_ptr[l];
     const uint32_t nm = T.ment_ptr[l + 1] - T.ment_ptr[l];
+    const uint32_t ml = T.ment_long[l];
     const uint32_t nd = l > 0 ? T.dvd_ptr[l] - T.dvd_ptr[l - 1] : 0;
-    return std::max({(nu + 3) / 4, nm, nd});
+    return std::max({(nu + 3) / 4, nm - ml, nd, ml * 64});
 }
 
 void factor_segments(glu_plan &p, const Tables &T)
@@ -594,7 +599,10 @@ void tri_segments(glu_plan &p, TriHost &H)
 {
     const int nlev = (int)H.lvl_ptr.size() - 1;
     std::vector<uint32_t> work(nlev);
-    for (int l = 0; l < nlev; ++l) work[l] = H.lvl_ptr[l + 1] - H.lvl_ptr[l];
+    for (int l = 0; l < nlev; ++l) {
+        const uint32_t rows = H.lvl_ptr[l + 1] - H.lvl_ptr[l];
+        work[l] = (rows - H.lvl_long[l]) + H.lvl_long[l] * 64;   // lanes
+    }
     make_segments(work, (uint32_t)p.fused_threads * 4, H.segs);
 }
 
@@ -672,7 +680,7 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
     for (auto &h : p->tri) tri_segments(*p, h);
 
     // Refinement / refresh tables from the analysis.
-    std::vector<uint32_t> a_rptr, a_cidx, a_src, lu_src;
+    std::vector<uint32_t> a_rptr, a_cidx, a_src, lu_src, lu_col, a_rows, a_cols;
     std::vector<double> rs_col;
     const Analysis *an = a ? &a->an : nullptr;
     if (an) {
@@ -699,6 +707,20 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
         }
         rs_col.resize(n);
         for (uint32_t c = 0; c < n; ++c) rs_col[c] = an->row_scale[an->row_perm[c]];
+        lu_col.resize(P.nnz);
+        for (uint32_t c = 0; c < n; ++c)
+            for (uint32_t q = P.sym_c_ptr[c]; q < P.sym_c_ptr[c + 1]; ++q) lu_col[q] = c;
+        // rows / columns of A with more than kLongRow entries first
+        auto long_first = [&](const uint32_t *ptr, std::vector<uint32_t> &order, uint32_t &nlong) {
+            order.clear();
+            for (uint32_t i = 0; i < n; ++i)
+                if (ptr[i + 1] - ptr[i] > kLongRow) order.push_back(i);
+            nlong = (uint32_t)order.size();
+            for (uint32_t i = 0; i < n; ++i)
+                if (ptr[i + 1] - ptr[i] <= kLongRow) order.push_back(i);
+        };
+        long_first(a_rptr.data(), a_rows, p->a_nlong);
+        long_first(an->in_ap.data(), a_cols, p->a_nlong_t);
         p->has_scaling = p->has_refine = true;
     }
 
@@ -715,6 +737,7 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
     const size_t o_uptr = U.put(T.upd_ptr);
     const size_t o_ment = U.put(T.ment);
     const size_t o_mptr = U.put(T.ment_ptr);
+    const size_t o_mlong = U.put(T.ment_long);
     const size_t o_msptr = U.put(T.msrc_ptr);
     const size_t o_msrc = U.put(T.msrc);
     const size_t o_dvd = U.put(T.dvd);
@@ -727,6 +750,7 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
     }
     size_t o_rp = 0, o_rpi = 0, o_cp = 0, o_cpi = 0, o_rs = 0, o_csp = 0;
     size_t o_arptr = 0, o_acidx = 0, o_asrc = 0, o_acptr = 0, o_aridx = 0, o_lusrc = 0, o_rscol = 0;
+    size_t o_lucol = 0, o_arows = 0, o_acols = 0;
     if (an) {
         o_rp = U.put(an->row_perm);
         o_rpi = U.put(an->row_perm_inv);
@@ -741,6 +765,9 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
         o_aridx = U.put(an->in_ai);
         o_lusrc = U.put(lu_src);
         o_rscol = U.put(rs_col);
+        o_lucol = U.put(lu_col);
+        o_arows = U.put(a_rows);
+        o_acols = U.put(a_cols);
     }
     const size_t o_desc = U.reserve(sizeof(MatDesc));
     const size_t o_normbits = U.reserve(sizeof(unsigned int));
@@ -763,6 +790,7 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
     F.upd_ptr = u32p(o_uptr);
     F.ment = u32p(o_ment);
     F.ment_ptr = u32p(o_mptr);
+    F.ment_long = u32p(o_mlong);
     F.msrc_ptr = u32p(o_msptr);
     F.msrc = reinterpret_cast<const MergeSrc *>(A + o_msrc);
     F.dvd = reinterpret_cast<const uint2 *>(A + o_dvd);
@@ -810,6 +838,11 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
         R.a_ridx = u32p(o_aridx);
         R.lu_src = u32p(o_lusrc);
         R.row_idx = rowidx;
+        R.lu_col = u32p(o_lucol);
+        R.a_rows = u32p(o_arows);
+        R.a_cols = u32p(o_acols);
+        R.a_nlong = p->a_nlong;
+        R.a_nlong_t = p->a_nlong_t;
         R.rs_col = reinterpret_cast<const double *>(A + o_rscol);
         R.csp = S.col_scale_perm;
         R.nnz_a = p->nnz_a;
@@ -981,7 +1014,7 @@ extern "C" glu_status glu_refresh(glu_plan *p, const double *d_ax, float *d_lu, 
 {
     if (!p || !d_ax || !d_lu || batch <= 0) return fail(GLU_ERR_ARGUMENT, "bad argument");
     if (!p->has_refine) return fail(GLU_ERR_ARGUMENT, "plan has no analysis");
-    launch::refresh(p->R, p->F.col_ptr, d_ax, d_lu, p->n, p->nnz, batch, static_cast<hipStream_t>(stream));
+    launch::refresh(p->R, d_ax, d_lu, p->nnz, batch, static_cast<hipStream_t>(stream));
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(GLU_ERR_DEVICE, std::string("refresh launch: ") + hipGetErrorString(e));
     return GLU_OK;
@@ -1018,7 +1051,8 @@ extern "C" glu_status glu_refine(glu_plan *p, const double *d_ax, const float *d
         else
             launch::update_x(p->d_t32, d_x, S.col_perm_inv, S.col_scale_perm, 1, n, st);
         launch::zero_u64(p->d_norms, 4, st);
-        launch::residual(p->R, d_ax, d_b, d_x, p->d_r64, n, transpose, it == 0, p->d_norms, st);
+        const uint32_t nl = transpose ? p->a_nlong_t : p->a_nlong;
+        launch::residual(p->R, d_ax, d_b, d_x, p->d_r64, n, transpose, it == 0, p->d_norms, (n - nl) + nl * 64, st);
         unsigned long long nb[4];
         HIP_TRY(hipMemcpyAsync(nb, p->d_norms, sizeof nb, hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
