@@ -29,6 +29,8 @@ struct MergeSrc {
     uint32_t pad;
 };
 
+constexpr uint32_t kLongTask = 0x80000000u;
+
 // Rows / merge entries longer than this are processed by a whole wave.
 constexpr uint32_t kLongRow = 64;
 constexpr uint32_t kLongMerge = 32;
@@ -73,6 +75,13 @@ struct TriDev {
     const uint32_t *rows;      // rows grouped by level, long rows first
     const uint32_t *lvl_ptr;   // [nlev + 1]
     const uint32_t *lvl_long;  // long rows at the head of each level [nlev]
+    // Dataflow schedule: tasks in topological order, each a run of <= 64
+    // short rows or one long row (start position | kLongTask); task k covers
+    // positions [dtask[k] & ~kLongTask, dtask[k + 1] & ~kLongTask).  Tasks
+    // never cross a fused-segment boundary; seg_task gives each fused
+    // segment's first task.
+    const uint32_t *dtask;     // [ntask + 1]
+    uint32_t ntask;
     uint32_t begoff;
     int32_t desc;
     int32_t nlev;
@@ -82,6 +91,8 @@ struct TriDev {
 struct SolveDev {
     TriDev lo, up;             // A x = b:   L then U
     TriDev tup, tlo;           // A' x = b:  U' then L'
+    TriDev lo_csr, up_csr;     // L, U reading a CSR-ordered copy of the factors
+    const uint32_t *upos;      // CSR position -> CSC position [nnz] (the copy)
     const uint32_t *row_perm, *row_perm_inv, *col_perm, *col_perm_inv;
     const double *row_scale, *col_scale_perm;
 };

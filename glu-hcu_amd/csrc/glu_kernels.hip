@@ -365,6 +365,205 @@ __global__ void __launch_bounds__(1024) k_tri_fused(TriDev T, const float *__res
     }
 }
 
+// ---------------------------------------------------------------------------
+// dataflow (barrier-free) triangular sweeps inside one workgroup
+// ---------------------------------------------------------------------------
+//
+// Rows are taken in the topological (level) order of T.rows, grouped into
+// tasks of up to 64 short rows or one long row; wave w runs tasks w, w + nw,
+// ... in order.  A row waits for each dependency's ready bit (LDS) before
+// reading its value, so no level barrier is needed.  Deadlock freedom: the
+// first unfinished task always has all its dependencies in earlier
+// (finished) tasks or inside itself, and its wave is at it; lanes of one
+// task poll cooperatively (a wave-uniform loop), so a row blocked on a row
+// of the same task never stops that row from completing.
+
+__device__ __forceinline__ float wave_sum(float v)
+{
+    for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+extern __shared__ float g_lds_vec[];   // dynamic LDS of the kernels below
+
+// Ready bit of row c.  df_peek is a relaxed read: a caller that reads x
+// after it issues an acquire fence first.
+__device__ __forceinline__ bool df_peek(const uint32_t *flags, uint32_t c)
+{
+    return (__hip_atomic_load(flags + (c >> 5), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >> (c & 31u)) & 1u;
+}
+
+__device__ __forceinline__ bool df_ready(const uint32_t *flags, uint32_t c)
+{
+    return (__hip_atomic_load(flags + (c >> 5), __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) >> (c & 31u)) & 1u;
+}
+
+__device__ __forceinline__ void df_set(uint32_t *flags, uint32_t r)
+{
+    __hip_atomic_fetch_or(flags + (r >> 5), 1u << (r & 31u), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Up to 64 short rows, one per lane: each lane prefetches its row's entries
+// 8 at a time and consumes them in the row's order (ascending, or
+// descending for the U sweep) as they become ready — the sequential
+// arithmetic of tri_row.
+__device__ __forceinline__ void df_short_task(const TriDev &T, const float *__restrict__ vals, float *x,
+                                              uint32_t *flags, uint32_t p0, uint32_t p1, uint32_t lane)
+{
+    constexpr uint32_t K = 8;
+    const uint32_t pos = p0 + lane;
+    bool pending = pos < p1;
+    uint32_t r = 0, q = 0, qend = 0;
+    float acc = 0.f;
+    if (pending) {
+        r = T.rows[pos];
+        const uint32_t b = T.beg[r] + T.begoff, e = T.end[r];
+        acc = x[r];
+        q = T.desc ? e : b;
+        qend = T.desc ? b : e;
+    }
+    uint32_t cidx[K];
+    float cval[K];
+    uint32_t cn = 0, ci = 0;
+    while (__ballot(pending)) {
+        bool progressed = false;
+        if (pending) {
+            if (ci == cn && q != qend) {
+                const uint32_t left = T.desc ? q - qend : qend - q;
+                cn = left < K ? left : K;
+                ci = 0;
+#pragma unroll
+                for (uint32_t j = 0; j < K; ++j)
+                    if (j < cn) {
+                        const uint32_t qq = T.desc ? q - 1 - j : q + j;
+                        cidx[j] = T.idx[qq];
+                        cval[j] = vals[T.vpos ? T.vpos[qq] : qq];
+                    }
+                q = T.desc ? q - cn : q + cn;
+            }
+            // ready bits of the chunk's unconsumed entries (independent
+            // relaxed reads), one acquire fence, then the x values, then
+            // in-order consumption of the ready prefix
+            bool rdy[K];
+#pragma unroll
+            for (uint32_t j = 0; j < K; ++j) rdy[j] = j >= ci && j < cn && df_peek(flags, cidx[j]);
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            float xv[K];
+#pragma unroll
+            for (uint32_t j = 0; j < K; ++j) xv[j] = rdy[j] ? x[cidx[j]] : 0.f;
+#pragma unroll
+            for (uint32_t j = 0; j < K; ++j)
+                if (j == ci && rdy[j]) {
+                    acc = acc - cval[j] * xv[j];
+                    ++ci;
+                    progressed = true;
+                }
+            if (ci == cn && q == qend) {
+                if (T.dpos) acc = acc / vals[T.dpos[r]];
+                x[r] = acc;
+                df_set(flags, r);
+                pending = false;
+                progressed = true;
+            }
+        }
+        if (!__ballot(progressed)) __builtin_amdgcn_s_sleep(1);
+    }
+}
+
+// One long row by a wave.  kExact: products of 64 entries in parallel, then
+// lane 0 subtracts them in the row's order (tri_row_wave's arithmetic);
+// otherwise a wave reduction (refinement sweeps, where order is free).
+template <bool kExact>
+__device__ __forceinline__ void df_long_task(const TriDev &T, const float *__restrict__ vals, float *x,
+                                             uint32_t *flags, uint32_t pos, uint32_t lane, float *lds)
+{
+    const uint32_t r = T.rows[pos];
+    const uint32_t b = T.beg[r] + T.begoff, e = T.end[r];
+    if (!kExact) {
+        float sum = 0.f;
+        for (uint32_t q = b + lane; q < e; q += kWave) {
+            const uint32_t c = T.idx[q];
+            const float v = vals[T.vpos ? T.vpos[q] : q];
+            while (!df_ready(flags, c)) __builtin_amdgcn_s_sleep(1);
+            sum += v * x[c];
+        }
+        sum = wave_sum(sum);
+        if (lane == 0) {
+            float acc = x[r] - sum;
+            if (T.dpos) acc = acc / vals[T.dpos[r]];
+            x[r] = acc;
+            df_set(flags, r);
+        }
+        return;
+    }
+    float acc = x[r];
+    const uint32_t nchunk = (e - b + kWave - 1) / kWave;
+    for (uint32_t c = 0; c < nchunk; ++c) {
+        const uint32_t lo = T.desc ? (e - b > (c + 1) * kWave ? e - (c + 1) * kWave : b) : b + c * kWave;
+        const uint32_t hi = T.desc ? e - c * kWave : min(e, lo + kWave);
+        const uint32_t q = lo + lane;
+        float p = 0.f;
+        if (q < hi) {
+            const uint32_t cc = T.idx[q];
+            const float v = vals[T.vpos ? T.vpos[q] : q];
+            while (!df_ready(flags, cc)) __builtin_amdgcn_s_sleep(1);
+            p = v * x[cc];
+        }
+        lds[lane] = p;
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        if (lane == 0) {
+            const uint32_t cnt = hi - lo;
+            if (!T.desc)
+                for (uint32_t i = 0; i < cnt; ++i) acc = acc - lds[i];
+            else
+                for (uint32_t i = cnt; i > 0; --i) acc = acc - lds[i - 1];
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+    if (lane == 0) {
+        if (T.dpos) acc = acc / vals[T.dpos[r]];
+        x[r] = acc;
+        df_set(flags, r);
+    }
+}
+
+// Tasks [k0, k1) of a sweep for the calling workgroup.  The caller has
+// cleared the ready bits of the tasks' rows and set all others.
+template <bool kExact>
+__device__ __forceinline__ void df_sweep(const TriDev &T, const float *__restrict__ vals, float *x, uint32_t *flags,
+                                         uint32_t k0, uint32_t k1, float *lds)
+{
+    const uint32_t lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave, nw = blockDim.x / kWave;
+    for (uint32_t k = k0 + w; k < k1; k += nw) {
+        const uint32_t a = T.dtask[k];
+        if (a & kLongTask)
+            df_long_task<kExact>(T, vals, x, flags, a & ~kLongTask, lane, lds);
+        else
+            df_short_task(T, vals, x, flags, a, T.dtask[k + 1] & ~kLongTask, lane);
+    }
+}
+
+// A run of narrow levels [l0, l1) of one sweep (tasks [k0, k1)) inside one
+// workgroup without level barriers, x in global memory; bit-exact with the
+// reference's sequential row arithmetic.  Dynamic LDS: ceil(n/32) words.
+__global__ void __launch_bounds__(1024) k_tri_df(TriDev T, const float *__restrict__ lu, float *__restrict__ x,
+                                                 int l0, int l1, uint32_t k0, uint32_t k1, uint32_t n)
+{
+    __shared__ float lds[1024 / kWave][kWave];
+    uint32_t *flags = reinterpret_cast<uint32_t *>(g_lds_vec);
+    const uint32_t nwords = (n + 31) / 32;
+    for (uint32_t i = threadIdx.x; i < nwords; i += blockDim.x) flags[i] = 0xffffffffu;
+    __syncthreads();
+    for (uint32_t t = T.lvl_ptr[l0] + threadIdx.x; t < T.lvl_ptr[l1]; t += blockDim.x) {
+        const uint32_t r = T.rows[t];
+        atomicAnd(flags + (r >> 5), ~(1u << (r & 31u)));
+    }
+    __syncthreads();
+    df_sweep<true>(T, lu, x, flags, k0, k1, lds[threadIdx.x / kWave]);
+}
+
 // out[j] = float(double(in[P[j]]) * S[by_perm ? P[j] : j])
 template <typename TIn>
 __global__ void k_perm_scale(const TIn *__restrict__ in, float *__restrict__ out,
@@ -546,15 +745,10 @@ struct MatIO {
     float *lu;          // nnz_lu
     float *t32;         // n scratch
     double *r64;        // n scratch
+    float *lucsr;       // nnz_lu scratch (factors in CSR order)
     int32_t *iters;
     double *eta;
 };
-
-__device__ __forceinline__ float wave_sum(float v)
-{
-    for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
-    return v;
-}
 
 // One triangular sweep of the refinement inside a workgroup (x may be LDS).
 // Rows longer than kLongRow (listed first in each level) take a wave each
@@ -588,11 +782,20 @@ __device__ __forceinline__ void wg_tri_refine(const TriDev &T, const float *__re
     }
 }
 
+// LDS placement of the batch pipeline's sweep state (dynamic LDS).
+enum : int {
+    kSweepLevels = 0,   // level-synchronous sweeps, correction vector in global memory
+    kSweepFlags = 1,    // dataflow sweeps, ready bits in LDS, vector in global memory
+    kSweepLds = 2,      // dataflow sweeps, ready bits and vector in LDS
+};
+
 // One workgroup runs one matrix end to end: value refresh, GESP value,
 // factorization, then refinement sweeps until eta <= tol or maxit.  xs is
-// the FP32 correction vector (LDS when it fits, else the global scratch).
-__device__ void wg_pipeline(const MatDesc &M, const MatIO &io, double tol, int maxit, int perturb,
-                            double *sh, float *xs, float *wlds)
+// the FP32 correction vector; flags the sweeps' ready bits (kSweepLevels:
+// unused).
+template <int kMode>
+__device__ __forceinline__ void wg_pipeline(const MatDesc &M, const MatIO &io, double tol, int maxit, int perturb,
+                                            double *sh, float *xs, uint32_t *flags, float *wlds)
 {
     const uint32_t n = M.n;
     const uint32_t tid = threadIdx.x, nt = blockDim.x;
@@ -630,15 +833,33 @@ __device__ void wg_pipeline(const MatDesc &M, const MatIO &io, double tol, int m
         __syncthreads();
     }
 
-    // 4. refinement: x += LU^-1 r (FP32), r = b - A x (FP64)
+    // 4. factors in CSR order for the sweeps (each row's entries contiguous)
+    if (kMode != kSweepLevels) {
+        for (uint32_t q = tid; q < M.nnz; q += nt) io.lucsr[q] = io.lu[S.upos[q]];
+        __syncthreads();
+    }
+
+    // 5. refinement: x += LU^-1 r (FP32), r = b - A x (FP64)
+    const uint32_t nwords = (n + 31) / 32;
     int it = 0;
     double eta = 0.0;
     for (;;) {
         for (uint32_t j = tid; j < n; j += nt)
             xs[j] = (float)(io.r64[S.col_perm[j]] * S.col_scale_perm[j]);
-        __syncthreads();
-        wg_tri_refine(S.lo, io.lu, xs);
-        wg_tri_refine(S.up, io.lu, xs);
+        if (kMode == kSweepLevels) {
+            __syncthreads();
+            wg_tri_refine(S.lo, io.lu, xs);
+            wg_tri_refine(S.up, io.lu, xs);
+        } else {
+            for (uint32_t i = tid; i < nwords; i += nt) flags[i] = 0u;
+            __syncthreads();
+            df_sweep<false>(S.lo_csr, io.lucsr, xs, flags, 0, S.lo_csr.ntask, wlds);
+            __syncthreads();
+            for (uint32_t i = tid; i < nwords; i += nt) flags[i] = 0u;
+            __syncthreads();
+            df_sweep<false>(S.up_csr, io.lucsr, xs, flags, 0, S.up_csr.ntask, wlds);
+            __syncthreads();
+        }
         for (uint32_t j = tid; j < n; j += nt)
             io.x[j] = io.x[j] + (double)xs[S.row_perm_inv[j]] * S.row_scale[j];
         __syncthreads();
@@ -661,15 +882,22 @@ __device__ void wg_pipeline(const MatDesc &M, const MatIO &io, double tol, int m
     }
 }
 
-extern __shared__ float g_lds_vec[];
+// Dynamic LDS of the batch kernels: ready bits (ceil(n/32) words), then,
+// for kSweepLds, the n-float correction vector.
+template <int kMode>
+__device__ __forceinline__ void batch_lds(uint32_t n, float *t32, uint32_t *&flags, float *&xs)
+{
+    flags = kMode == kSweepLevels ? nullptr : reinterpret_cast<uint32_t *>(g_lds_vec);
+    xs = kMode == kSweepLds ? g_lds_vec + (n + 31) / 32 : t32;
+}
 
 // Same pattern: matrix b = blockIdx.x uses desc[0] and contiguous buffers.
-// kLds: the correction vector lives in dynamic LDS (n floats).
-template <bool kLds>
+template <int kMode>
 __global__ void __launch_bounds__(1024) k_batch_same(const MatDesc *__restrict__ desc, const double *__restrict__ ax,
                              const double *__restrict__ b, double *__restrict__ x, float *__restrict__ lu,
-                             float *__restrict__ t32, double *__restrict__ r64, int32_t *__restrict__ iters,
-                             double *__restrict__ eta, double tol, int maxit, int perturb)
+                             float *__restrict__ lucsr, float *__restrict__ t32, double *__restrict__ r64,
+                             int32_t *__restrict__ iters, double *__restrict__ eta, double tol, int maxit,
+                             int perturb)
 {
     __shared__ double sh[1024 / kWave];
     __shared__ float wl[1024 / kWave][kWave];
@@ -682,20 +910,27 @@ __global__ void __launch_bounds__(1024) k_batch_same(const MatDesc *__restrict__
     io.lu = lu + m * M.nnz;
     io.t32 = t32 + m * M.n;
     io.r64 = r64 + m * M.n;
+    io.lucsr = lucsr ? lucsr + m * M.nnz : nullptr;
     io.iters = iters + m;
     io.eta = eta + m;
-    wg_pipeline(M, io, tol, maxit, perturb, sh, kLds ? g_lds_vec : io.t32, wl[threadIdx.x / kWave]);
+    uint32_t *flags;
+    float *xs;
+    batch_lds<kMode>(M.n, io.t32, flags, xs);
+    wg_pipeline<kMode>(M, io, tol, maxit, perturb, sh, xs, flags, wl[threadIdx.x / kWave]);
 }
 
 // Distinct patterns: workgroup g runs matrix order[g] with its own tables.
-template <bool kLds>
+template <int kMode>
 __global__ void __launch_bounds__(1024) k_batch_group(const MatDesc *__restrict__ descs, const MatIO *__restrict__ ios,
                               const uint32_t *__restrict__ order, double tol, int maxit, int perturb)
 {
     __shared__ double sh[1024 / kWave];
     __shared__ float wl[1024 / kWave][kWave];
     const uint32_t m = order[blockIdx.x];
-    wg_pipeline(descs[m], ios[m], tol, maxit, perturb, sh, kLds ? g_lds_vec : ios[m].t32, wl[threadIdx.x / kWave]);
+    uint32_t *flags;
+    float *xs;
+    batch_lds<kMode>(descs[m].n, ios[m].t32, flags, xs);
+    wg_pipeline<kMode>(descs[m], ios[m], tol, maxit, perturb, sh, xs, flags, wl[threadIdx.x / kWave]);
 }
 
 // ---------------------------------------------------------------------------
@@ -767,31 +1002,49 @@ void residual(const RefineDev &R, const double *ax, const double *b, const doubl
     k_residual<<<blocks_for(lanes, 256), 256, 0, st>>>(R, ax, b, x, r, n, transpose, with_anorm, norms);
 }
 
-// lds_bytes > 0: the correction vector is kept in that much dynamic LDS.
-void batch_same(const MatDesc *desc, int batch, const double *ax, const double *b, double *x, float *lu,
-                float *t32, double *r64, int32_t *iters, double *eta, double tol, int maxit, int perturb,
-                int threads, size_t lds_bytes, hipStream_t st)
+// Dynamic LDS bytes of a batch kernel in sweep mode `mode` for order n.
+size_t batch_lds_bytes(int mode, uint32_t n)
 {
-    if (lds_bytes)
-        k_batch_same<true><<<batch, threads, lds_bytes, st>>>(desc, ax, b, x, lu, t32, r64, iters, eta, tol, maxit,
-                                                             perturb);
+    const size_t words = (n + 31) / 32;
+    return mode == kSweepLevels ? 0 : 4 * words + (mode == kSweepLds ? 4 * (size_t)n : 0);
+}
+
+void batch_same(const MatDesc *desc, int batch, const double *ax, const double *b, double *x, float *lu,
+                float *lucsr, float *t32, double *r64, int32_t *iters, double *eta, double tol, int maxit,
+                int perturb, int threads, int mode, size_t lds_bytes, hipStream_t st)
+{
+    if (mode == kSweepLds)
+        k_batch_same<kSweepLds><<<batch, threads, lds_bytes, st>>>(desc, ax, b, x, lu, lucsr, t32, r64, iters, eta,
+                                                                  tol, maxit, perturb);
+    else if (mode == kSweepFlags)
+        k_batch_same<kSweepFlags><<<batch, threads, lds_bytes, st>>>(desc, ax, b, x, lu, lucsr, t32, r64, iters,
+                                                                    eta, tol, maxit, perturb);
     else
-        k_batch_same<false><<<batch, threads, 0, st>>>(desc, ax, b, x, lu, t32, r64, iters, eta, tol, maxit,
-                                                      perturb);
+        k_batch_same<kSweepLevels><<<batch, threads, 0, st>>>(desc, ax, b, x, lu, lucsr, t32, r64, iters, eta, tol,
+                                                             maxit, perturb);
 }
 
 void batch_group(const MatDesc *descs, const void *ios, const uint32_t *order, int count, double tol,
-                 int maxit, int perturb, int threads, size_t lds_bytes, hipStream_t st)
+                 int maxit, int perturb, int threads, int mode, size_t lds_bytes, hipStream_t st)
 {
     const MatIO *io = static_cast<const MatIO *>(ios);
-    if (lds_bytes)
-        k_batch_group<true><<<count, threads, lds_bytes, st>>>(descs, io, order, tol, maxit, perturb);
+    if (mode == kSweepLds)
+        k_batch_group<kSweepLds><<<count, threads, lds_bytes, st>>>(descs, io, order, tol, maxit, perturb);
+    else if (mode == kSweepFlags)
+        k_batch_group<kSweepFlags><<<count, threads, lds_bytes, st>>>(descs, io, order, tol, maxit, perturb);
     else
-        k_batch_group<false><<<count, threads, 0, st>>>(descs, io, order, tol, maxit, perturb);
+        k_batch_group<kSweepLevels><<<count, threads, 0, st>>>(descs, io, order, tol, maxit, perturb);
 }
 
-// Allows dynamic LDS above the default cap for the batch kernels; returns
-// the per-workgroup LDS limit in bytes.
+// Dataflow sweep of fused levels [l0, l1) (tasks [k0, k1)) in one workgroup.
+void tri_df(const TriDev &T, const float *lu, float *x, int l0, int l1, uint32_t k0, uint32_t k1, uint32_t n,
+            int threads, hipStream_t st)
+{
+    k_tri_df<<<1, threads, 4 * (size_t)((n + 31) / 32), st>>>(T, lu, x, l0, l1, k0, k1, n);
+}
+
+// Allows `bytes` of dynamic LDS for the batch and dataflow kernels; returns
+// the per-workgroup limit for dynamic LDS (0 if no device).
 size_t enable_lds(size_t bytes)
 {
     int dev = 0, lim = 0;
@@ -799,10 +1052,14 @@ size_t enable_lds(size_t bytes)
     if (hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return 0;
     const size_t avail = (size_t)lim > 8192 ? (size_t)lim - 8192 : 0;  // static LDS of the kernels (< 8 KiB)
     if (bytes > avail) return avail;
-    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_batch_same<true>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_batch_group<true>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    const void *fns[] = {
+        reinterpret_cast<const void *>(&k_batch_same<kSweepLds>),
+        reinterpret_cast<const void *>(&k_batch_same<kSweepFlags>),
+        reinterpret_cast<const void *>(&k_batch_group<kSweepLds>),
+        reinterpret_cast<const void *>(&k_batch_group<kSweepFlags>),
+        reinterpret_cast<const void *>(&k_tri_df),
+    };
+    for (const void *f : fns) (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
     return avail;
 }
 

@@ -41,11 +41,14 @@ void update_x(const float *t, double *x, const uint32_t *P, const double *S, int
 void refresh(const RefineDev &R, const double *ax, float *lu, uint32_t nnz_lu, int batch, hipStream_t st);
 void residual(const RefineDev &R, const double *ax, const double *b, const double *x, double *r, uint32_t n,
               int transpose, int with_anorm, unsigned long long *norms, uint32_t lanes, hipStream_t st);
+size_t batch_lds_bytes(int mode, uint32_t n);
 void batch_same(const MatDesc *desc, int batch, const double *ax, const double *b, double *x, float *lu,
-                float *t32, double *r64, int32_t *iters, double *eta, double tol, int maxit, int perturb,
-                int threads, size_t lds_bytes, hipStream_t st);
+                float *lucsr, float *t32, double *r64, int32_t *iters, double *eta, double tol, int maxit,
+                int perturb, int threads, int mode, size_t lds_bytes, hipStream_t st);
 void batch_group(const MatDesc *descs, const void *ios, const uint32_t *order, int count, double tol,
-                 int maxit, int perturb, int threads, size_t lds_bytes, hipStream_t st);
+                 int maxit, int perturb, int threads, int mode, size_t lds_bytes, hipStream_t st);
+void tri_df(const TriDev &T, const float *lu, float *x, int l0, int l1, uint32_t k0, uint32_t k1, uint32_t n,
+            int threads, hipStream_t st);
 size_t enable_lds(size_t bytes);
 void zero_u64(unsigned long long *p, int count, hipStream_t st);
 void adjoint_values(const RefineDev &R, const double *lam, const double *x, double *grad, uint32_t n,
@@ -204,10 +207,12 @@ struct Segment {
     int l0, l1;      // levels [l0, l1)
     bool fused;
     uint32_t items;  // grid segments: items (factor) or rows (solve)
+    uint32_t k0 = 0, k1 = 0;   // solve segments: dataflow tasks [k0, k1)
 };
 
 struct TriHost {
     std::vector<uint32_t> rows, lvl_ptr, lvl_long;
+    std::vector<uint32_t> tasks;   // dataflow task starts (+ end sentinel)
     std::vector<Segment> segs;
 };
 
@@ -239,6 +244,7 @@ struct glu_plan {
 
     std::vector<Segment> fsegs;
     TriHost tri[4];          // lo, up, tup, tlo
+    bool tri_dataflow = false;   // fused solve segments run barrier-free
 
     char *d_arena = nullptr;
     size_t arena_bytes = 0;
@@ -258,6 +264,7 @@ struct glu_plan {
     // batch workspaces, grown on demand
     float *d_bt32 = nullptr;
     double *d_br64 = nullptr;
+    float *d_blucsr = nullptr;   // factors in CSR order, nnz per matrix
     int batch_cap = 0;
 
     ~glu_plan()
@@ -266,6 +273,7 @@ struct glu_plan {
         if (d_val) (void)hipFree(d_val);
         if (d_bt32) (void)hipFree(d_bt32);
         if (d_br64) (void)hipFree(d_br64);
+        if (d_blucsr) (void)hipFree(d_blucsr);
     }
 };
 
@@ -606,6 +614,30 @@ void tri_segments(glu_plan &p, TriHost &H)
     make_segments(work, (uint32_t)p.fused_threads * 4, H.segs);
 }
 
+// Dataflow tasks of a sweep: in schedule order, each long row is a task of
+// its own and consecutive short rows form tasks of up to 64 (one wave, a
+// row per lane), which may span levels.  No task crosses a segment
+// boundary, so a fused segment runs tasks [k0, k1) by itself.
+void tri_tasks(TriHost &H)
+{
+    H.tasks.clear();
+    for (Segment &s : H.segs) {
+        s.k0 = (uint32_t)H.tasks.size();
+        uint32_t run = 0;   // rows in the open short task
+        for (int l = s.l0; l < s.l1; ++l) {
+            const uint32_t p0 = H.lvl_ptr[l], pl = p0 + H.lvl_long[l], p1 = H.lvl_ptr[l + 1];
+            if (pl > p0) run = 0;
+            for (uint32_t q = p0; q < pl; ++q) H.tasks.push_back(q | kLongTask);
+            for (uint32_t q = pl; q < p1; ++q) {
+                if (run == 0) H.tasks.push_back(q);
+                run = run + 1 == 64 ? 0 : run + 1;   // a wave of rows
+            }
+        }
+        s.k1 = (uint32_t)H.tasks.size();
+    }
+    H.tasks.push_back(H.lvl_ptr.back());
+}
+
 struct Uploader {
     Arena arena;
     struct Item {
@@ -677,7 +709,10 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
     tri_levels(n, -1, [&](uint32_t r, auto f) {
         for (uint32_t q = P.l_col_ptr[r] + 1; q < P.sym_c_ptr[r + 1]; ++q) f(P.sym_r_idx[q]);
     }, [&](uint32_t r) { return P.sym_c_ptr[r + 1] - P.l_col_ptr[r] - 1; }, p->tri[3]);
-    for (auto &h : p->tri) tri_segments(*p, h);
+    for (auto &h : p->tri) {
+        tri_segments(*p, h);
+        tri_tasks(h);
+    }
 
     // Refinement / refresh tables from the analysis.
     std::vector<uint32_t> a_rptr, a_cidx, a_src, lu_src, lu_col, a_rows, a_cols;
@@ -742,11 +777,12 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
     const size_t o_msrc = U.put(T.msrc);
     const size_t o_dvd = U.put(T.dvd);
     const size_t o_dptr = U.put(T.dvd_ptr);
-    size_t o_trows[4], o_tptr[4], o_tlong[4];
+    size_t o_trows[4], o_tptr[4], o_tlong[4], o_ttask[4];
     for (int s = 0; s < 4; ++s) {
         o_trows[s] = U.put(p->tri[s].rows);
         o_tptr[s] = U.put(p->tri[s].lvl_ptr);
         o_tlong[s] = U.put(p->tri[s].lvl_long);
+        o_ttask[s] = U.put(p->tri[s].tasks);
     }
     size_t o_rp = 0, o_rpi = 0, o_cp = 0, o_cpi = 0, o_rs = 0, o_csp = 0;
     size_t o_arptr = 0, o_acidx = 0, o_asrc = 0, o_acptr = 0, o_aridx = 0, o_lusrc = 0, o_rscol = 0;
@@ -814,6 +850,8 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
         t.lvl_ptr = u32p(o_tptr[s]);
         t.lvl_long = u32p(o_tlong[s]);
         t.nlev = (int)p->tri[s].lvl_ptr.size() - 1;
+        t.dtask = u32p(o_ttask[s]);
+        t.ntask = (uint32_t)p->tri[s].tasks.size() - 1;
         return t;
     };
     const uint32_t *colptr = u32p(o_colptr), *rowidx = u32p(o_rowidx), *diag = u32p(o_diag);
@@ -823,6 +861,14 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
     S.up = tri(1, cdiag, 1, crptr + 1, ccidx, upos_d, diag, 1);
     S.tup = tri(2, colptr, 0, diag, rowidx, nullptr, diag, 0);
     S.tlo = tri(3, diag, 1, colptr + 1, rowidx, nullptr, nullptr, 0);
+    // the same L and U sweeps over the factors copied into CSR order
+    S.lo_csr = tri(0, crptr, 0, cdiag, ccidx, nullptr, nullptr, 0);
+    S.up_csr = tri(1, cdiag, 1, crptr + 1, ccidx, nullptr, cdiag, 1);
+    S.upos = upos_d;
+    {
+        const size_t need = 4 * (size_t)((n + 31) / 32);
+        p->tri_dataflow = launch::enable_lds(need) >= need;
+    }
     if (an) {
         S.row_perm = u32p(o_rp);
         S.row_perm_inv = u32p(o_rpi);
@@ -898,7 +944,9 @@ glu_status run_factor(glu_plan *p, float *d_val, int perturb, hipStream_t st)
 void run_tri(glu_plan *p, const TriDev &T, const TriHost &H, const float *lu, float *x, hipStream_t st)
 {
     for (const Segment &s : H.segs) {
-        if (s.fused)
+        if (s.fused && p->tri_dataflow)
+            launch::tri_df(T, lu, x, s.l0, s.l1, s.k0, s.k1, p->n, p->fused_threads, st);
+        else if (s.fused)
             launch::tri_fused(T, lu, x, s.l0, s.l1, p->fused_threads, st);
         else
             launch::tri_grid(T, lu, x, s.l0, s.items, st);
@@ -917,16 +965,31 @@ void run_sweeps(glu_plan *p, const float *lu, float *t, int transpose, hipStream
     }
 }
 
+// The richest sweep mode of the batch kernels whose LDS fits order n:
+// 2 = ready bits and correction vector in LDS, 1 = ready bits only,
+// 0 = level-synchronous sweeps.
+int sweep_mode(uint32_t n)
+{
+    for (int mode = 2; mode > 0; --mode) {
+        const size_t need = launch::batch_lds_bytes(mode, n);
+        if (launch::enable_lds(need) >= need) return mode;
+    }
+    return 0;
+}
+
 glu_status ensure_batch(glu_plan *p, int batch)
 {
     if (batch <= p->batch_cap) return GLU_OK;
     if (p->d_bt32) (void)hipFree(p->d_bt32);
     if (p->d_br64) (void)hipFree(p->d_br64);
+    if (p->d_blucsr) (void)hipFree(p->d_blucsr);
     p->d_bt32 = nullptr;
     p->d_br64 = nullptr;
+    p->d_blucsr = nullptr;
     p->batch_cap = 0;
     HIP_TRY(hipMalloc(&p->d_bt32, sizeof(float) * (size_t)p->n * batch));
     HIP_TRY(hipMalloc(&p->d_br64, sizeof(double) * (size_t)p->n * batch));
+    HIP_TRY(hipMalloc(&p->d_blucsr, sizeof(float) * (size_t)p->nnz * batch));
     p->batch_cap = batch;
     return GLU_OK;
 }
@@ -1091,10 +1154,10 @@ extern "C" glu_status glu_batch_solve(glu_plan *p, int batch, const double *d_ax
     if (!p->has_refine) return fail(GLU_ERR_ARGUMENT, "plan has no analysis");
     glu_status s = ensure_batch(p, batch);
     if (s != GLU_OK) return s;
-    const size_t lds = sizeof(float) * p->n;
-    const size_t use = (launch::enable_lds(lds) >= lds) ? lds : 0;
-    launch::batch_same(p->d_desc, batch, d_ax, d_b, d_x, d_lu, p->d_bt32, p->d_br64, d_iters, d_eta, tol, maxit,
-                       (int)(flags & 1u), p->fused_threads, use, static_cast<hipStream_t>(stream));
+    const int mode = sweep_mode(p->n);
+    launch::batch_same(p->d_desc, batch, d_ax, d_b, d_x, d_lu, p->d_blucsr, p->d_bt32, p->d_br64, d_iters, d_eta,
+                       tol, maxit, (int)(flags & 1u), p->fused_threads, mode, launch::batch_lds_bytes(mode, p->n),
+                       static_cast<hipStream_t>(stream));
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(GLU_ERR_DEVICE, std::string("batch launch: ") + hipGetErrorString(e));
     return GLU_OK;
@@ -1111,6 +1174,7 @@ struct MatIOHost {            // mirrors the device-side MatIO layout
     float *lu;
     float *t32;
     double *r64;
+    float *lucsr;
     int32_t *iters;
     double *eta;
 };
@@ -1122,10 +1186,13 @@ struct glu_group {
     uint32_t *d_order = nullptr;
     float *d_t32 = nullptr;
     double *d_r64 = nullptr;
+    float *d_lucsr = nullptr;
     std::vector<size_t> voff;   // per-matrix offset into the scratch vectors
+    std::vector<size_t> foff;   // per-matrix offset into d_lucsr
     int threads = kDefaultFusedThreads;
     ~glu_group()
     {
+        if (d_lucsr) (void)hipFree(d_lucsr);
         if (d_descs) (void)hipFree(d_descs);
         if (d_ios) (void)hipFree(d_ios);
         if (d_order) (void)hipFree(d_order);
@@ -1142,7 +1209,7 @@ extern "C" glu_status glu_group_create(glu_plan *const *plans, int count, glu_gr
     std::unique_ptr<glu_group> g(new glu_group());
     std::vector<MatDesc> descs(count);
     std::vector<std::pair<int64_t, uint32_t>> work(count);
-    size_t vtot = 0;
+    size_t vtot = 0, ftot = 0;
     for (int i = 0; i < count; ++i) {
         glu_plan *p = plans[i];
         if (!p || !p->has_refine) return fail(GLU_ERR_ARGUMENT, "every plan must come from an analysis");
@@ -1154,6 +1221,8 @@ extern "C" glu_status glu_group_create(glu_plan *const *plans, int count, glu_gr
         descs[i].nnz = p->nnz;
         g->voff.push_back(vtot);
         vtot += p->n;
+        g->foff.push_back(ftot);
+        ftot += p->nnz;
         g->threads = std::max(g->threads, p->fused_threads);
         // heaviest first: levels dominate the latency, updates the work
         work[i] = {-(int64_t)(p->num_lev + p->tri[0].lvl_ptr.size() + p->tri[1].lvl_ptr.size()) * 64 - p->updates,
@@ -1167,6 +1236,7 @@ extern "C" glu_status glu_group_create(glu_plan *const *plans, int count, glu_gr
     HIP_TRY(hipMalloc(&g->d_order, sizeof(uint32_t) * count));
     HIP_TRY(hipMalloc(&g->d_t32, sizeof(float) * vtot));
     HIP_TRY(hipMalloc(&g->d_r64, sizeof(double) * vtot));
+    HIP_TRY(hipMalloc(&g->d_lucsr, sizeof(float) * ftot));
     HIP_TRY(hipMemcpy(g->d_descs, descs.data(), sizeof(MatDesc) * count, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(g->d_order, order.data(), sizeof(uint32_t) * count, hipMemcpyHostToDevice));
     *out = g.release();
@@ -1186,15 +1256,15 @@ extern "C" glu_status glu_group_solve(glu_group *g, const double *const *d_ax, c
     for (int i = 0; i < count; ++i) {
         if (!d_ax[i] || !d_b[i] || !d_x[i] || !d_lu[i]) return fail(GLU_ERR_ARGUMENT, "null matrix pointer");
         ios[i] = MatIOHost{d_ax[i], d_b[i], d_x[i], d_lu[i], g->d_t32 + g->voff[i], g->d_r64 + g->voff[i],
-                           d_iters + i, d_eta + i};
+                           g->d_lucsr + g->foff[i], d_iters + i, d_eta + i};
     }
     hipStream_t st = static_cast<hipStream_t>(stream);
     HIP_TRY(hipMemcpyAsync(g->d_ios, ios.data(), sizeof(MatIOHost) * count, hipMemcpyHostToDevice, st));
     size_t nmax = 0;
     for (glu_plan *p : g->plans) nmax = std::max<size_t>(nmax, p->n);
-    const size_t lds = sizeof(float) * nmax;
-    const size_t use = (launch::enable_lds(lds) >= lds) ? lds : 0;
-    launch::batch_group(g->d_descs, g->d_ios, g->d_order, count, tol, maxit, (int)(flags & 1u), g->threads, use, st);
+    const int mode = sweep_mode((uint32_t)nmax);
+    launch::batch_group(g->d_descs, g->d_ios, g->d_order, count, tol, maxit, (int)(flags & 1u), g->threads, mode,
+                        launch::batch_lds_bytes(mode, (uint32_t)nmax), st);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(GLU_ERR_DEVICE, std::string("group launch: ") + hipGetErrorString(e));
     // ios is read by the kernel from device memory; the host copy may go.
