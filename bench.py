@@ -8,13 +8,21 @@ permutation), factors in FP32, solves in FP32 and refines in FP64 until the
 normwise backward error is <= 1e-10.  Inputs are resident on the GPU before
 the timed region.
 
-Reported beside it (same JSON line): the rate of looping the plain FP32
-drop-in path (single-matrix level-scheduled factor + refinement) over the
-same matrices, the single-matrix numeric refactor time of the n=500,000
-configuration, the batch kernel's HBM roofline fraction, and the reference
-NICSLU CPU refactor rate on the host cores.
+Reported beside it (same JSON line):
+* the rate of looping the plain FP32 drop-in path (single-matrix
+  level-scheduled factor + refinement) over the same matrices;
+* the batch rate including the host-to-device copy of values and RHS;
+* the plain FP32 factor + solve of the n=4,960 MNA matrix;
+* the single-matrix numeric refactor time of the n=500,000 matrix;
+* the uneven group (64 distinct patterns, n in [1k, 50k]) in one launch;
+* forward + backward of the differentiable solve at n=50,000;
+* the batch kernel's HBM roofline fraction;
+* the reference NICSLU CPU refactor rate on the host cores.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+--copy-probe adds one device copy of 256 MiB (read) + 256 MiB (write) of
+known size, whose counter readings calibrate FETCH_SIZE / WRITE_SIZE.
 """
 
 from __future__ import annotations
@@ -45,6 +53,130 @@ def alg_bytes_refine(nnz_a, nnz_lu, n, share):
     return 12 * nnz_a + 8 * (n + 1) / share + 8 * nnz_lu + 56 * n
 
 
+def _events(fn, reps):
+    """Median time (ms) of fn() over reps, HIP events on the current stream."""
+    import torch
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+    for s_, e_ in evs:
+        s_.record()
+        fn()
+        e_.record()
+    torch.cuda.synchronize()
+    return sorted(s_.elapsed_time(e_) for s_, e_ in evs)[reps // 2]
+
+
+def bench_small(dev, log):
+    """Plain GLU FP32 factor + FP32 solve of the n=4960 MNA matrix."""
+    import torch
+    from pyglu import mna
+    from pyglu.analysis import Analysis
+    from pyglu.device import Plan
+    d, i, p, n = mna.mna_csc(mna.SPEC_SMALL, mna.SEED_SMALL)
+    an = Analysis(d, i, p, n)
+    plan = Plan(an)
+    ax = torch.from_numpy(d).to(dev)
+    base = plan.refresh(ax).reshape(-1).clone()
+    lu = base.clone()
+    b32 = torch.from_numpy(mna.rhs(n, 101)).to(dev).to(torch.float32)
+    x32 = torch.empty_like(b32)
+
+    def step():
+        lu.copy_(base)
+        plan.factor(lu)
+        plan.solve(lu, b32, x32)
+
+    for _ in range(3):
+        step()
+    ms = _events(step, 11)
+    fms = _events(lambda: (lu.copy_(base), plan.factor(lu)), 11)
+    out = {"n": n, "nnz": int(d.size), "nnz_lu": an.nnz_lu, "levels": an.num_lev,
+           "factor_solve_ms": round(ms, 4), "factor_ms": round(fms, 4),
+           "us_per_level": round(fms * 1e3 / max(an.num_lev, 1), 2)}
+    log(f"[bench] n=4960: factor+solve {ms:.3f} ms (factor {fms:.3f} ms, {an.num_lev} levels)")
+    plan.close()
+    return out
+
+
+def bench_uneven(dev, log, tol, maxit):
+    """64 distinct MNA patterns (n log-uniform in [1k, 50k]) in one launch."""
+    import numpy as np
+    import torch
+    from pyglu import mna
+    from pyglu.analysis import Analysis
+    from pyglu.device import Group, Plan
+    t0 = time.perf_counter()
+    plans, axs, bs = [], [], []
+    for m, (spec, seed) in enumerate(mna.uneven_specs()):
+        d, i, p, n = mna.mna_csc(spec, seed)
+        plans.append(Plan(Analysis(d, i, p, n)))
+        axs.append(torch.from_numpy(d).to(dev))
+        bs.append(torch.from_numpy(mna.rhs(n, 4500 + m)).to(dev))
+    g = Group(plans)
+    t_an = time.perf_counter() - t0
+    g.solve(axs, bs, tol=tol, maxit=maxit)
+    torch.cuda.synchronize()
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        _, _, iters, eta = g.solve(axs, bs, tol=tol, maxit=maxit)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / reps * 1e3
+    it_h, eta_h = iters.cpu().numpy(), eta.cpu().numpy()
+    # the plain loop over the same matrices
+    t0 = time.perf_counter()
+    for p, ax, b in zip(plans, axs, bs):
+        lu = p.refresh(ax).reshape(-1)
+        p.factor(lu)
+        p.refine(ax, lu, b, tol=tol, maxit=maxit)
+    torch.cuda.synchronize()
+    loop_ms = (time.perf_counter() - t0) * 1e3
+    out = {"count": len(plans), "sum_n": int(sum(p.n for p in plans)), "max_n": int(max(p.n for p in plans)),
+           "max_levels": int(max(p.num_lev for p in plans)), "launch_ms": round(ms, 3),
+           "refactors_per_s": round(len(plans) / ms * 1e3, 1), "plain_loop_ms": round(loop_ms, 3),
+           "refine_iters_max": int(it_h.max()), "eta_max": float(eta_h.max()),
+           "correct": bool(np.all(eta_h <= tol)), "host_analysis_s": round(t_an, 2)}
+    log(f"[bench] uneven group: {ms:.3f} ms per launch (plain loop {loop_ms:.1f} ms), "
+        f"iters max {it_h.max()}, eta max {eta_h.max():.2e}")
+    g.close()
+    for p in plans:
+        p.close()
+    return out
+
+
+def bench_autograd(dev, log):
+    """Forward + backward of spsolve_autograd at n=50,000."""
+    import numpy as np
+    import torch
+    from pyglu import mna
+    from pyglu.analysis import Analysis
+    from pyglu.device import Plan, spsolve_autograd
+    d, i, p, n = mna.mna_csc(mna.SPEC_GRAD, mna.SEED_GRAD)
+    plan = Plan(Analysis(d, i, p, n))
+    xstar = torch.from_numpy(np.random.default_rng(5000).standard_normal(n)).to(dev)
+    b0 = torch.from_numpy(mna.rhs(n, 5001)).to(dev)
+    vals0 = torch.from_numpy(d).to(dev)
+
+    def step():
+        v = vals0.clone().requires_grad_(True)
+        b = b0.clone().requires_grad_(True)
+        x = spsolve_autograd(v, b, plan)
+        loss = ((x - xstar) ** 2).sum()
+        loss.backward()
+        return v.grad, b.grad
+
+    step()
+    torch.cuda.synchronize()
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / reps * 1e3
+    log(f"[bench] autograd n={n}: forward+backward {ms:.3f} ms")
+    plan.close()
+    return {"n": n, "nnz": int(d.size), "forward_backward_ms": round(ms, 3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -59,6 +191,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the NICSLU CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--block-threads", type=int, default=0)
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the n=4960, uneven-group and autograd side measurements")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run one 256 MiB device copy (counter calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -126,8 +262,31 @@ def main():
     log(f"[bench] batch: {wall / K * 1e3:.3f} ms/step (events {kernel_ms:.3f} ms) -> {value:.1f} refactors/s; "
         f"iters {iters_h.min()}..{iters_h.max()} max eta {eta_h.max():.2e} ok={ok}")
 
+    if args.copy_probe:
+        src = torch.empty(64 << 20, dtype=torch.float32, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"[bench] copy probe: {src.numel() * 4} bytes read, {dst.numel() * 4} bytes written")
+        del src, dst
+
+    # the same step with the value sets and RHS crossing PCIe first
+    ax_pin = torch.from_numpy(ax_h).pin_memory()
+    b_pin = torch.from_numpy(b_h).pin_memory()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(K):
+        ax.copy_(ax_pin, non_blocking=True)
+        bb.copy_(b_pin, non_blocking=True)
+        plan.batch_solve(ax, bb, tol=args.tol, maxit=args.maxit, lu=lu, x=x)
+    torch.cuda.synchronize()
+    pcie_rate = B * K / (time.perf_counter() - t0)
+    log(f"[bench] batch incl. H2D of values + rhs: {pcie_rate:.1f} refactors/s")
+    del ax_pin, b_pin
+
     # ---------------- plain FP32 drop-in loop ---------------------------
-    S = min(args.loop_sample, B)
+    S = max(1, min(args.loop_sample, B))
     one_lu = torch.empty(an.nnz_lu, dtype=torch.float32, device=dev)
     one_x = torch.empty(n, dtype=torch.float64, device=dev)
 
@@ -168,6 +327,13 @@ def main():
     if os.path.exists(prof):
         with open(prof) as f:
             traffic = json.load(f).get("bytes_per_launch")
+
+    # ---------------- side configurations --------------------------------
+    small = uneven = grad = None
+    if not args.no_extra:
+        small = bench_small(dev, log)
+        uneven = bench_uneven(dev, log, args.tol, args.maxit)
+        grad = bench_autograd(dev, log)
 
     # ---------------- n = 500k single refactor --------------------------
     large = None
@@ -242,7 +408,11 @@ def main():
         "plain_fp32_loop": {"value": round(loop_rate, 2), "unit": "refactors/s", "sample": S,
                             "single_factor_ms": round(single_factor_ms, 4),
                             "speedup": round(value / loop_rate, 2)},
+        "pcie_inclusive": {"value": round(pcie_rate, 2), "unit": "refactors/s"},
+        "small_factor_solve": small,
         "large_refactor": large,
+        "uneven_group": uneven,
+        "autograd": grad,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                      "kernel_ms": round(kernel_ms, 4), "alg_bytes": round(traffic_alg)},

@@ -246,6 +246,17 @@ struct glu_plan {
     TriHost tri[4];          // lo, up, tup, tlo
     bool tri_dataflow = false;   // fused solve segments run barrier-free
 
+    // hipGraph replay of the launch sequences (use_graph): one executable
+    // graph per entry point, keyed on the buffers it was captured with
+    struct GraphSlot {
+        hipGraphExec_t exec = nullptr;
+        const void *k[3] = {nullptr, nullptr, nullptr};
+        int mode = -1;
+    };
+    bool use_graph = false;
+    hipStream_t cap_stream = nullptr;
+    GraphSlot g_factor, g_solve[2];
+
     char *d_arena = nullptr;
     size_t arena_bytes = 0;
     FactorDev F{};
@@ -274,6 +285,9 @@ struct glu_plan {
         if (d_bt32) (void)hipFree(d_bt32);
         if (d_br64) (void)hipFree(d_br64);
         if (d_blucsr) (void)hipFree(d_blucsr);
+        for (GraphSlot *g : {&g_factor, &g_solve[0], &g_solve[1]})
+            if (g->exec) (void)hipGraphExecDestroy(g->exec);
+        if (cap_stream) (void)hipStreamDestroy(cap_stream);
     }
 };
 
@@ -678,6 +692,7 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
         }
         if (opts->fuse_units < 0) return fail(GLU_ERR_ARGUMENT, "fuse_units must be >= 0");
         if (opts->fuse_units) p->fuse_units = opts->fuse_units;
+        p->use_graph = opts->use_graph != 0;
     }
     p->n = P.n;
     p->nnz = P.nnz;
@@ -1034,10 +1049,55 @@ extern "C" glu_status glu_plan_info(const glu_plan *p, int64_t info[8])
     return GLU_OK;
 }
 
+namespace {
+
+// Replays slot's graph on st, capturing `issue` (which enqueues the launch
+// sequence on the stream it is given) first when the slot is empty or was
+// captured for other buffers / another mode.
+template <typename Issue>
+glu_status replay(glu_plan *p, glu_plan::GraphSlot &slot, const void *k0, const void *k1, const void *k2, int mode,
+                  hipStream_t st, Issue issue)
+{
+    if (!slot.exec || slot.k[0] != k0 || slot.k[1] != k1 || slot.k[2] != k2 || slot.mode != mode) {
+        if (slot.exec) {
+            HIP_TRY(hipStreamSynchronize(st));   // the old graph may still run
+            HIP_TRY(hipGraphExecDestroy(slot.exec));
+            slot.exec = nullptr;
+        }
+        if (!p->cap_stream) HIP_TRY(hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking));
+        HIP_TRY(hipStreamBeginCapture(p->cap_stream, hipStreamCaptureModeThreadLocal));
+        glu_status s = issue(p->cap_stream);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(p->cap_stream, &g);
+        if (s != GLU_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return s;
+        }
+        if (e != hipSuccess) return fail(GLU_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(e));
+        e = hipGraphInstantiate(&slot.exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            slot.exec = nullptr;
+            return fail(GLU_ERR_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(e));
+        }
+        slot.k[0] = k0;
+        slot.k[1] = k1;
+        slot.k[2] = k2;
+        slot.mode = mode;
+    }
+    HIP_TRY(hipGraphLaunch(slot.exec, st));
+    return GLU_OK;
+}
+
+}  // namespace
+
 extern "C" glu_status glu_factor(glu_plan *p, float *d_val, int perturb, void *stream)
 {
     if (!p || !d_val) return fail(GLU_ERR_ARGUMENT, "null argument");
-    return run_factor(p, d_val, perturb, static_cast<hipStream_t>(stream));
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (!p->use_graph) return run_factor(p, d_val, perturb, st);
+    return replay(p, p->g_factor, d_val, nullptr, nullptr, perturb ? 1 : 0, st,
+                  [&](hipStream_t cs) { return run_factor(p, d_val, perturb, cs); });
 }
 
 extern "C" glu_status glu_factor_host(glu_plan *p, float *h_val, int perturb)
@@ -1058,19 +1118,23 @@ extern "C" glu_status glu_solve(glu_plan *p, const float *d_lu, const float *d_b
     if (!p || !d_lu || !d_b || !d_x) return fail(GLU_ERR_ARGUMENT, "null argument");
     if (!p->has_scaling) return fail(GLU_ERR_ARGUMENT, "plan has no permutation/scaling (create it from an analysis)");
     hipStream_t st = static_cast<hipStream_t>(stream);
-    const SolveDev &S = p->S;
-    if (!transpose) {
-        launch::perm_scale_f32(d_b, p->d_t32, S.col_perm, S.col_scale_perm, 0, p->n, st);
-        run_sweeps(p, d_lu, p->d_t32, 0, st);
-        launch::perm_scale_f32(p->d_t32, d_x, S.row_perm_inv, S.row_scale, 0, p->n, st);
-    } else {
-        launch::perm_scale_f32(d_b, p->d_t32, S.row_perm, S.row_scale, 1, p->n, st);
-        run_sweeps(p, d_lu, p->d_t32, 1, st);
-        launch::perm_scale_f32(p->d_t32, d_x, S.col_perm_inv, S.col_scale_perm, 1, p->n, st);
-    }
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return fail(GLU_ERR_DEVICE, std::string("solve launch: ") + hipGetErrorString(e));
-    return GLU_OK;
+    auto issue = [&](hipStream_t s) {
+        const SolveDev &S = p->S;
+        if (!transpose) {
+            launch::perm_scale_f32(d_b, p->d_t32, S.col_perm, S.col_scale_perm, 0, p->n, s);
+            run_sweeps(p, d_lu, p->d_t32, 0, s);
+            launch::perm_scale_f32(p->d_t32, d_x, S.row_perm_inv, S.row_scale, 0, p->n, s);
+        } else {
+            launch::perm_scale_f32(d_b, p->d_t32, S.row_perm, S.row_scale, 1, p->n, s);
+            run_sweeps(p, d_lu, p->d_t32, 1, s);
+            launch::perm_scale_f32(p->d_t32, d_x, S.col_perm_inv, S.col_scale_perm, 1, p->n, s);
+        }
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return fail(GLU_ERR_DEVICE, std::string("solve launch: ") + hipGetErrorString(e));
+        return GLU_OK;
+    };
+    if (!p->use_graph) return issue(st);
+    return replay(p, p->g_solve[transpose ? 1 : 0], d_lu, d_b, d_x, 0, st, issue);
 }
 
 extern "C" glu_status glu_refresh(glu_plan *p, const double *d_ax, float *d_lu, int batch, void *stream)

@@ -46,13 +46,15 @@ class Plan:
 
     ``source`` is an :class:`Analysis` (full plan: factor, solve, refresh,
     refine, batch) or any object exposing the Symbolic_Matrix arrays
-    (factor-only plan, the LUonDevice contract).
+    (factor-only plan, the LUonDevice contract).  ``use_graph`` replays the
+    factor and solve launch sequences from captured hipGraphs (captured
+    again when called with other buffers).
     """
 
-    def __init__(self, source, fuse_units: int = 0, block_threads: int = 0):
+    def __init__(self, source, fuse_units: int = 0, block_threads: int = 0, use_graph: bool = False):
         _require_device()
         L = capi.lib()
-        opts = capi.PlanOpts(fuse_units, block_threads, 0)
+        opts = capi.PlanOpts(fuse_units, block_threads, int(bool(use_graph)))
         h = C.c_void_p()
         if isinstance(source, Analysis):
             capi.check(L.glu_plan_create_from_analysis(source.handle, C.byref(opts), C.byref(h)),

@@ -1,0 +1,98 @@
+"""Full-size GPU checks of the uneven group (64 distinct MNA patterns, n in
+[1k, 50k]) and of the differentiable solve at n=50,000."""
+
+import numpy as np
+import pytest
+
+from helpers import backward_error
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch_mod():
+    import torch
+    return torch
+
+
+def test_uneven_group_full(torch_mod):
+    """One fused launch over 64 patterns: every member refined to the
+    backward-error target, and the factors of the group bit-identical to
+    the single-matrix path on the same values (same deterministic order)."""
+    from pyglu import mna
+    from pyglu.analysis import Analysis
+    from pyglu.device import Group, Plan
+    torch = torch_mod
+    mats, plans, axs, bs = [], [], [], []
+    for m, (spec, seed) in enumerate(mna.uneven_specs()):
+        d, i, p, n = mna.mna_csc(spec, seed)
+        mats.append((d, i, p, n))
+        plans.append(Plan(Analysis(d, i, p, n)))
+        axs.append(torch.from_numpy(mna.perturb_values(d, seed + 7)).cuda())
+        bs.append(torch.from_numpy(mna.rhs(n, 4500 + m)).cuda())
+    print(f"64 patterns: n {min(p.n for p in plans)}..{max(p.n for p in plans)}, "
+          f"levels up to {max(p.num_lev for p in plans)}", flush=True)
+    g = Group(plans)
+    xs, lus, iters, eta = g.solve(axs, bs, tol=1e-10, maxit=20)
+    torch.cuda.synchronize()
+    eta_h = eta.cpu().numpy()
+    assert np.all(eta_h <= 1e-10), eta_h.max()
+    for k, (d, i, p, n) in enumerate(mats):
+        e = backward_error(axs[k].cpu().numpy(), i, p, n, xs[k].cpu().numpy(), bs[k].cpu().numpy())
+        assert e <= 2e-10, (k, n, e)
+    order = np.argsort([p.n for p in plans])
+    for k in (int(order[0]), int(order[len(order) // 2]), int(order[-1])):
+        lu = plans[k].refresh(axs[k]).reshape(-1)
+        plans[k].factor(lu)
+        assert torch.equal(lu.view(torch.int32), lus[k].view(torch.int32)), k
+    g.close()
+
+
+def test_autograd_full_size(torch_mod):
+    """loss = |x - x*|^2 through spsolve_autograd at n=50,000: gradients for
+    A's stored values and b within 1e-6 (normwise relative) of the sparse
+    FP64 oracle's adjoint gradient, and central finite differences (h=1e-6)
+    of the FP64 loss on 100 random entries, each perturbed solve taken
+    exactly by the Sherman-Morrison formula over the FP64 factors."""
+    from oracle import oracle as O
+    from pyglu import mna
+    from pyglu.analysis import Analysis
+    from pyglu.device import Plan, spsolve_autograd
+    torch = torch_mod
+    d, i, p, n = mna.mna_csc(mna.SPEC_GRAD, mna.SEED_GRAD)
+    an = Analysis(d, i, p, n)
+    plan = Plan(an)
+    xstar = np.random.default_rng(5000).standard_normal(n)
+    b = mna.rhs(n, 5001)
+    vals = torch.from_numpy(d.copy()).cuda().requires_grad_(True)
+    bt = torch.from_numpy(b.copy()).cuda().requires_grad_(True)
+    x = spsolve_autograd(vals, bt, plan)
+    loss = ((x - torch.from_numpy(xstar).cuda()) ** 2).sum()
+    loss.backward()
+    gA = vals.grad.cpu().numpy()
+    gb = bt.grad.cpu().numpy()
+
+    pat = O.SymPattern(an)
+    lu64 = O.factor_f64(pat, O.lu_values_f64(pat, an.dump_ax, an.dump_ai, an.dump_ap))
+    x64 = O.solve_f64(pat, lu64, an, b)
+    lam = O.solve_f64(pat, lu64, an, 2 * (x64 - xstar), transpose=True)
+    cols = np.repeat(np.arange(n), np.diff(p))
+    gA_ref = -lam[i] * x64[cols]
+    relA = np.abs(gA - gA_ref).max() / np.abs(gA_ref).max()
+    relb = np.abs(gb - lam).max() / np.abs(lam).max()
+    print(f"n={n}: grad rel err A {relA:.2e}  b {relb:.2e}", flush=True)
+    assert relA <= 1e-6 and relb <= 1e-6
+
+    h = 1e-6
+    gmax = np.abs(gA).max()
+    for k in np.random.default_rng(5002).choice(d.size, 100, replace=False):
+        r, c = int(i[k]), int(cols[k])
+        e_r = np.zeros(n)
+        e_r[r] = 1.0
+        z = O.solve_f64(pat, lu64, an, e_r)          # A^-1 e_r
+        f = []
+        for s in (h, -h):                             # (A + s e_r e_c^T)^-1 b
+            xp = x64 - (s * x64[c] / (1.0 + s * z[c])) * z
+            f.append(((xp - xstar) ** 2).sum())
+        fd = (f[0] - f[1]) / (2 * h)
+        assert abs(fd - gA[k]) <= 1e-5 * max(abs(fd), 1e-3 * gmax), (k, fd, gA[k])

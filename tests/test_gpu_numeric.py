@@ -220,3 +220,40 @@ def test_autograd_against_dense_fp64(torch_mod):
         fm = ((np.linalg.solve(Am, b) - xstar) ** 2).sum()
         fd = (fp - fm) / (2 * h)
         assert abs(fd - gA[k]) <= 1e-5 * max(abs(fd), 1e-3 * np.abs(gA).max())
+
+
+def test_graph_replay_matches_eager(torch_mod):
+    """Factor and solve replayed from captured hipGraphs: every replay, with
+    new contents in the captured buffers and eager work between replays,
+    bit-identical to the eager launches; new buffers trigger a recapture."""
+    from pyglu import mna
+    from pyglu.analysis import Analysis
+    from pyglu.device import Plan
+    torch = torch_mod
+    d, i, p, n = mna_case("mna2k")
+    an = Analysis(d, i, p, n)
+    pg, pe = Plan(an, use_graph=True), Plan(an)
+    lu_g = torch.empty(an.nnz_lu, dtype=torch.float32, device="cuda")
+    b_g = torch.empty(n, dtype=torch.float32, device="cuda")
+    x_g = torch.empty(n, dtype=torch.float32, device="cuda")
+    for rep in range(4):
+        ax = torch.from_numpy(mna.perturb_values(d, 900 + rep)).cuda()
+        b = torch.from_numpy(mna.rhs(n, 950 + rep).astype(np.float32)).cuda()
+        lu_e = pe.refresh(ax).reshape(-1)
+        pe.factor(lu_e, perturb=rep % 2 == 1)
+        x_e = pe.solve(lu_e, b)
+        torch.cuda.synchronize()
+        if rep == 3:   # other buffers: recapture
+            lu_g = torch.empty_like(lu_g)
+            x_g = torch.empty_like(x_g)
+        pg.refresh(ax, lu_g)
+        pg.factor(lu_g, perturb=rep % 2 == 1)
+        b_g.copy_(b)
+        pg.solve(lu_g, b_g, x_g)
+        torch.cuda.synchronize()
+        assert torch.equal(lu_g.view(torch.int32), lu_e.view(torch.int32)), rep
+        assert torch.equal(x_g.view(torch.int32), x_e.view(torch.int32)), rep
+        xt_e = pe.solve(lu_e, b, transpose=True)
+        xt_g = pg.solve(lu_g, b_g, torch.empty_like(x_g), transpose=True)
+        torch.cuda.synchronize()
+        assert torch.equal(xt_g.view(torch.int32), xt_e.view(torch.int32)), rep
