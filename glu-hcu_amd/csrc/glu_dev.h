@@ -30,6 +30,9 @@ struct MergeSrc {
 };
 
 constexpr uint32_t kLongTask = 0x80000000u;
+// Dataflow sweeps: rows with at most this many entries are taken a lane
+// each, with all their entries prefetched; longer rows a wave each.
+constexpr uint32_t kDfLaneRow = 16;
 
 // Rows / merge entries longer than this are processed by a whole wave.
 constexpr uint32_t kLongRow = 64;
@@ -48,8 +51,17 @@ constexpr uint32_t kFirstBit = 0x80000000u;
 //           {lpos, dpos | kFirstBit} (deferred so a level needs one barrier).
 struct FactorDev {
     const uint32_t *col_ptr;   // sym_c_ptr [n+1] (GESP column sums)
-    const uint4 *upd;          // update items, by level
-    const uint32_t *upd_ptr;   // [num_lev + 2]
+    // Column tasks, by level (tasks ctask_ptr[l] .. ctask_ptr[l+1]).  Lane
+    // `lane` of task t owns one L entry of a pivot column k of the level:
+    // cln[64 t + lane] = {L(i,k) position, pivot position, |U_k|, 0}; its
+    // e-th update (e < |U_k|) reads U(k,j) at cup[ctask[t].x + 64 e + lane]
+    // and targets ctgt[same index] (~0u: a merge entry does it);
+    // ctask[t].y = the longest U list of the task.
+    const uint2 *ctask;
+    const uint32_t *ctask_ptr; // [num_lev + 2]
+    const uint4 *cln;
+    const uint32_t *ctgt;
+    const uint32_t *cup;
     const uint32_t *ment;      // merge entries (target CSC positions), by level,
                                // entries with > kLongMerge sources first
     const uint32_t *ment_ptr;  // [num_lev + 2]
@@ -60,6 +72,20 @@ struct FactorDev {
     const uint32_t *dvd_ptr;   // [num_lev + 1]
     int32_t num_lev;
     uint32_t n, nnz;
+    // LDS-resident tail: steps [tail_lev, num_lev] work on tail_nslot values
+    // staged in LDS.  Their items, in t_cln / t_ctgt / t_cup / t_ment /
+    // t_msrc / t_dvd (same indexing as cln / ctgt / cup / ment / msrc /
+    // dvd), address LDS slots instead of CSC positions; tail_pos[slot] is
+    // the slot's CSC position.
+    int32_t tail_lev;          // num_lev + 1 when there is no tail
+    uint32_t tail_nslot;
+    const uint32_t *tail_pos;
+    const uint4 *t_cln;
+    const uint32_t *t_ctgt;
+    const uint32_t *t_cup;
+    const uint32_t *t_ment;
+    const MergeSrc *t_msrc;
+    const uint2 *t_dvd;
 };
 
 // One level-scheduled triangular sweep, row-pull form:
@@ -88,11 +114,33 @@ struct TriDev {
     int32_t pad;
 };
 
+// A pipelined dataflow sweep over a per-matrix copy of the factor values
+// laid out task by task (the batch refinement).  Task k = {base, p0,
+// rows | kLongTask, m}:
+//  - rows of up to kDfK entries, one per lane: lane l takes position p0 + l
+//    (l < rows), its entry e at slot base + 64 e + l (e < m, m = the task's
+//    longest row; shorter rows leave padding slots);
+//  - one longer row (kLongTask): position p0, entry e at slot base + e.
+// Each position's {row, entries} is in rl; entries are stored in the order
+// the row consumes them.  div: U sweeps, the pivot of position p is value
+// nslots + p.
+constexpr uint32_t kDfK = 8;
+
+struct DfDev {
+    const uint4 *task;         // [ntask]
+    const uint2 *rl;           // [n]
+    const uint32_t *idx;       // column of each slot [nslots]
+    uint32_t ntask, nslots;
+    int32_t div;
+    uint32_t voff;             // this sweep's first value in the per-matrix copy
+};
+
 struct SolveDev {
     TriDev lo, up;             // A x = b:   L then U
     TriDev tup, tlo;           // A' x = b:  U' then L'
-    TriDev lo_csr, up_csr;     // L, U reading a CSR-ordered copy of the factors
-    const uint32_t *upos;      // CSR position -> CSC position [nnz] (the copy)
+    DfDev lo_df, up_df;        // L, U over the per-matrix value copy
+    const uint32_t *df_vpos;   // value copy: CSC position of each value, ~0u for padding [df_nval]
+    uint32_t df_nval;
     const uint32_t *row_perm, *row_perm_inv, *col_perm, *col_perm_inv;
     const double *row_scale, *col_scale_perm;
 };
@@ -112,6 +160,10 @@ struct RefineDev {
     uint32_t a_nlong_t;        // number of long columns
     const double *rs_col;      // row_scale[row_perm[c]] per GLU column c [n]
     const double *csp;         // col_scale_perm [n]
+    // per caller value k: its LU position and the two scales of its entry
+    const uint32_t *a_dst;     // [nnz_a]
+    const double *a_rs;        // rs_col of the entry's LU column [nnz_a]
+    const double *a_cs;        // csp of the entry's LU row [nnz_a]
     uint32_t nnz_a;
 };
 

@@ -83,40 +83,80 @@ __device__ __forceinline__ float gesp(float d, float pert, int perturb)
     return (perturb && fabsf(d) < pert) ? pert : d;
 }
 
-// Update items [t, t + Kunroll*stride, ...) of one level, Kunroll at a time:
-// all loads of the group are issued before its atomics, so a lane keeps
-// several independent memory requests in flight.
-//   A(i,j) += -(l * A(k,j)),  l = A(i,k) / A(k,k)  (A(k,k) GESP-perturbed)
+// Column tasks.  A wave takes a task; each lane owns one L entry A(i,k) of a
+// pivot column k of the level, forms l = A(i,k) / A(k,k) (A(k,k)
+// GESP-perturbed) once and, for each U entry u = A(k,j) of the column,
+//   A(i,j) += -(l * u)
 // Each target here receives exactly this one update in this level, so the
 // fire-and-forget atomic add gives an order-independent result.
-template <int Scope>
-__device__ __forceinline__ void factor_updates(const uint4 *__restrict__ upd, uint32_t count, uint32_t t,
-                                               uint32_t stride, float *__restrict__ val, float pert, int perturb)
+//
+// CTab holds a task's table data for this lane (independent of the values,
+// so it can be loaded ahead): the L and pivot positions, the update count,
+// and the first four U positions and targets.
+struct CTab {
+    uint32_t lpos, dpos, cnt, base;
+    uint32_t tt[4], up[4];
+};
+
+__device__ __forceinline__ void ctab_load(const FactorDev &F, uint32_t t, uint32_t lane, CTab &c)
 {
-    constexpr int U = 4;
-    for (; t < count; t += U * stride) {
-        uint4 it[U];
-        bool on[U];
+    const uint2 h = F.ctask[t];
+    const uint4 ln = F.cln[(size_t)t * kWave + lane];
+    c.lpos = ln.x;
+    c.dpos = ln.y;
+    c.cnt = ln.z;
+    c.base = h.x + lane;
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const uint32_t q = t + u * stride;
-            on[u] = q < count;
-            if (on[u]) it[u] = upd[q];
+    for (uint32_t u = 0; u < 4; ++u)
+        if (u < h.y) {
+            c.tt[u] = F.ctgt[c.base + u * kWave];
+            c.up[u] = F.cup[c.base + u * kWave];
         }
-        float d[U], l[U], v[U];
+}
+
+template <int Scope>
+__device__ __forceinline__ void ctab_run(const FactorDev &F, float *__restrict__ val, const CTab &c, float pert,
+                                         int perturb)
+{
+    if (c.cnt == 0) return;
+    const float d = val[c.dpos], a = val[c.lpos];
+    float uv[4];
 #pragma unroll
-        for (int u = 0; u < U; ++u)
-            if (on[u]) {
-                d[u] = val[it[u].w];
-                l[u] = val[it[u].y];
-                v[u] = val[it[u].z];
-            }
+    for (uint32_t u = 0; u < 4; ++u)
+        if (u < c.cnt) uv[u] = val[c.up[u]];
+    const float l = a / gesp(d, pert, perturb);
 #pragma unroll
-        for (int u = 0; u < U; ++u)
-            if (on[u]) {
-                const float lk = l[u] / gesp(d[u], pert, perturb);
-                atomic_sub<Scope>(val + it[u].x, lk * v[u]);
+    for (uint32_t u = 0; u < 4; ++u)
+        if (u < c.cnt && c.tt[u] != ~0u) atomic_sub<Scope>(val + c.tt[u], l * uv[u]);
+    for (uint32_t e = 4; e < c.cnt; e += 4) {
+        uint32_t tt[4], up[4];
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u)
+            if (e + u < c.cnt) {
+                tt[u] = F.ctgt[c.base + (e + u) * kWave];
+                up[u] = F.cup[c.base + (e + u) * kWave];
             }
+        float v[4];
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u)
+            if (e + u < c.cnt) v[u] = val[up[u]];
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u)
+            if (e + u < c.cnt && tt[u] != ~0u) atomic_sub<Scope>(val + tt[u], l * v[u]);
+    }
+}
+
+// Column tasks [c0, c0 + nc) of one level for a group of cooperating
+// waves (w of nw), one at a time.
+template <int Scope>
+__device__ __forceinline__ void factor_ctasks(const FactorDev &F, float *__restrict__ val, uint32_t c0, uint32_t nc,
+                                              uint32_t tid, uint32_t nthreads, float pert, int perturb)
+{
+    const uint32_t lane = tid & (kWave - 1), w = tid / kWave, nw = nthreads / kWave;
+    for (uint32_t t = c0 + w; t < c0 + nc; t += nw) {
+        CTab c;
+        ctab_load(F, t, lane, c);
+        ctab_run<Scope>(F, val, c, pert, perturb);
     }
 }
 
@@ -191,14 +231,14 @@ __device__ __forceinline__ void factor_divide(float *__restrict__ val, uint2 it,
 }
 
 struct LevelSpan {
-    uint32_t u0, nu, m0, nm, ml, d0, nd;
+    uint32_t c0, nc, m0, nm, ml, d0, nd;
 };
 
 __device__ __forceinline__ LevelSpan level_span(const FactorDev &F, int lvl)
 {
     LevelSpan s;
-    s.u0 = F.upd_ptr[lvl];
-    s.nu = F.upd_ptr[lvl + 1] - s.u0;
+    s.c0 = F.ctask_ptr[lvl];
+    s.nc = F.ctask_ptr[lvl + 1] - s.c0;
     s.m0 = F.ment_ptr[lvl];
     s.nm = F.ment_ptr[lvl + 1] - s.m0;
     s.ml = F.ment_long[lvl];
@@ -213,7 +253,7 @@ __device__ __forceinline__ LevelSpan level_span(const FactorDev &F, int lvl)
 
 // All items of one level for a group of `nthreads` cooperating lanes (a
 // workgroup or the whole grid), `tid` = this lane's index in the group:
-// long merges one wave each, then update items, short merges and divisions
+// long merges and column tasks one wave each, short merges and divisions
 // one lane each.  lds: this wave's 64-float slice.
 template <int Scope>
 __device__ __forceinline__ void factor_level(const FactorDev &F, float *__restrict__ val, const LevelSpan &s,
@@ -221,9 +261,42 @@ __device__ __forceinline__ void factor_level(const FactorDev &F, float *__restri
 {
     const uint32_t lane = tid & (kWave - 1), w = tid / kWave, nw = nthreads / kWave;
     for (uint32_t e = w; e < s.ml; e += nw) factor_merge_wave(F, val, s.m0 + e, pert, perturb, lane, lds);
-    factor_updates<Scope>(F.upd + s.u0, s.nu, tid, nthreads, val, pert, perturb);
+    factor_ctasks<Scope>(F, val, s.c0, s.nc, tid, nthreads, pert, perturb);
     for (uint32_t t = s.ml + tid; t < s.nm; t += nthreads) factor_merge(F, val, s.m0 + t, pert, perturb);
     for (uint32_t t = tid; t < s.nd; t += nthreads) factor_divide(val, F.dvd[s.d0 + t], pert, perturb);
+}
+
+// Steps [l0, l1) inside one workgroup, a barrier after each, with the column
+// tasks software-pipelined: a wave loads the table data of its next task
+// (the next one of the level, or its first one of the next level, issued
+// before the barrier) while it runs the current one, so a task waits only
+// for its values.  Same items and order as factor_level.
+template <int Scope>
+__device__ __forceinline__ void factor_levels_wg(const FactorDev &F, float *__restrict__ val, int l0, int l1,
+                                                 float pert, int perturb, float *lds)
+{
+    const uint32_t tid = threadIdx.x, nt = blockDim.x;
+    const uint32_t lane = tid & (kWave - 1), w = tid / kWave, nw = nt / kWave;
+    CTab cur;
+    if (l0 < l1 && F.ctask_ptr[l0] + w < F.ctask_ptr[l0 + 1]) ctab_load(F, F.ctask_ptr[l0] + w, lane, cur);
+    for (int lvl = l0; lvl < l1; ++lvl) {
+        const LevelSpan s = level_span(F, lvl);
+        for (uint32_t e = w; e < s.ml; e += nw) factor_merge_wave(F, val, s.m0 + e, pert, perturb, lane, lds);
+        for (uint32_t t = s.c0 + w; t < s.c0 + s.nc; t += nw) {
+            const bool more = t + nw < s.c0 + s.nc;
+            CTab nxt;
+            if (more) ctab_load(F, t + nw, lane, nxt);
+            ctab_run<Scope>(F, val, cur, pert, perturb);
+            if (more) cur = nxt;
+        }
+        for (uint32_t t = s.ml + tid; t < s.nm; t += nt) factor_merge(F, val, s.m0 + t, pert, perturb);
+        for (uint32_t t = tid; t < s.nd; t += nt) factor_divide(val, F.dvd[s.d0 + t], pert, perturb);
+        if (lvl + 1 < l1) {
+            const uint32_t n0 = F.ctask_ptr[lvl + 1], n1 = F.ctask_ptr[lvl + 2];
+            if (n0 + w < n1) ctab_load(F, n0 + w, lane, cur);
+        }
+        __syncthreads();
+    }
 }
 
 // GESP: pert = 3.45e-4 * max_c sum_p |val[p]| (FP32 sums in column order).
@@ -250,7 +323,7 @@ __global__ void k_zero_u64(unsigned long long *__restrict__ p, int count)
 }
 
 // One level across the grid; the host sizes the grid with the lanes the
-// level needs (update items count a quarter, long merges a wave).
+// level needs (column tasks and long merges a wave each).
 __global__ void __launch_bounds__(256) k_factor_grid(FactorDev F, float *__restrict__ val,
                                                      const float *__restrict__ d_pert, int lvl, int perturb)
 {
@@ -267,11 +340,46 @@ __global__ void __launch_bounds__(1024) k_factor_fused(FactorDev F, float *__res
 {
     __shared__ float lds[1024 / kWave][kWave];
     const float pert = perturb ? *d_pert : 0.f;
-    for (int lvl = l0; lvl < l1; ++lvl) {
-        factor_level<__HIP_MEMORY_SCOPE_AGENT>(F, val, level_span(F, lvl), threadIdx.x, blockDim.x, pert, perturb,
-                                               lds[threadIdx.x / kWave]);
-        __syncthreads();
-    }
+    factor_levels_wg<__HIP_MEMORY_SCOPE_AGENT>(F, val, l0, l1, pert, perturb, lds[threadIdx.x / kWave]);
+}
+
+// The tail's view of the tables: the same level spans, items that address
+// LDS slots.
+__device__ __forceinline__ FactorDev tail_view(const FactorDev &F)
+{
+    FactorDev T = F;
+    T.cln = F.t_cln;
+    T.ctgt = F.t_ctgt;
+    T.cup = F.t_cup;
+    T.ment = F.t_ment;
+    T.msrc = F.t_msrc;
+    T.dvd = F.t_dvd;
+    return T;
+}
+
+// Steps [tail_lev, num_lev] inside one workgroup with the values they touch
+// staged in LDS (slots): load, one barrier per level, store back.
+__device__ __forceinline__ void factor_tail(const FactorDev &F, float *__restrict__ val, float *slots, float pert,
+                                            int perturb, float *wlds)
+{
+    const uint32_t tid = threadIdx.x, nt = blockDim.x;
+    for (uint32_t s = tid; s < F.tail_nslot; s += nt) slots[s] = val[F.tail_pos[s]];
+    __syncthreads();
+    const FactorDev T = tail_view(F);
+    factor_levels_wg<__HIP_MEMORY_SCOPE_WORKGROUP>(T, slots, F.tail_lev, F.num_lev + 1, pert, perturb, wlds);
+    for (uint32_t s = tid; s < F.tail_nslot; s += nt) val[F.tail_pos[s]] = slots[s];
+    __syncthreads();
+}
+
+extern __shared__ float g_lds_vec[];   // dynamic LDS of the kernels below
+
+// The tail of one factorization in one workgroup (dynamic LDS: 4 * tail_nslot).
+__global__ void __launch_bounds__(1024) k_factor_tail(FactorDev F, float *__restrict__ val,
+                                                      const float *__restrict__ d_pert, int perturb)
+{
+    __shared__ float lds[1024 / kWave][kWave];
+    const float pert = perturb ? *d_pert : 0.f;
+    factor_tail(F, val, g_lds_vec, pert, perturb, lds[threadIdx.x / kWave]);
 }
 
 // ---------------------------------------------------------------------------
@@ -384,8 +492,6 @@ __device__ __forceinline__ float wave_sum(float v)
     return v;
 }
 
-extern __shared__ float g_lds_vec[];   // dynamic LDS of the kernels below
-
 // Ready bit of row c.  df_peek is a relaxed read: a caller that reads x
 // after it issues an acquire fence first.
 __device__ __forceinline__ bool df_peek(const uint32_t *flags, uint32_t c)
@@ -403,14 +509,15 @@ __device__ __forceinline__ void df_set(uint32_t *flags, uint32_t r)
     __hip_atomic_fetch_or(flags + (r >> 5), 1u << (r & 31u), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// Up to 64 short rows, one per lane: each lane prefetches its row's entries
-// 8 at a time and consumes them in the row's order (ascending, or
-// descending for the U sweep) as they become ready — the sequential
-// arithmetic of tri_row.
+// Up to 64 rows of at most kDfLaneRow entries, one per lane: each lane
+// loads all its row's entries at once and consumes them in the row's order
+// (ascending, or descending for the U sweep) as they become ready — the
+// sequential arithmetic of tri_row.  (Rows longer than K would be refilled
+// K at a time; the task builder gives them a wave instead.)
 __device__ __forceinline__ void df_short_task(const TriDev &T, const float *__restrict__ vals, float *x,
                                               uint32_t *flags, uint32_t p0, uint32_t p1, uint32_t lane)
 {
-    constexpr uint32_t K = 8;
+    constexpr uint32_t K = kDfLaneRow;
     const uint32_t pos = p0 + lane;
     bool pending = pos < p1;
     uint32_t r = 0, q = 0, qend = 0;
@@ -442,19 +549,17 @@ __device__ __forceinline__ void df_short_task(const TriDev &T, const float *__re
                 q = T.desc ? q - cn : q + cn;
             }
             // ready bits of the chunk's unconsumed entries (independent
-            // relaxed reads), one acquire fence, then the x values, then
+            // relaxed reads into one mask), one acquire fence, then
             // in-order consumption of the ready prefix
-            bool rdy[K];
-#pragma unroll
-            for (uint32_t j = 0; j < K; ++j) rdy[j] = j >= ci && j < cn && df_peek(flags, cidx[j]);
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-            float xv[K];
-#pragma unroll
-            for (uint32_t j = 0; j < K; ++j) xv[j] = rdy[j] ? x[cidx[j]] : 0.f;
+            uint32_t mask = 0;
 #pragma unroll
             for (uint32_t j = 0; j < K; ++j)
-                if (j == ci && rdy[j]) {
-                    acc = acc - cval[j] * xv[j];
+                if (j >= ci && j < cn && df_peek(flags, cidx[j])) mask |= 1u << j;
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+#pragma unroll
+            for (uint32_t j = 0; j < K; ++j)
+                if (j == ci && ((mask >> j) & 1u)) {
+                    acc = acc - cval[j] * x[cidx[j]];
                     ++ci;
                     progressed = true;
                 }
@@ -666,6 +771,81 @@ __device__ __forceinline__ void for_rows_of_a(const RefineDev &R, const double *
     }
 }
 
+// for_rows_of_a for one workgroup, with the short rows taken in natural
+// order (long ones skipped: their waves take them), four rows per lane in
+// flight and four entries of each per step, so a lane keeps up to 16
+// independent gathers outstanding.
+template <typename F>
+__device__ __forceinline__ void wg_rows_of_a(const RefineDev &R, const double *__restrict__ ax,
+                                             const double *__restrict__ x, uint32_t n, F f)
+{
+    constexpr int RB = 2, EB = 4;
+    const uint32_t tid = threadIdx.x, nt = blockDim.x;
+    const uint32_t lane = tid & (kWave - 1), w = tid / kWave, nw = nt / kWave;
+    for (uint32_t t = w; t < R.a_nlong; t += nw) {
+        const uint32_t i = R.a_rows[t];
+        double dot, asum;
+        a_row_partial(R, ax, x, a_row(R, i, 0), lane, kWave, 0, dot, asum);
+        dot = wave_sum64(dot);
+        asum = wave_sum64(asum);
+        if (lane == 0) f(i, dot, asum);
+    }
+    for (uint32_t i0 = tid; i0 < n; i0 += RB * nt) {
+        uint32_t q[RB], qe[RB];
+        double dot[RB], asum[RB];
+#pragma unroll
+        for (int r = 0; r < RB; ++r) {
+            const uint32_t i = i0 + r * nt;
+            q[r] = qe[r] = 0;
+            dot[r] = asum[r] = 0.0;
+            if (i < n) {
+                q[r] = R.a_rptr[i];
+                qe[r] = R.a_rptr[i + 1];
+                if (qe[r] - q[r] > kLongRow) qe[r] = q[r];   // a wave's row
+            }
+        }
+        for (;;) {
+            bool more = false;
+#pragma unroll
+            for (int r = 0; r < RB; ++r) more |= q[r] < qe[r];
+            if (!more) break;
+            uint32_t src[RB][EB], col[RB][EB];
+#pragma unroll
+            for (int r = 0; r < RB; ++r)
+#pragma unroll
+                for (int e = 0; e < EB; ++e)
+                    if (q[r] + e < qe[r]) {
+                        src[r][e] = R.a_src[q[r] + e];
+                        col[r][e] = R.a_cidx[q[r] + e];
+                    }
+            double av[RB][EB], xv[RB][EB];
+#pragma unroll
+            for (int r = 0; r < RB; ++r)
+#pragma unroll
+                for (int e = 0; e < EB; ++e)
+                    if (q[r] + e < qe[r]) {
+                        av[r][e] = ax[src[r][e]];
+                        xv[r][e] = x ? x[col[r][e]] : 0.0;
+                    }
+#pragma unroll
+            for (int r = 0; r < RB; ++r)
+#pragma unroll
+                for (int e = 0; e < EB; ++e)
+                    if (q[r] + e < qe[r]) {
+                        if (x) dot[r] = fma(av[r][e], xv[r][e], dot[r]);
+                        asum[r] += fabs(av[r][e]);
+                    }
+#pragma unroll
+            for (int r = 0; r < RB; ++r) q[r] = q[r] + EB < qe[r] ? q[r] + EB : qe[r];
+        }
+#pragma unroll
+        for (int r = 0; r < RB; ++r) {
+            const uint32_t i = i0 + r * nt;
+            if (i < n && R.a_rptr[i + 1] - R.a_rptr[i] <= kLongRow) f(i, dot[r], asum[r]);
+        }
+    }
+}
+
 // norms[0] = |r|_inf, [1] = |x|_inf, [2] = |A|_inf, [3] = |b|_inf (as u64
 // bits of non-negative doubles; NaN compares largest).
 __device__ __forceinline__ void umax64(unsigned long long *p, double v)
@@ -745,7 +925,7 @@ struct MatIO {
     float *lu;          // nnz_lu
     float *t32;         // n scratch
     double *r64;        // n scratch
-    float *lucsr;       // nnz_lu scratch (factors in CSR order)
+    float *lucsr;       // df_nval scratch: the sweeps' value copy
     int32_t *iters;
     double *eta;
 };
@@ -782,6 +962,229 @@ __device__ __forceinline__ void wg_tri_refine(const TriDev &T, const float *__re
     }
 }
 
+// Pointers read from device memory (the matrix descriptors) are generic to
+// the compiler, which then emits flat loads: these count against the LDS
+// wait counter too, so every LDS wait would also wait for global loads in
+// flight.  The tables and buffers are all in global memory; rebuilding each
+// pointer as a global-address-space pointer (through its integer value, a
+// cast the optimizer keeps, unlike a generic->global->generic round trip)
+// lets address-space inference turn its accesses into global loads.
+template <typename T>
+__device__ __forceinline__ T *glb(T *p)
+{
+    return (T *)(__attribute__((address_space(1))) T *)(uintptr_t)p;
+}
+
+__device__ __forceinline__ void globalize(FactorDev &F)
+{
+    F.col_ptr = glb(F.col_ptr);
+    F.ctask = glb(F.ctask);
+    F.ctask_ptr = glb(F.ctask_ptr);
+    F.cln = glb(F.cln);
+    F.ctgt = glb(F.ctgt);
+    F.cup = glb(F.cup);
+    F.ment = glb(F.ment);
+    F.ment_ptr = glb(F.ment_ptr);
+    F.ment_long = glb(F.ment_long);
+    F.msrc_ptr = glb(F.msrc_ptr);
+    F.msrc = glb(F.msrc);
+    F.dvd = glb(F.dvd);
+    F.dvd_ptr = glb(F.dvd_ptr);
+    F.tail_pos = glb(F.tail_pos);
+    F.t_cln = glb(F.t_cln);
+    F.t_ctgt = glb(F.t_ctgt);
+    F.t_cup = glb(F.t_cup);
+    F.t_ment = glb(F.t_ment);
+    F.t_msrc = glb(F.t_msrc);
+    F.t_dvd = glb(F.t_dvd);
+}
+
+__device__ __forceinline__ void globalize(TriDev &T)
+{
+    T.beg = glb(T.beg);
+    T.end = glb(T.end);
+    T.idx = glb(T.idx);
+    if (T.vpos) T.vpos = glb(T.vpos);
+    if (T.dpos) T.dpos = glb(T.dpos);
+    T.rows = glb(T.rows);
+    T.lvl_ptr = glb(T.lvl_ptr);
+    T.lvl_long = glb(T.lvl_long);
+    T.dtask = glb(T.dtask);
+}
+
+__device__ __forceinline__ void globalize(DfDev &D)
+{
+    D.task = glb(D.task);
+    D.rl = glb(D.rl);
+    D.idx = glb(D.idx);
+}
+
+__device__ __forceinline__ void globalize(RefineDev &R)
+{
+    R.a_rptr = glb(R.a_rptr);
+    R.a_cidx = glb(R.a_cidx);
+    R.a_src = glb(R.a_src);
+    R.a_cptr = glb(R.a_cptr);
+    R.a_ridx = glb(R.a_ridx);
+    R.lu_src = glb(R.lu_src);
+    R.row_idx = glb(R.row_idx);
+    R.lu_col = glb(R.lu_col);
+    R.a_rows = glb(R.a_rows);
+    R.a_cols = glb(R.a_cols);
+    R.rs_col = glb(R.rs_col);
+    R.csp = glb(R.csp);
+    R.a_dst = glb(R.a_dst);
+    R.a_rs = glb(R.a_rs);
+    R.a_cs = glb(R.a_cs);
+}
+
+__device__ __forceinline__ void globalize(MatIO &io)
+{
+    io.ax = glb(io.ax);
+    io.b = glb(io.b);
+    io.x = glb(io.x);
+    io.lu = glb(io.lu);
+    io.t32 = glb(io.t32);
+    io.r64 = glb(io.r64);
+    if (io.lucsr) io.lucsr = glb(io.lucsr);
+    io.iters = glb(io.iters);
+    io.eta = glb(io.eta);
+}
+
+// ---------------------------------------------------------------------------
+// pipelined dataflow sweeps (refinement of the batch)
+// ---------------------------------------------------------------------------
+//
+// The same dataflow rule as df_sweep, over the task-contiguous layout of
+// DfDev, with the memory latency of each task hidden: a wave holds the
+// descriptors of its next 64 tasks in registers (one per lane) and loads
+// task j + 1's rows, columns and values while it runs task j.  Summation
+// order within a row is free here (a correction solve).
+
+struct DfTask {
+    uint32_t base, p0, cnt, m;
+    bool wave;
+    uint32_t row, len;
+    uint32_t idx[kDfK];
+    float val[kDfK];
+    float dval;
+};
+
+__device__ __forceinline__ void df2_issue(const DfDev &D, const float *__restrict__ sv, uint4 d, uint32_t lane,
+                                          DfTask &t)
+{
+    t.base = d.x;
+    t.p0 = d.y;
+    t.wave = (d.z & kLongTask) != 0;
+    t.cnt = d.z & ~kLongTask;
+    t.m = d.w;
+    t.dval = 1.f;
+    if (!t.wave) {
+        const bool on = lane < t.cnt;
+        const uint2 rl = on ? D.rl[t.p0 + lane] : make_uint2(0u, 0u);
+        t.row = rl.x;
+        t.len = rl.y;
+#pragma unroll
+        for (uint32_t e = 0; e < kDfK; ++e)
+            if (e < t.m) {
+                t.idx[e] = D.idx[t.base + e * kWave + lane];
+                t.val[e] = sv[t.base + e * kWave + lane];
+            }
+        if (D.div && on) t.dval = sv[D.nslots + t.p0 + lane];
+    } else {
+        const uint2 rl = D.rl[t.p0];
+        t.row = rl.x;
+        t.len = rl.y;
+        if (lane < t.len) {
+            t.idx[0] = D.idx[t.base + lane];
+            t.val[0] = sv[t.base + lane];
+        }
+        if (D.div) t.dval = sv[D.nslots + t.p0];
+    }
+}
+
+__device__ __forceinline__ void df2_run(const DfDev &D, const float *__restrict__ sv, float *x, uint32_t *flags,
+                                        const DfTask &t, uint32_t lane)
+{
+    if (!t.wave) {
+        bool pending = lane < t.cnt;
+        float acc = pending ? x[t.row] : 0.f;
+        uint32_t ci = 0;
+        while (__ballot(pending)) {
+            bool progressed = false;
+            if (pending) {
+                uint32_t mask = 0;
+#pragma unroll
+                for (uint32_t e = 0; e < kDfK; ++e)
+                    if (e >= ci && e < t.len && df_peek(flags, t.idx[e])) mask |= 1u << e;
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+#pragma unroll
+                for (uint32_t e = 0; e < kDfK; ++e)
+                    if (e == ci && ((mask >> e) & 1u)) {
+                        acc = acc - t.val[e] * x[t.idx[e]];
+                        ++ci;
+                        progressed = true;
+                    }
+                if (ci == t.len) {
+                    if (D.div) acc = acc / t.dval;
+                    x[t.row] = acc;
+                    df_set(flags, t.row);
+                    pending = false;
+                    progressed = true;
+                }
+            }
+            if (!__ballot(progressed)) __builtin_amdgcn_s_sleep(1);
+        }
+        return;
+    }
+    float sum = 0.f;
+    for (uint32_t c0 = 0; c0 < t.len; c0 += kWave) {
+        const uint32_t e = c0 + lane;
+        if (e < t.len) {
+            const uint32_t col = c0 ? D.idx[t.base + e] : t.idx[0];
+            const float v = c0 ? sv[t.base + e] : t.val[0];
+            while (!df_ready(flags, col)) __builtin_amdgcn_s_sleep(1);
+            sum += v * x[col];
+        }
+    }
+    sum = wave_sum(sum);
+    if (lane == 0) {
+        float acc = x[t.row] - sum;
+        if (D.div) acc = acc / t.dval;
+        x[t.row] = acc;
+        df_set(flags, t.row);
+    }
+}
+
+// All tasks of sweep D for the calling workgroup; sv: this sweep's values.
+// The caller has cleared every ready bit.
+__device__ __forceinline__ void df2_sweep(const DfDev &D, const float *__restrict__ sv, float *x, uint32_t *flags)
+{
+    const uint32_t lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave, nw = blockDim.x / kWave;
+    const uint32_t mine = D.ntask > w ? (D.ntask - w + nw - 1) / nw : 0;   // tasks w, w + nw, ...
+    if (mine == 0) return;
+    uint4 dl = make_uint4(0u, 0u, 0u, 0u);
+    auto descs = [&](uint32_t j0) {
+        if (j0 + lane < mine) dl = D.task[w + (j0 + lane) * nw];
+    };
+    auto desc = [&](uint32_t j) {
+        const int l = (int)(j & (kWave - 1));
+        return make_uint4((uint32_t)__builtin_amdgcn_readlane((int)dl.x, l), (uint32_t)__builtin_amdgcn_readlane((int)dl.y, l),
+                          (uint32_t)__builtin_amdgcn_readlane((int)dl.z, l), (uint32_t)__builtin_amdgcn_readlane((int)dl.w, l));
+    };
+    descs(0);
+    DfTask cur, nxt;
+    df2_issue(D, sv, desc(0), lane, cur);
+    for (uint32_t j = 0; j < mine; ++j) {
+        if (j + 1 < mine) {
+            if (((j + 1) & (kWave - 1)) == 0) descs(j + 1);
+            df2_issue(D, sv, desc(j + 1), lane, nxt);
+        }
+        df2_run(D, sv, x, flags, cur, lane);
+        cur = nxt;
+    }
+}
+
 // LDS placement of the batch pipeline's sweep state (dynamic LDS).
 enum : int {
     kSweepLevels = 0,   // level-synchronous sweeps, correction vector in global memory
@@ -792,50 +1195,93 @@ enum : int {
 // One workgroup runs one matrix end to end: value refresh, GESP value,
 // factorization, then refinement sweeps until eta <= tol or maxit.  xs is
 // the FP32 correction vector; flags the sweeps' ready bits (kSweepLevels:
-// unused).
+// unused).  Md is the matrix's descriptor in device memory: each phase
+// takes the (globalized) tables it needs from it, so the descriptor is not
+// held in registers as a whole.
 template <int kMode>
-__device__ __forceinline__ void wg_pipeline(const MatDesc &M, const MatIO &io, double tol, int maxit, int perturb,
+__device__ __forceinline__ void wg_pipeline(const MatDesc *Md, const MatIO &io, double tol, int maxit, int perturb,
                                             double *sh, float *xs, uint32_t *flags, float *wlds)
 {
-    const uint32_t n = M.n;
+    const uint32_t n = Md->n, nnz = Md->nnz;
     const uint32_t tid = threadIdx.x, nt = blockDim.x;
-    const FactorDev &F = M.f;
-    const SolveDev &S = M.s;
-    const RefineDev &R = M.r;
 
-    // 1. values: scale + place A into the LU pattern; |A|_inf and |b|_inf
-    for (uint32_t p = tid; p < M.nnz; p += nt) refresh_entry(R, io.ax, io.lu, p);
-    double amax = 0.0, bmax = 0.0;
-    for_rows_of_a(R, io.ax, nullptr, n, 0, tid, nt, [&](uint32_t, double, double asum) { amax = nmax(amax, asum); });
-    for (uint32_t c = tid; c < n; c += nt) {
-        bmax = nmax(bmax, fabs(io.b[c]));
-        io.x[c] = 0.0;
-        io.r64[c] = io.b[c];
+    // 1. values: zero the pattern, then scatter the scaled entries of A
+    //    (lu[a_dst[k]] = float((ax[k] * rs) * cs), the refactor order);
+    //    |A|_inf and |b|_inf
+    for (uint32_t p = tid; p < nnz; p += nt) io.lu[p] = 0.f;
+    __syncthreads();
+    double anorm, bnorm;
+    {
+        RefineDev R = Md->r;
+        globalize(R);
+        constexpr int U = 4;
+        const uint32_t na = R.nnz_a;
+        for (uint32_t k0 = tid; k0 < na; k0 += U * nt) {
+            double a[U], s1[U], s2[U];
+            uint32_t dst[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint32_t k = k0 + u * nt;
+                if (k < na) {
+                    a[u] = io.ax[k];
+                    dst[u] = R.a_dst[k];
+                    s1[u] = R.a_rs[k];
+                    s2[u] = R.a_cs[k];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (k0 + u * nt < na) io.lu[dst[u]] = (float)((a[u] * s1[u]) * s2[u]);
+        }
+        double amax = 0.0, bmax = 0.0;
+        wg_rows_of_a(R, io.ax, nullptr, n, [&](uint32_t, double, double asum) { amax = nmax(amax, asum); });
+        for (uint32_t c = tid; c < n; c += nt) {
+            bmax = nmax(bmax, fabs(io.b[c]));
+            io.x[c] = 0.0;
+            io.r64[c] = io.b[c];
+        }
+        anorm = wg_max(amax, sh);
+        bnorm = wg_max(bmax, sh);
     }
-    const double anorm = wg_max(amax, sh);
-    const double bnorm = wg_max(bmax, sh);
 
     // 2. GESP value from the freshly placed values
     float pert = 0.f;
     if (perturb) {
+        const uint32_t *col_ptr = glb(Md->f.col_ptr);
         double cm = 0.0;
         for (uint32_t c = tid; c < n; c += nt) {
             float s = 0.f;
-            for (uint32_t p = F.col_ptr[c]; p < F.col_ptr[c + 1]; ++p) s += fabsf(io.lu[p]);
+            for (uint32_t p = col_ptr[c]; p < col_ptr[c + 1]; ++p) s += fabsf(io.lu[p]);
             if (s == s) cm = fmax(cm, (double)s);
         }
         pert = kPertFactor * (float)wg_max(cm, sh);
     }
 
-    // 3. factorization, level by level
-    for (int lvl = 0; lvl <= F.num_lev; ++lvl) {
-        factor_level<__HIP_MEMORY_SCOPE_WORKGROUP>(F, io.lu, level_span(F, lvl), tid, nt, pert, perturb, wlds);
-        __syncthreads();
+    // 3. factorization, level by level; the tail with its values in LDS
+    //    (the sweeps' LDS is free until step 5)
+    {
+        FactorDev F = Md->f;
+        globalize(F);
+        factor_levels_wg<__HIP_MEMORY_SCOPE_WORKGROUP>(F, io.lu, 0, F.tail_lev, pert, perturb, wlds);
+        if (F.tail_lev <= F.num_lev) factor_tail(F, io.lu, g_lds_vec, pert, perturb, wlds);
     }
 
-    // 4. factors in CSR order for the sweeps (each row's entries contiguous)
+    // 4. the factors copied into the sweeps' task-contiguous layout
     if (kMode != kSweepLevels) {
-        for (uint32_t q = tid; q < M.nnz; q += nt) io.lucsr[q] = io.lu[S.upos[q]];
+        const uint32_t *vpos = glb(Md->s.df_vpos);
+        const uint32_t nval = Md->s.df_nval;
+        constexpr int U = 8;
+        for (uint32_t q0 = tid; q0 < nval; q0 += U * nt) {
+            uint32_t src[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) src[u] = q0 + u * nt < nval ? vpos[q0 + u * nt] : ~0u;
+            float v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) v[u] = src[u] != ~0u ? io.lu[src[u]] : 0.f;
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (q0 + u * nt < nval) io.lucsr[q0 + u * nt] = v[u];
+        }
         __syncthreads();
     }
 
@@ -844,27 +1290,46 @@ __device__ __forceinline__ void wg_pipeline(const MatDesc &M, const MatIO &io, d
     int it = 0;
     double eta = 0.0;
     for (;;) {
-        for (uint32_t j = tid; j < n; j += nt)
-            xs[j] = (float)(io.r64[S.col_perm[j]] * S.col_scale_perm[j]);
+        {
+            const uint32_t *cp = glb(Md->s.col_perm);
+            const double *csp = glb(Md->s.col_scale_perm);
+            for (uint32_t j = tid; j < n; j += nt) xs[j] = (float)(io.r64[cp[j]] * csp[j]);
+        }
         if (kMode == kSweepLevels) {
             __syncthreads();
-            wg_tri_refine(S.lo, io.lu, xs);
-            wg_tri_refine(S.up, io.lu, xs);
+            TriDev lo = Md->s.lo, up = Md->s.up;
+            globalize(lo);
+            globalize(up);
+            wg_tri_refine(lo, io.lu, xs);
+            wg_tri_refine(up, io.lu, xs);
         } else {
             for (uint32_t i = tid; i < nwords; i += nt) flags[i] = 0u;
             __syncthreads();
-            df_sweep<false>(S.lo_csr, io.lucsr, xs, flags, 0, S.lo_csr.ntask, wlds);
+            {
+                DfDev lo = Md->s.lo_df;
+                globalize(lo);
+                df2_sweep(lo, io.lucsr + lo.voff, xs, flags);
+            }
             __syncthreads();
             for (uint32_t i = tid; i < nwords; i += nt) flags[i] = 0u;
             __syncthreads();
-            df_sweep<false>(S.up_csr, io.lucsr, xs, flags, 0, S.up_csr.ntask, wlds);
+            {
+                DfDev up = Md->s.up_df;
+                globalize(up);
+                df2_sweep(up, io.lucsr + up.voff, xs, flags);
+            }
             __syncthreads();
         }
-        for (uint32_t j = tid; j < n; j += nt)
-            io.x[j] = io.x[j] + (double)xs[S.row_perm_inv[j]] * S.row_scale[j];
+        {
+            const uint32_t *rpi = glb(Md->s.row_perm_inv);
+            const double *rs = glb(Md->s.row_scale);
+            for (uint32_t j = tid; j < n; j += nt) io.x[j] = io.x[j] + (double)xs[rpi[j]] * rs[j];
+        }
         __syncthreads();
         double rmax = 0.0, xmax = 0.0;
-        for_rows_of_a(R, io.ax, io.x, n, 0, tid, nt, [&](uint32_t i, double dot, double) {
+        RefineDev R = Md->r;
+        globalize(R);
+        wg_rows_of_a(R, io.ax, io.x, n, [&](uint32_t i, double dot, double) {
             const double ri = io.b[i] - dot;
             io.r64[i] = ri;
             rmax = nmax(rmax, fabs(ri));
@@ -901,22 +1366,23 @@ __global__ void __launch_bounds__(1024) k_batch_same(const MatDesc *__restrict__
 {
     __shared__ double sh[1024 / kWave];
     __shared__ float wl[1024 / kWave][kWave];
-    const MatDesc &M = desc[0];
+    const uint32_t n = desc->n, nnz = desc->nnz, nnz_a = desc->r.nnz_a, nval = desc->s.df_nval;
     const uint64_t m = blockIdx.x;
     MatIO io;
-    io.ax = ax + m * M.r.nnz_a;
-    io.b = b + m * M.n;
-    io.x = x + m * M.n;
-    io.lu = lu + m * M.nnz;
-    io.t32 = t32 + m * M.n;
-    io.r64 = r64 + m * M.n;
-    io.lucsr = lucsr ? lucsr + m * M.nnz : nullptr;
+    io.ax = ax + m * nnz_a;
+    io.b = b + m * n;
+    io.x = x + m * n;
+    io.lu = lu + m * nnz;
+    io.t32 = t32 + m * n;
+    io.r64 = r64 + m * n;
+    io.lucsr = lucsr ? lucsr + m * nval : nullptr;
     io.iters = iters + m;
     io.eta = eta + m;
+    globalize(io);
     uint32_t *flags;
     float *xs;
-    batch_lds<kMode>(M.n, io.t32, flags, xs);
-    wg_pipeline<kMode>(M, io, tol, maxit, perturb, sh, xs, flags, wl[threadIdx.x / kWave]);
+    batch_lds<kMode>(n, io.t32, flags, xs);
+    wg_pipeline<kMode>(desc, io, tol, maxit, perturb, sh, xs, flags, wl[threadIdx.x / kWave]);
 }
 
 // Distinct patterns: workgroup g runs matrix order[g] with its own tables.
@@ -927,10 +1393,12 @@ __global__ void __launch_bounds__(1024) k_batch_group(const MatDesc *__restrict_
     __shared__ double sh[1024 / kWave];
     __shared__ float wl[1024 / kWave][kWave];
     const uint32_t m = order[blockIdx.x];
+    MatIO io = ios[m];
+    globalize(io);
     uint32_t *flags;
     float *xs;
-    batch_lds<kMode>(descs[m].n, ios[m].t32, flags, xs);
-    wg_pipeline<kMode>(descs[m], ios[m], tol, maxit, perturb, sh, xs, flags, wl[threadIdx.x / kWave]);
+    batch_lds<kMode>(descs[m].n, io.t32, flags, xs);
+    wg_pipeline<kMode>(descs + m, io, tol, maxit, perturb, sh, xs, flags, wl[threadIdx.x / kWave]);
 }
 
 // ---------------------------------------------------------------------------
@@ -948,7 +1416,7 @@ void pert(const FactorDev &F, const float *val, unsigned int *norm_bits, float *
     k_pert_finish<<<1, 1, 0, st>>>(norm_bits, d_pert);
 }
 
-// lanes: the lanes the level needs (update items count 1/4 each)
+// lanes: the lanes the level needs (column tasks and long merges 64 each)
 void factor_grid(const FactorDev &F, float *val, const float *d_pert, int lvl, uint32_t lanes,
                  int perturb, int threads, hipStream_t st)
 {
@@ -1020,8 +1488,8 @@ void batch_same(const MatDesc *desc, int batch, const double *ax, const double *
         k_batch_same<kSweepFlags><<<batch, threads, lds_bytes, st>>>(desc, ax, b, x, lu, lucsr, t32, r64, iters,
                                                                     eta, tol, maxit, perturb);
     else
-        k_batch_same<kSweepLevels><<<batch, threads, 0, st>>>(desc, ax, b, x, lu, lucsr, t32, r64, iters, eta, tol,
-                                                             maxit, perturb);
+        k_batch_same<kSweepLevels><<<batch, threads, lds_bytes, st>>>(desc, ax, b, x, lu, lucsr, t32, r64, iters,
+                                                                     eta, tol, maxit, perturb);
 }
 
 void batch_group(const MatDesc *descs, const void *ios, const uint32_t *order, int count, double tol,
@@ -1033,7 +1501,12 @@ void batch_group(const MatDesc *descs, const void *ios, const uint32_t *order, i
     else if (mode == kSweepFlags)
         k_batch_group<kSweepFlags><<<count, threads, lds_bytes, st>>>(descs, io, order, tol, maxit, perturb);
     else
-        k_batch_group<kSweepLevels><<<count, threads, 0, st>>>(descs, io, order, tol, maxit, perturb);
+        k_batch_group<kSweepLevels><<<count, threads, lds_bytes, st>>>(descs, io, order, tol, maxit, perturb);
+}
+
+void factor_tail(const FactorDev &F, float *val, const float *d_pert, int perturb, int threads, hipStream_t st)
+{
+    k_factor_tail<<<1, threads, 4 * (size_t)F.tail_nslot, st>>>(F, val, d_pert, perturb);
 }
 
 // Dataflow sweep of fused levels [l0, l1) (tasks [k0, k1)) in one workgroup.
@@ -1043,23 +1516,27 @@ void tri_df(const TriDev &T, const float *lu, float *x, int l0, int l1, uint32_t
     k_tri_df<<<1, threads, 4 * (size_t)((n + 31) / 32), st>>>(T, lu, x, l0, l1, k0, k1, n);
 }
 
-// Allows `bytes` of dynamic LDS for the batch and dataflow kernels; returns
-// the per-workgroup limit for dynamic LDS (0 if no device).
-size_t enable_lds(size_t bytes)
+// Raises the dynamic-LDS cap of the batch, dataflow and tail kernels to the
+// device's per-workgroup limit (less their static LDS) and returns that
+// limit in bytes (0 if no device).  Launches then size their own dynamic LDS
+// up to it.
+size_t enable_lds()
 {
     int dev = 0, lim = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 0;
     if (hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return 0;
     const size_t avail = (size_t)lim > 8192 ? (size_t)lim - 8192 : 0;  // static LDS of the kernels (< 8 KiB)
-    if (bytes > avail) return avail;
     const void *fns[] = {
         reinterpret_cast<const void *>(&k_batch_same<kSweepLds>),
         reinterpret_cast<const void *>(&k_batch_same<kSweepFlags>),
+        reinterpret_cast<const void *>(&k_batch_same<kSweepLevels>),
         reinterpret_cast<const void *>(&k_batch_group<kSweepLds>),
         reinterpret_cast<const void *>(&k_batch_group<kSweepFlags>),
+        reinterpret_cast<const void *>(&k_batch_group<kSweepLevels>),
         reinterpret_cast<const void *>(&k_tri_df),
+        reinterpret_cast<const void *>(&k_factor_tail),
     };
-    for (const void *f : fns) (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    for (const void *f : fns) (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)avail);
     return avail;
 }
 

@@ -49,7 +49,8 @@ void batch_group(const MatDesc *descs, const void *ios, const uint32_t *order, i
                  int maxit, int perturb, int threads, int mode, size_t lds_bytes, hipStream_t st);
 void tri_df(const TriDev &T, const float *lu, float *x, int l0, int l1, uint32_t k0, uint32_t k1, uint32_t n,
             int threads, hipStream_t st);
-size_t enable_lds(size_t bytes);
+size_t enable_lds();
+void factor_tail(const FactorDev &F, float *val, const float *d_pert, int perturb, int threads, hipStream_t st);
 void zero_u64(unsigned long long *p, int count, hipStream_t st);
 void adjoint_values(const RefineDev &R, const double *lam, const double *x, double *grad, uint32_t n,
                     hipStream_t st);
@@ -212,6 +213,7 @@ struct Segment {
 
 struct TriHost {
     std::vector<uint32_t> rows, lvl_ptr, lvl_long;
+    std::vector<uint32_t> len;     // entries of each row
     std::vector<uint32_t> tasks;   // dataflow task starts (+ end sentinel)
     std::vector<Segment> segs;
 };
@@ -245,6 +247,10 @@ struct glu_plan {
     std::vector<Segment> fsegs;
     TriHost tri[4];          // lo, up, tup, tlo
     bool tri_dataflow = false;   // fused solve segments run barrier-free
+    size_t lds_cap = 0;          // dynamic LDS per workgroup (bytes)
+    uint32_t df_nval = 0;        // values of the per-matrix sweep copy
+    int tail_lev = 0;            // first step of the LDS-resident tail
+    uint32_t tail_nslot = 0;
 
     // hipGraph replay of the launch sequences (use_graph): one executable
     // graph per entry point, keyed on the buffers it was captured with
@@ -275,7 +281,7 @@ struct glu_plan {
     // batch workspaces, grown on demand
     float *d_bt32 = nullptr;
     double *d_br64 = nullptr;
-    float *d_blucsr = nullptr;   // factors in CSR order, nnz per matrix
+    float *d_blucsr = nullptr;   // sweep value copies, df_nval per matrix
     int batch_cap = 0;
 
     ~glu_plan()
@@ -400,8 +406,14 @@ bool validate_pattern(const glu_pattern &P, std::string &err)
 
 struct Tables {
     std::vector<uint32_t> upos;        // CSR position -> CSC position
-    std::vector<uint4> upd;            // update items, by level
-    std::vector<uint32_t> upd_ptr;     // [num_lev + 2]
+    // column tasks, by level: task t has lanes 64 t .. 64 t + 63 in cln
+    // ({L position, pivot position, U count, 0}); lane's e-th update reads
+    // U value cup[ctask[t].x + 64 e + lane] and targets ctgt[same index]
+    // for e < ctask[t].y
+    std::vector<uint2> ctask;
+    std::vector<uint4> cln;
+    std::vector<uint32_t> ctgt, cup;
+    std::vector<uint32_t> ctask_ptr;   // [num_lev + 2]
     std::vector<uint32_t> ment, ment_ptr, ment_long, msrc_ptr;
     std::vector<MergeSrc> msrc;
     std::vector<uint2> dvd;            // division items, by level
@@ -474,14 +486,15 @@ bool build_factor_tables(const glu_pattern &P, Tables &T, int64_t &updates, std:
         }
     }
     // Items per level.
-    T.upd_ptr.assign(nl + 2, 0);
+    T.ctask_ptr.assign(nl + 2, 0);
     T.ment_ptr.assign(nl + 2, 0);
     T.ment_long.assign(nl + 1, 0);
     T.dvd_ptr.assign(nl + 1, 0);
     T.msrc_ptr.assign(1, 0);
     std::vector<uint32_t> stamp(nnz, UINT32_MAX), cnt(nnz, 0), slot(nnz, 0), touched;
+    std::vector<std::pair<uint32_t, uint32_t>> cols;   // (|U_k|, k) of the level
     for (int l = 0; l < nl; ++l) {
-        T.upd_ptr[l] = (uint32_t)T.upd.size();
+        T.ctask_ptr[l] = (uint32_t)T.ctask.size();
         T.ment_ptr[l] = (uint32_t)T.ment.size();
         T.dvd_ptr[l] = (uint32_t)T.dvd.size();
         auto each_update = [&](auto f) {
@@ -522,11 +535,50 @@ bool build_factor_tables(const glu_pattern &P, Tables &T, int64_t &updates, std:
         std::vector<uint32_t> fillpos(T.msrc_ptr.begin() + first, T.msrc_ptr.end() - 1);
         T.msrc.resize(T.msrc_ptr.back());
         each_update([&](uint32_t tgt, uint32_t lpos, uint32_t up, uint32_t dpos) {
-            if (cnt[tgt] >= 2)
-                T.msrc[fillpos[slot[tgt] - first]++] = MergeSrc{lpos, up, dpos, 0};
-            else
-                T.upd.push_back(make_uint4(tgt, lpos, up, dpos));
+            if (cnt[tgt] >= 2) T.msrc[fillpos[slot[tgt] - first]++] = MergeSrc{lpos, up, dpos, 0};
         });
+        // Column tasks: the level's columns with updates, longest U list
+        // first, cut into pieces of <= 64 L entries and packed into tasks of
+        // 64 lanes; merge targets are left to their merge entries (~0u).
+        cols.clear();
+        for (int t = P.level_ptr[l]; t < P.level_ptr[l + 1]; ++t) {
+            const uint32_t k = (uint32_t)P.level_idx[t];
+            const uint32_t L = P.sym_c_ptr[k + 1] - P.l_col_ptr[k] - 1;
+            const uint32_t U = P.csr_r_ptr[k + 1] - P.csr_diag_ptr[k] - 1;
+            if (L && U) cols.push_back({U, k});
+        }
+        std::stable_sort(cols.begin(), cols.end(), [](auto a, auto b) { return a.first > b.first; });
+        uint32_t used = 64;   // lanes used in the open task
+        for (const auto &cu : cols) {
+            const uint32_t k = cu.second, U = cu.first;
+            const uint32_t dpos = P.l_col_ptr[k];
+            const uint32_t L = P.sym_c_ptr[k + 1] - dpos - 1;
+            const uint32_t uf = P.csr_diag_ptr[k] + 1;
+            for (uint32_t i0 = 0; i0 < L; i0 += 64) {
+                const uint32_t pc = std::min<uint32_t>(64, L - i0);
+                if (used + pc > 64) {   // open a new task, its lanes idle until filled
+                    T.ctask.push_back(make_uint2((uint32_t)T.ctgt.size(), 0u));
+                    T.cln.resize(T.cln.size() + 64, make_uint4(0u, 0u, 0u, 0u));
+                    used = 0;
+                }
+                uint2 &th = T.ctask.back();
+                if (U > th.y) {   // widen the task's block (tasks fill longest U first)
+                    T.ctgt.resize(th.x + 64 * (size_t)U, UINT32_MAX);
+                    T.cup.resize(th.x + 64 * (size_t)U, 0u);
+                    th.y = U;
+                }
+                const size_t lane0 = T.cln.size() - 64 + used;
+                for (uint32_t ii = 0; ii < pc; ++ii) {
+                    T.cln[lane0 + ii] = make_uint4(dpos + 1 + i0 + ii, dpos, U, 0u);
+                    for (uint32_t jj = 0; jj < U; ++jj) {
+                        const uint32_t tgt = umap[moff[k] + (uint64_t)jj * L + i0 + ii];
+                        T.ctgt[th.x + 64 * (size_t)jj + used + ii] = cnt[tgt] >= 2 ? UINT32_MAX : tgt;
+                        T.cup[th.x + 64 * (size_t)jj + used + ii] = T.upos[uf + jj];
+                    }
+                }
+                used += pc;
+            }
+        }
         for (int t = P.level_ptr[l]; t < P.level_ptr[l + 1]; ++t) {
             const uint32_t k = (uint32_t)P.level_idx[t];
             const uint32_t dpos = P.l_col_ptr[k];
@@ -534,10 +586,10 @@ bool build_factor_tables(const glu_pattern &P, Tables &T, int64_t &updates, std:
                 T.dvd.push_back(make_uint2(p, dpos | (p == dpos + 1 ? kFirstBit : 0u)));
         }
     }
-    T.upd_ptr[nl] = T.upd_ptr[nl + 1] = (uint32_t)T.upd.size();
+    T.ctask_ptr[nl] = T.ctask_ptr[nl + 1] = (uint32_t)T.ctask.size();
     T.ment_ptr[nl] = T.ment_ptr[nl + 1] = (uint32_t)T.ment.size();
     T.dvd_ptr[nl] = (uint32_t)T.dvd.size();
-    if (T.upd.size() >= UINT32_MAX || T.msrc.size() >= UINT32_MAX) {
+    if (T.ctgt.size() >= UINT32_MAX || T.msrc.size() >= UINT32_MAX) {
         err = "more than 2^32 update items";
         return false;
     }
@@ -567,6 +619,8 @@ void tri_levels(uint32_t n, int dir, Deps deps, Len len, TriHost &H)
     }
     H.lvl_ptr.assign(nlev + 1, 0);
     H.lvl_long.assign(nlev, 0);
+    H.len.resize(n);
+    for (uint32_t r = 0; r < n; ++r) H.len[r] = len(r);
     for (uint32_t r = 0; r < n; ++r) {
         ++H.lvl_ptr[lev[r] + 1];
         if (len(r) > kLongRow) ++H.lvl_long[lev[r]];
@@ -599,22 +653,119 @@ void make_segments(const std::vector<uint32_t> &work, uint32_t narrow, std::vect
     }
 }
 
-// Lanes a level needs: update items count a quarter each (4 per lane),
-// long merge entries a wave each.
+// Lanes a level needs: column tasks and long merge entries a wave each.
 uint32_t level_lanes(const Tables &T, int l)
 {
-    const uint32_t nu = T.upd_ptr[l + 1] - T.upd_ptr[l];
+    const uint32_t nc = T.ctask_ptr[l + 1] - T.ctask_ptr[l];
     const uint32_t nm = T.ment_ptr[l + 1] - T.ment_ptr[l];
     const uint32_t ml = T.ment_long[l];
     const uint32_t nd = l > 0 ? T.dvd_ptr[l] - T.dvd_ptr[l - 1] : 0;
-    return std::max({(nu + 3) / 4, nm - ml, nd, ml * 64});
+    return std::max({nc * 64, nm - ml, nd, ml * 64});
 }
 
+// Segments of the steps before the LDS tail.
 void factor_segments(glu_plan &p, const Tables &T)
 {
-    std::vector<uint32_t> work(p.num_lev + 1);
-    for (int l = 0; l <= p.num_lev; ++l) work[l] = level_lanes(T, l);
+    std::vector<uint32_t> work(p.tail_lev);
+    for (int l = 0; l < p.tail_lev; ++l) work[l] = level_lanes(T, l);
     make_segments(work, (uint32_t)p.fuse_units, p.fsegs);
+}
+
+// The LDS-resident tail: the longest run of final steps whose items touch
+// at most `budget` distinct CSC positions.  Those positions become LDS slots
+// (in increasing position order) and the run's items are rewritten to
+// address slots.
+struct Tail {
+    int lev = 0;
+    std::vector<uint32_t> pos;   // slot -> CSC position
+    std::vector<uint4> cln;
+    std::vector<uint32_t> ctgt, cup;
+    std::vector<uint32_t> ment;
+    std::vector<MergeSrc> msrc;
+    std::vector<uint2> dvd;
+    uint32_t c0 = 0, g0 = 0, m0 = 0, s0 = 0, d0 = 0;   // first tail task / target / item of each kind
+};
+
+void choose_tail(const Tables &T, int num_lev, uint32_t nnz, uint32_t budget, Tail &t)
+{
+    t = Tail();
+    t.lev = num_lev + 1;
+    if (budget == 0) return;
+    std::vector<uint32_t> slot(nnz, UINT32_MAX);
+    std::vector<uint32_t> fresh;
+    uint32_t count = 0;
+    auto touch = [&](uint32_t q) {
+        if (slot[q] == UINT32_MAX) {
+            slot[q] = 0;
+            fresh.push_back(q);
+        }
+    };
+    for (int l = num_lev; l >= 0; --l) {
+        fresh.clear();
+        for (uint32_t t = T.ctask_ptr[l]; t < T.ctask_ptr[l + 1]; ++t)
+            for (uint32_t lane = 0; lane < 64; ++lane) {
+                const uint4 &ln = T.cln[(size_t)t * 64 + lane];
+                if (ln.z == 0) continue;
+                touch(ln.x), touch(ln.y);
+                for (uint32_t e = 0; e < ln.z; ++e) {
+                    const size_t q = T.ctask[t].x + 64 * (size_t)e + lane;
+                    touch(T.cup[q]);
+                    if (T.ctgt[q] != UINT32_MAX) touch(T.ctgt[q]);
+                }
+            }
+        for (uint32_t e = T.ment_ptr[l]; e < T.ment_ptr[l + 1]; ++e) {
+            touch(T.ment[e]);
+            for (uint32_t k = T.msrc_ptr[e]; k < T.msrc_ptr[e + 1]; ++k) {
+                const MergeSrc &m = T.msrc[k];
+                touch(m.lpos), touch(m.upos), touch(m.dpos);
+            }
+        }
+        if (l > 0)
+            for (uint32_t q = T.dvd_ptr[l - 1]; q < T.dvd_ptr[l]; ++q) {
+                touch(T.dvd[q].x);
+                touch(T.dvd[q].y & ~kFirstBit);
+            }
+        if (count + fresh.size() > budget) {
+            for (uint32_t q : fresh) slot[q] = UINT32_MAX;
+            break;
+        }
+        count += (uint32_t)fresh.size();
+        t.lev = l;
+    }
+    if (t.lev > num_lev) return;
+    for (uint32_t q = 0; q < nnz; ++q)
+        if (slot[q] != UINT32_MAX) {
+            slot[q] = (uint32_t)t.pos.size();
+            t.pos.push_back(q);
+        }
+    const int L = t.lev;
+    t.c0 = T.ctask_ptr[L];
+    const uint32_t c1 = T.ctask_ptr[num_lev + 1];
+    t.g0 = t.c0 < c1 ? T.ctask[t.c0].x : 0;
+    // padding entries (no update) keep their values: ~0u targets, 0 U positions
+    for (uint32_t q = t.g0; q < T.ctgt.size(); ++q) {
+        const bool live = T.ctgt[q] != UINT32_MAX || slot[T.cup[q]] != UINT32_MAX;
+        t.ctgt.push_back(T.ctgt[q] == UINT32_MAX ? UINT32_MAX : slot[T.ctgt[q]]);
+        t.cup.push_back(live && slot[T.cup[q]] != UINT32_MAX ? slot[T.cup[q]] : 0u);
+    }
+    for (size_t q = (size_t)t.c0 * 64; q < (size_t)c1 * 64; ++q) {
+        uint4 ln = T.cln[q];
+        if (ln.z) ln = make_uint4(slot[ln.x], slot[ln.y], ln.z, 0u);
+        t.cln.push_back(ln);
+    }
+    t.m0 = T.ment_ptr[L];
+    const uint32_t m1 = T.ment_ptr[num_lev + 1];
+    t.s0 = T.msrc_ptr[t.m0];
+    for (uint32_t e = t.m0; e < m1; ++e) t.ment.push_back(slot[T.ment[e]]);
+    for (uint32_t k = t.s0; k < T.msrc_ptr[m1]; ++k) {
+        const MergeSrc &m = T.msrc[k];
+        t.msrc.push_back(MergeSrc{slot[m.lpos], slot[m.upos], slot[m.dpos], 0});
+    }
+    t.d0 = L > 0 ? T.dvd_ptr[L - 1] : 0;
+    for (uint32_t q = t.d0; q < T.dvd_ptr[num_lev]; ++q) {
+        const uint2 &d = T.dvd[q];
+        t.dvd.push_back(make_uint2(slot[d.x], slot[d.y & ~kFirstBit] | (d.y & kFirstBit)));
+    }
 }
 
 void tri_segments(glu_plan &p, TriHost &H)
@@ -628,28 +779,82 @@ void tri_segments(glu_plan &p, TriHost &H)
     make_segments(work, (uint32_t)p.fused_threads * 4, H.segs);
 }
 
-// Dataflow tasks of a sweep: in schedule order, each long row is a task of
-// its own and consecutive short rows form tasks of up to 64 (one wave, a
-// row per lane), which may span levels.  No task crosses a segment
-// boundary, so a fused segment runs tasks [k0, k1) by itself.
+// Dataflow tasks of a sweep: in schedule order, each row with more than
+// kDfLaneRow entries is a task of its own (a wave reads its entries at
+// once), and consecutive shorter rows form tasks of up to 64 (one wave, a
+// row per lane, every entry prefetched at the start), which may span
+// levels.  No task crosses a segment boundary, so a fused segment runs
+// tasks [k0, k1) by itself.
 void tri_tasks(TriHost &H)
 {
     H.tasks.clear();
     for (Segment &s : H.segs) {
         s.k0 = (uint32_t)H.tasks.size();
         uint32_t run = 0;   // rows in the open short task
-        for (int l = s.l0; l < s.l1; ++l) {
-            const uint32_t p0 = H.lvl_ptr[l], pl = p0 + H.lvl_long[l], p1 = H.lvl_ptr[l + 1];
-            if (pl > p0) run = 0;
-            for (uint32_t q = p0; q < pl; ++q) H.tasks.push_back(q | kLongTask);
-            for (uint32_t q = pl; q < p1; ++q) {
-                if (run == 0) H.tasks.push_back(q);
-                run = run + 1 == 64 ? 0 : run + 1;   // a wave of rows
+        for (uint32_t q = H.lvl_ptr[s.l0]; q < H.lvl_ptr[s.l1]; ++q) {
+            if (H.len[H.rows[q]] > kDfLaneRow) {
+                H.tasks.push_back(q | kLongTask);
+                run = 0;
+                continue;
             }
+            if (run == 0) H.tasks.push_back(q);
+            run = run + 1 == 64 ? 0 : run + 1;   // a wave of rows
         }
         s.k1 = (uint32_t)H.tasks.size();
     }
     H.tasks.push_back(H.lvl_ptr.back());
+}
+
+// Task-contiguous layout of one sweep for the pipelined dataflow kernels
+// (refinement sweeps of the batch).  entries(r, out) lists row r's
+// {column, CSC position} pairs in the order the row consumes them; pivot
+// (div sweeps) gives the CSC position of row r's divisor.
+struct DfHost {
+    std::vector<uint4> task;
+    std::vector<uint2> rl;
+    std::vector<uint32_t> idx, vpos;   // per slot
+    std::vector<uint32_t> pvpos;       // per position: pivot CSC position (div)
+};
+
+template <typename Entries, typename Pivot>
+void build_df(const TriHost &H, Entries entries, bool div, Pivot pivot, DfHost &D)
+{
+    const uint32_t n = (uint32_t)H.rows.size();
+    std::vector<std::vector<uint2>> ent(n);
+    D.rl.resize(n);
+    for (uint32_t p = 0; p < n; ++p) {
+        const uint32_t r = H.rows[p];
+        entries(r, ent[p]);
+        D.rl[p] = make_uint2(r, (uint32_t)ent[p].size());
+    }
+    auto put = [&](const uint2 &e) {
+        D.idx.push_back(e.x);
+        D.vpos.push_back(e.y);
+    };
+    uint32_t p = 0;
+    while (p < n) {
+        const uint32_t base = (uint32_t)D.idx.size();
+        if (ent[p].size() > kDfK) {
+            for (const uint2 &e : ent[p]) put(e);
+            D.task.push_back(make_uint4(base, p, kLongTask | 1u, (uint32_t)ent[p].size()));
+            ++p;
+            continue;
+        }
+        const uint32_t p0 = p;
+        uint32_t m = 0;
+        while (p < n && p - p0 < 64 && ent[p].size() <= kDfK) m = std::max(m, (uint32_t)ent[p++].size());
+        for (uint32_t e = 0; e < m; ++e)
+            for (uint32_t l = 0; l < 64; ++l) {
+                const uint32_t q = p0 + l;
+                if (q < p && e < ent[q].size()) put(ent[q][e]);
+                else put(make_uint2(0u, UINT32_MAX));
+            }
+        D.task.push_back(make_uint4(base, p0, p - p0, m));
+    }
+    if (div) {
+        D.pvpos.resize(n);
+        for (uint32_t q = 0; q < n; ++q) D.pvpos[q] = pivot(H.rows[q]);
+    }
 }
 
 struct Uploader {
@@ -707,8 +912,13 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(GLU_ERR_DEVICE, "no HIP device");
     HIP_TRY(hipGetDevice(&p->device));
+    p->lds_cap = launch::enable_lds();
+    Tail tail;
+    choose_tail(T, P.num_lev, P.nnz, (uint32_t)(p->lds_cap / sizeof(float)), tail);
+    p->tail_lev = tail.lev;
+    p->tail_nslot = (uint32_t)tail.pos.size();
     factor_segments(*p, T);
-    p->units = (int64_t)T.upd.size();
+    p->units = (int64_t)T.ctask.size();
 
     const uint32_t n = P.n;
     // Solve schedules.
@@ -728,10 +938,24 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
         tri_segments(*p, h);
         tri_tasks(h);
     }
+    DfHost dlo, dup;
+    build_df(p->tri[0], [&](uint32_t r, std::vector<uint2> &out) {
+        for (uint32_t q = P.csr_r_ptr[r]; q < P.csr_diag_ptr[r]; ++q) out.push_back(make_uint2(P.csr_c_idx[q], T.upos[q]));
+    }, false, [&](uint32_t) { return 0u; }, dlo);
+    build_df(p->tri[1], [&](uint32_t r, std::vector<uint2> &out) {
+        for (uint32_t q = P.csr_r_ptr[r + 1]; q > P.csr_diag_ptr[r] + 1; --q)
+            out.push_back(make_uint2(P.csr_c_idx[q - 1], T.upos[q - 1]));
+    }, true, [&](uint32_t r) { return P.l_col_ptr[r]; }, dup);
+    // per-matrix value copy: L slots, U slots, U pivots
+    std::vector<uint32_t> df_vpos(dlo.vpos);
+    df_vpos.insert(df_vpos.end(), dup.vpos.begin(), dup.vpos.end());
+    df_vpos.insert(df_vpos.end(), dup.pvpos.begin(), dup.pvpos.end());
+    p->df_nval = (uint32_t)df_vpos.size();
 
     // Refinement / refresh tables from the analysis.
     std::vector<uint32_t> a_rptr, a_cidx, a_src, lu_src, lu_col, a_rows, a_cols;
-    std::vector<double> rs_col;
+    std::vector<double> rs_col, a_rs, a_cs;
+    std::vector<uint32_t> a_dst;
     const Analysis *an = a ? &a->an : nullptr;
     if (an) {
         const uint32_t nnz_a = an->nnz;
@@ -760,6 +984,20 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
         lu_col.resize(P.nnz);
         for (uint32_t c = 0; c < n; ++c)
             for (uint32_t q = P.sym_c_ptr[c]; q < P.sym_c_ptr[c + 1]; ++q) lu_col[q] = c;
+        // the inverse map: every caller value lands on exactly one LU position
+        a_dst.assign(nnz_a, UINT32_MAX);
+        a_rs.assign(nnz_a, 0.0);
+        a_cs.assign(nnz_a, 0.0);
+        for (uint32_t q = 0; q < P.nnz; ++q) {
+            const uint32_t k = lu_src[q];
+            if (k == UINT32_MAX) continue;
+            if (k >= nnz_a || a_dst[k] != UINT32_MAX) return fail(GLU_ERR_INVALID, "value map is not one-to-one");
+            a_dst[k] = q;
+            a_rs[k] = rs_col[lu_col[q]];
+            a_cs[k] = an->col_scale_perm[P.sym_r_idx[q]];
+        }
+        for (uint32_t k = 0; k < nnz_a; ++k)
+            if (a_dst[k] == UINT32_MAX) return fail(GLU_ERR_INVALID, "a value of A has no LU position");
         // rows / columns of A with more than kLongRow entries first
         auto long_first = [&](const uint32_t *ptr, std::vector<uint32_t> &order, uint32_t &nlong) {
             order.clear();
@@ -783,8 +1021,11 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
     const size_t o_ccidx = U.put(P.csr_c_idx, P.nnz);
     const size_t o_cdiag = U.put(P.csr_diag_ptr, n);
     const size_t o_upos = U.put(T.upos);
-    const size_t o_upd = U.put(T.upd);
-    const size_t o_uptr = U.put(T.upd_ptr);
+    const size_t o_ctask = U.put(T.ctask);
+    const size_t o_cptr = U.put(T.ctask_ptr);
+    const size_t o_cln = U.put(T.cln);
+    const size_t o_ctgt = U.put(T.ctgt);
+    const size_t o_cup = U.put(T.cup);
     const size_t o_ment = U.put(T.ment);
     const size_t o_mptr = U.put(T.ment_ptr);
     const size_t o_mlong = U.put(T.ment_long);
@@ -792,6 +1033,16 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
     const size_t o_msrc = U.put(T.msrc);
     const size_t o_dvd = U.put(T.dvd);
     const size_t o_dptr = U.put(T.dvd_ptr);
+    const size_t o_tpos = U.put(tail.pos);
+    const size_t o_dlo_task = U.put(dlo.task), o_dlo_rl = U.put(dlo.rl), o_dlo_idx = U.put(dlo.idx);
+    const size_t o_dup_task = U.put(dup.task), o_dup_rl = U.put(dup.rl), o_dup_idx = U.put(dup.idx);
+    const size_t o_dfvpos = U.put(df_vpos);
+    const size_t o_tcln = U.put(tail.cln);
+    const size_t o_tctgt = U.put(tail.ctgt);
+    const size_t o_tcup = U.put(tail.cup);
+    const size_t o_tment = U.put(tail.ment);
+    const size_t o_tmsrc = U.put(tail.msrc);
+    const size_t o_tdvd = U.put(tail.dvd);
     size_t o_trows[4], o_tptr[4], o_tlong[4], o_ttask[4];
     for (int s = 0; s < 4; ++s) {
         o_trows[s] = U.put(p->tri[s].rows);
@@ -801,7 +1052,7 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
     }
     size_t o_rp = 0, o_rpi = 0, o_cp = 0, o_cpi = 0, o_rs = 0, o_csp = 0;
     size_t o_arptr = 0, o_acidx = 0, o_asrc = 0, o_acptr = 0, o_aridx = 0, o_lusrc = 0, o_rscol = 0;
-    size_t o_lucol = 0, o_arows = 0, o_acols = 0;
+    size_t o_lucol = 0, o_arows = 0, o_acols = 0, o_adst = 0, o_ars = 0, o_acs = 0;
     if (an) {
         o_rp = U.put(an->row_perm);
         o_rpi = U.put(an->row_perm_inv);
@@ -819,6 +1070,9 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
         o_lucol = U.put(lu_col);
         o_arows = U.put(a_rows);
         o_acols = U.put(a_cols);
+        o_adst = U.put(a_dst);
+        o_ars = U.put(a_rs);
+        o_acs = U.put(a_cs);
     }
     const size_t o_desc = U.reserve(sizeof(MatDesc));
     const size_t o_normbits = U.reserve(sizeof(unsigned int));
@@ -837,8 +1091,11 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
     auto u32p = [&](size_t o) { return reinterpret_cast<const uint32_t *>(A + o); };
     FactorDev &F = p->F;
     F.col_ptr = u32p(o_colptr);
-    F.upd = reinterpret_cast<const uint4 *>(A + o_upd);
-    F.upd_ptr = u32p(o_uptr);
+    F.ctask = reinterpret_cast<const uint2 *>(A + o_ctask);
+    F.ctask_ptr = u32p(o_cptr);
+    F.cln = reinterpret_cast<const uint4 *>(A + o_cln);
+    F.ctgt = u32p(o_ctgt);
+    F.cup = u32p(o_cup);
     F.ment = u32p(o_ment);
     F.ment_ptr = u32p(o_mptr);
     F.ment_long = u32p(o_mlong);
@@ -847,9 +1104,22 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
     F.dvd = reinterpret_cast<const uint2 *>(A + o_dvd);
     F.dvd_ptr = u32p(o_dptr);
     const uint32_t *upos_d = u32p(o_upos);
+    (void)upos_d;
     F.num_lev = P.num_lev;
     F.n = n;
     F.nnz = P.nnz;
+    // tail items are indexed like the full tables: shift each base back by
+    // the index of its first tail item (only indices >= that are used)
+    F.tail_lev = tail.lev;
+    F.tail_nslot = p->tail_nslot;
+    F.tail_pos = u32p(o_tpos);
+    auto shifted = [&](size_t off, size_t first, size_t item) { return (uintptr_t)(A + off) - first * item; };
+    F.t_cln = reinterpret_cast<const uint4 *>(shifted(o_tcln, (size_t)tail.c0 * 64, sizeof(uint4)));
+    F.t_ctgt = reinterpret_cast<const uint32_t *>(shifted(o_tctgt, tail.g0, sizeof(uint32_t)));
+    F.t_cup = reinterpret_cast<const uint32_t *>(shifted(o_tcup, tail.g0, sizeof(uint32_t)));
+    F.t_ment = reinterpret_cast<const uint32_t *>(shifted(o_tment, tail.m0, sizeof(uint32_t)));
+    F.t_msrc = reinterpret_cast<const MergeSrc *>(shifted(o_tmsrc, tail.s0, sizeof(MergeSrc)));
+    F.t_dvd = reinterpret_cast<const uint2 *>(shifted(o_tdvd, tail.d0, sizeof(uint2)));
 
     auto tri = [&](int s, const uint32_t *beg, uint32_t begoff, const uint32_t *end, const uint32_t *idx,
                    const uint32_t *vpos, const uint32_t *dpos, int desc) {
@@ -876,14 +1146,23 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
     S.up = tri(1, cdiag, 1, crptr + 1, ccidx, upos_d, diag, 1);
     S.tup = tri(2, colptr, 0, diag, rowidx, nullptr, diag, 0);
     S.tlo = tri(3, diag, 1, colptr + 1, rowidx, nullptr, nullptr, 0);
-    // the same L and U sweeps over the factors copied into CSR order
-    S.lo_csr = tri(0, crptr, 0, cdiag, ccidx, nullptr, nullptr, 0);
-    S.up_csr = tri(1, cdiag, 1, crptr + 1, ccidx, nullptr, cdiag, 1);
-    S.upos = upos_d;
-    {
-        const size_t need = 4 * (size_t)((n + 31) / 32);
-        p->tri_dataflow = launch::enable_lds(need) >= need;
-    }
+    // the same L and U sweeps, pipelined, over the per-matrix value copy
+    auto df = [&](const DfHost &h, size_t ot, size_t orl, size_t oi, int div, uint32_t voff) {
+        DfDev d{};
+        d.task = reinterpret_cast<const uint4 *>(A + ot);
+        d.rl = reinterpret_cast<const uint2 *>(A + orl);
+        d.idx = u32p(oi);
+        d.ntask = (uint32_t)h.task.size();
+        d.nslots = (uint32_t)h.idx.size();
+        d.div = div;
+        d.voff = voff;
+        return d;
+    };
+    S.lo_df = df(dlo, o_dlo_task, o_dlo_rl, o_dlo_idx, 0, 0);
+    S.up_df = df(dup, o_dup_task, o_dup_rl, o_dup_idx, 1, (uint32_t)dlo.idx.size());
+    S.df_vpos = u32p(o_dfvpos);
+    S.df_nval = p->df_nval;
+    p->tri_dataflow = p->lds_cap >= 4 * (size_t)((n + 31) / 32);
     if (an) {
         S.row_perm = u32p(o_rp);
         S.row_perm_inv = u32p(o_rpi);
@@ -905,6 +1184,9 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
         R.a_nlong = p->a_nlong;
         R.a_nlong_t = p->a_nlong_t;
         R.rs_col = reinterpret_cast<const double *>(A + o_rscol);
+        R.a_dst = u32p(o_adst);
+        R.a_rs = reinterpret_cast<const double *>(A + o_ars);
+        R.a_cs = reinterpret_cast<const double *>(A + o_acs);
         R.csp = S.col_scale_perm;
         R.nnz_a = p->nnz_a;
     }
@@ -951,6 +1233,7 @@ glu_status run_factor(glu_plan *p, float *d_val, int perturb, hipStream_t st)
         else
             launch::factor_grid(p->F, d_val, p->d_pert, s.l0, s.items, perturb, kGridThreads, st);
     }
+    if (p->tail_lev <= p->num_lev) launch::factor_tail(p->F, d_val, p->d_pert, perturb, p->fused_threads, st);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(GLU_ERR_DEVICE, std::string("factor launch: ") + hipGetErrorString(e));
     return GLU_OK;
@@ -983,12 +1266,10 @@ void run_sweeps(glu_plan *p, const float *lu, float *t, int transpose, hipStream
 // The richest sweep mode of the batch kernels whose LDS fits order n:
 // 2 = ready bits and correction vector in LDS, 1 = ready bits only,
 // 0 = level-synchronous sweeps.
-int sweep_mode(uint32_t n)
+int sweep_mode(uint32_t n, size_t cap)
 {
-    for (int mode = 2; mode > 0; --mode) {
-        const size_t need = launch::batch_lds_bytes(mode, n);
-        if (launch::enable_lds(need) >= need) return mode;
-    }
+    for (int mode = 2; mode > 0; --mode)
+        if (launch::batch_lds_bytes(mode, n) <= cap) return mode;
     return 0;
 }
 
@@ -1004,7 +1285,7 @@ glu_status ensure_batch(glu_plan *p, int batch)
     p->batch_cap = 0;
     HIP_TRY(hipMalloc(&p->d_bt32, sizeof(float) * (size_t)p->n * batch));
     HIP_TRY(hipMalloc(&p->d_br64, sizeof(double) * (size_t)p->n * batch));
-    HIP_TRY(hipMalloc(&p->d_blucsr, sizeof(float) * (size_t)p->nnz * batch));
+    HIP_TRY(hipMalloc(&p->d_blucsr, sizeof(float) * (size_t)p->df_nval * batch));
     p->batch_cap = batch;
     return GLU_OK;
 }
@@ -1035,7 +1316,7 @@ extern "C" glu_status glu_plan_create_from_analysis(const glu_analysis *a, const
 
 extern "C" void glu_plan_destroy(glu_plan *p) { delete p; }
 
-extern "C" glu_status glu_plan_info(const glu_plan *p, int64_t info[8])
+extern "C" glu_status glu_plan_info(const glu_plan *p, int64_t info[10])
 {
     if (!p || !info) return fail(GLU_ERR_ARGUMENT, "null argument");
     info[0] = p->n;
@@ -1046,6 +1327,8 @@ extern "C" glu_status glu_plan_info(const glu_plan *p, int64_t info[8])
     info[5] = (int64_t)p->fsegs.size();
     info[6] = (int64_t)p->arena_bytes;
     info[7] = p->units;
+    info[8] = p->tail_lev;
+    info[9] = p->tail_nslot;
     return GLU_OK;
 }
 
@@ -1218,10 +1501,11 @@ extern "C" glu_status glu_batch_solve(glu_plan *p, int batch, const double *d_ax
     if (!p->has_refine) return fail(GLU_ERR_ARGUMENT, "plan has no analysis");
     glu_status s = ensure_batch(p, batch);
     if (s != GLU_OK) return s;
-    const int mode = sweep_mode(p->n);
+    const size_t cap = launch::enable_lds();
+    const int mode = sweep_mode(p->n, cap);
+    const size_t lds = std::max(launch::batch_lds_bytes(mode, p->n), sizeof(float) * p->tail_nslot);
     launch::batch_same(p->d_desc, batch, d_ax, d_b, d_x, d_lu, p->d_blucsr, p->d_bt32, p->d_br64, d_iters, d_eta,
-                       tol, maxit, (int)(flags & 1u), p->fused_threads, mode, launch::batch_lds_bytes(mode, p->n),
-                       static_cast<hipStream_t>(stream));
+                       tol, maxit, (int)(flags & 1u), p->fused_threads, mode, lds, static_cast<hipStream_t>(stream));
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(GLU_ERR_DEVICE, std::string("batch launch: ") + hipGetErrorString(e));
     return GLU_OK;
@@ -1286,7 +1570,7 @@ extern "C" glu_status glu_group_create(glu_plan *const *plans, int count, glu_gr
         g->voff.push_back(vtot);
         vtot += p->n;
         g->foff.push_back(ftot);
-        ftot += p->nnz;
+        ftot += p->df_nval;
         g->threads = std::max(g->threads, p->fused_threads);
         // heaviest first: levels dominate the latency, updates the work
         work[i] = {-(int64_t)(p->num_lev + p->tri[0].lvl_ptr.size() + p->tri[1].lvl_ptr.size()) * 64 - p->updates,
@@ -1324,11 +1608,16 @@ extern "C" glu_status glu_group_solve(glu_group *g, const double *const *d_ax, c
     }
     hipStream_t st = static_cast<hipStream_t>(stream);
     HIP_TRY(hipMemcpyAsync(g->d_ios, ios.data(), sizeof(MatIOHost) * count, hipMemcpyHostToDevice, st));
-    size_t nmax = 0;
-    for (glu_plan *p : g->plans) nmax = std::max<size_t>(nmax, p->n);
-    const int mode = sweep_mode((uint32_t)nmax);
-    launch::batch_group(g->d_descs, g->d_ios, g->d_order, count, tol, maxit, (int)(flags & 1u), g->threads, mode,
-                        launch::batch_lds_bytes(mode, (uint32_t)nmax), st);
+    size_t nmax = 0, tmax = 0;
+    for (glu_plan *p : g->plans) {
+        nmax = std::max<size_t>(nmax, p->n);
+        tmax = std::max<size_t>(tmax, sizeof(float) * p->tail_nslot);
+    }
+    const size_t cap = launch::enable_lds();
+    const int mode = sweep_mode((uint32_t)nmax, cap);
+    const size_t lds = std::max(launch::batch_lds_bytes(mode, (uint32_t)nmax), tmax);
+    launch::batch_group(g->d_descs, g->d_ios, g->d_order, count, tol, maxit, (int)(flags & 1u), g->threads, mode, lds,
+                        st);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(GLU_ERR_DEVICE, std::string("group launch: ") + hipGetErrorString(e));
     // ios is read by the kernel from device memory; the host copy may go.

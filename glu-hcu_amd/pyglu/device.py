@@ -69,10 +69,10 @@ class Plan:
             capi.check(L.glu_plan_create(C.byref(pat), C.byref(opts), C.byref(h)), "glu_plan_create")
             self.analysis = None
         self._h = h
-        info = np.zeros(8, dtype=np.int64)
+        info = np.zeros(10, dtype=np.int64)
         capi.check(L.glu_plan_info(h, capi.ptr(info)), "glu_plan_info")
         self.n, self.nnz_lu, self.nnz_a, self.num_lev, self.updates, self.segments, \
-            self.device_bytes, self.units = (int(v) for v in info)
+            self.device_bytes, self.units, self.tail_level, self.tail_values = (int(v) for v in info)
 
     @property
     def handle(self):

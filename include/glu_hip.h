@@ -124,8 +124,9 @@ void glu_plan_destroy(glu_plan *p);
 
 /* info[0]=n info[1]=nnz_lu info[2]=nnz_a info[3]=num_lev info[4]=updates
  * info[5]=segments in the factor schedule info[6]=device bytes held
- * info[7]=units */
-glu_status glu_plan_info(const glu_plan *p, int64_t info[8]);
+ * info[7]=units info[8]=first step of the LDS-resident tail (num_lev + 1:
+ * none) info[9]=values the tail stages in LDS */
+glu_status glu_plan_info(const glu_plan *p, int64_t info[10]);
 
 /* ------------------------------------------------------------------------
  * Numeric factorization (FP32, in place, L unit-lower strictly below the
