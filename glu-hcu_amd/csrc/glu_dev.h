@@ -52,12 +52,11 @@ constexpr uint32_t kFirstBit = 0x80000000u;
 struct FactorDev {
     const uint32_t *col_ptr;   // sym_c_ptr [n+1] (GESP column sums)
     // Column tasks, by level (tasks ctask_ptr[l] .. ctask_ptr[l+1]).  Lane
-    // `lane` of task t owns one L entry of a pivot column k of the level:
-    // cln[64 t + lane] = {L(i,k) position, pivot position, |U_k|, 0}; its
-    // e-th update (e < |U_k|) reads U(k,j) at cup[ctask[t].x + 64 e + lane]
-    // and targets ctgt[same index] (~0u: a merge entry does it);
-    // ctask[t].y = the longest U list of the task.
-    const uint2 *ctask;
+    // `lane` of task t owns one L entry of a pivot column k of the level and
+    // up to four consecutive entries of U_k: cln[64 t + lane] = {L(i,k)
+    // position, pivot position, update count <= 4, 0}; its e-th update reads
+    // U(k,j) at cup[256 t + 64 e + lane] and targets ctgt[same index] (~0u:
+    // a merge entry does it).
     const uint32_t *ctask_ptr; // [num_lev + 2]
     const uint4 *cln;
     const uint32_t *ctgt;
@@ -115,16 +114,19 @@ struct TriDev {
 };
 
 // A pipelined dataflow sweep over a per-matrix copy of the factor values
-// laid out task by task (the batch refinement).  Task k = {base, p0,
-// rows | kLongTask, m}:
-//  - rows of up to kDfK entries, one per lane: lane l takes position p0 + l
-//    (l < rows), its entry e at slot base + 64 e + l (e < m, m = the task's
-//    longest row; shorter rows leave padding slots);
-//  - one longer row (kLongTask): position p0, entry e at slot base + e.
+// laid out task by task (the batch refinement).  Task k = {base, p0, z, m}:
+//  - z = rows (< 2^24), no kLongTask bit: rows of up to kDfK entries, one
+//    per lane: lane l takes position p0 + l (l < rows), its entry e at slot
+//    base + 64 e + l (e < m, m = the task's longest row; shorter rows leave
+//    padding slots);
+//  - z = kLongTask | log2(gw) << 24 | rows: rows of one level (so no row of
+//    the task depends on another) taken gw = 16, 32 or 64 lanes each: row
+//    g at position p0 + g, its entry j at slot base + gw g + j; a 64-lane
+//    row (rows = 1) may have more than 64 entries, at base + j.
 // Each position's {row, entries} is in rl; entries are stored in the order
 // the row consumes them.  div: U sweeps, the pivot of position p is value
 // nslots + p.
-constexpr uint32_t kDfK = 8;
+constexpr uint32_t kDfK = 6;
 
 struct DfDev {
     const uint4 *task;         // [ntask]

@@ -84,34 +84,32 @@ __device__ __forceinline__ float gesp(float d, float pert, int perturb)
 }
 
 // Column tasks.  A wave takes a task; each lane owns one L entry A(i,k) of a
-// pivot column k of the level, forms l = A(i,k) / A(k,k) (A(k,k)
-// GESP-perturbed) once and, for each U entry u = A(k,j) of the column,
+// pivot column k of the level and up to four entries u = A(k,j) of its U
+// list, forms l = A(i,k) / A(k,k) (A(k,k) GESP-perturbed) and applies
 //   A(i,j) += -(l * u)
 // Each target here receives exactly this one update in this level, so the
 // fire-and-forget atomic add gives an order-independent result.
 //
 // CTab holds a task's table data for this lane (independent of the values,
 // so it can be loaded ahead): the L and pivot positions, the update count,
-// and the first four U positions and targets.
+// the U positions and the targets.
 struct CTab {
-    uint32_t lpos, dpos, cnt, base;
+    uint32_t lpos, dpos, cnt;
     uint32_t tt[4], up[4];
 };
 
 __device__ __forceinline__ void ctab_load(const FactorDev &F, uint32_t t, uint32_t lane, CTab &c)
 {
-    const uint2 h = F.ctask[t];
     const uint4 ln = F.cln[(size_t)t * kWave + lane];
     c.lpos = ln.x;
     c.dpos = ln.y;
     c.cnt = ln.z;
-    c.base = h.x + lane;
+    const size_t q = (size_t)t * (4 * kWave) + lane;
 #pragma unroll
-    for (uint32_t u = 0; u < 4; ++u)
-        if (u < h.y) {
-            c.tt[u] = F.ctgt[c.base + u * kWave];
-            c.up[u] = F.cup[c.base + u * kWave];
-        }
+    for (uint32_t u = 0; u < 4; ++u) {
+        c.tt[u] = F.ctgt[q + u * kWave];
+        c.up[u] = F.cup[q + u * kWave];
+    }
 }
 
 template <int Scope>
@@ -128,22 +126,6 @@ __device__ __forceinline__ void ctab_run(const FactorDev &F, float *__restrict__
 #pragma unroll
     for (uint32_t u = 0; u < 4; ++u)
         if (u < c.cnt && c.tt[u] != ~0u) atomic_sub<Scope>(val + c.tt[u], l * uv[u]);
-    for (uint32_t e = 4; e < c.cnt; e += 4) {
-        uint32_t tt[4], up[4];
-#pragma unroll
-        for (uint32_t u = 0; u < 4; ++u)
-            if (e + u < c.cnt) {
-                tt[u] = F.ctgt[c.base + (e + u) * kWave];
-                up[u] = F.cup[c.base + (e + u) * kWave];
-            }
-        float v[4];
-#pragma unroll
-        for (uint32_t u = 0; u < 4; ++u)
-            if (e + u < c.cnt) v[u] = val[up[u]];
-#pragma unroll
-        for (uint32_t u = 0; u < 4; ++u)
-            if (e + u < c.cnt && tt[u] != ~0u) atomic_sub<Scope>(val + tt[u], l * v[u]);
-    }
 }
 
 // Column tasks [c0, c0 + nc) of one level for a group of cooperating
@@ -278,7 +260,12 @@ __device__ __forceinline__ void factor_levels_wg(const FactorDev &F, float *__re
     const uint32_t tid = threadIdx.x, nt = blockDim.x;
     const uint32_t lane = tid & (kWave - 1), w = tid / kWave, nw = nt / kWave;
     CTab cur;
-    if (l0 < l1 && F.ctask_ptr[l0] + w < F.ctask_ptr[l0 + 1]) ctab_load(F, F.ctask_ptr[l0] + w, lane, cur);
+    uint2 dv = make_uint2(0u, 0u);   // this lane's first division item of the step
+    if (l0 < l1) {
+        if (F.ctask_ptr[l0] + w < F.ctask_ptr[l0 + 1]) ctab_load(F, F.ctask_ptr[l0] + w, lane, cur);
+        const LevelSpan s = level_span(F, l0);
+        if (tid < s.nd) dv = F.dvd[s.d0 + tid];
+    }
     for (int lvl = l0; lvl < l1; ++lvl) {
         const LevelSpan s = level_span(F, lvl);
         for (uint32_t e = w; e < s.ml; e += nw) factor_merge_wave(F, val, s.m0 + e, pert, perturb, lane, lds);
@@ -290,10 +277,12 @@ __device__ __forceinline__ void factor_levels_wg(const FactorDev &F, float *__re
             if (more) cur = nxt;
         }
         for (uint32_t t = s.ml + tid; t < s.nm; t += nt) factor_merge(F, val, s.m0 + t, pert, perturb);
-        for (uint32_t t = tid; t < s.nd; t += nt) factor_divide(val, F.dvd[s.d0 + t], pert, perturb);
-        if (lvl + 1 < l1) {
-            const uint32_t n0 = F.ctask_ptr[lvl + 1], n1 = F.ctask_ptr[lvl + 2];
-            if (n0 + w < n1) ctab_load(F, n0 + w, lane, cur);
+        if (tid < s.nd) factor_divide(val, dv, pert, perturb);
+        for (uint32_t t = tid + nt; t < s.nd; t += nt) factor_divide(val, F.dvd[s.d0 + t], pert, perturb);
+        if (lvl + 1 < l1) {   // the next step's table data, loaded across the barrier
+            const LevelSpan q = level_span(F, lvl + 1);
+            if (q.c0 + w < q.c0 + q.nc) ctab_load(F, q.c0 + w, lane, cur);
+            if (tid < q.nd) dv = F.dvd[q.d0 + tid];
         }
         __syncthreads();
     }
@@ -926,6 +915,7 @@ struct MatIO {
     float *t32;         // n scratch
     double *r64;        // n scratch
     float *lucsr;       // df_nval scratch: the sweeps' value copy
+    double *norms;      // 2 scratch: |A|_inf, |b|_inf (factor -> refine kernel)
     int32_t *iters;
     double *eta;
 };
@@ -978,7 +968,6 @@ __device__ __forceinline__ T *glb(T *p)
 __device__ __forceinline__ void globalize(FactorDev &F)
 {
     F.col_ptr = glb(F.col_ptr);
-    F.ctask = glb(F.ctask);
     F.ctask_ptr = glb(F.ctask_ptr);
     F.cln = glb(F.cln);
     F.ctgt = glb(F.ctgt);
@@ -1047,6 +1036,7 @@ __device__ __forceinline__ void globalize(MatIO &io)
     io.t32 = glb(io.t32);
     io.r64 = glb(io.r64);
     if (io.lucsr) io.lucsr = glb(io.lucsr);
+    io.norms = glb(io.norms);
     io.iters = glb(io.iters);
     io.eta = glb(io.eta);
 }
@@ -1061,8 +1051,27 @@ __device__ __forceinline__ void globalize(MatIO &io)
 // task j + 1's rows, columns and values while it runs task j.  Summation
 // order within a row is free here (a correction solve).
 
+// Sum of v over each group of gw = 16, 32 or 64 lanes, in every lane of the
+// group (gw = 64: uniform).  Rows of 16 by DPP lane permutes (quad
+// swaps, half-row and row mirrors), then one 32-lane swizzle or the four
+// row sums.
+__device__ __forceinline__ float group_sum(float v, uint32_t gw)
+{
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xf, 0xf, false));   // quad [1,0,3,2]
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4E, 0xf, 0xf, false));   // quad [2,3,0,1]
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x141, 0xf, 0xf, false));  // row half mirror
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x140, 0xf, 0xf, false));  // row mirror
+    if (gw == 32) v += __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x401F));        // lane ^ 16
+    if (gw == 64)
+        v = (__int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 15)) +
+             __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 31))) +
+            (__int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 47)) +
+             __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63)));
+    return v;
+}
+
 struct DfTask {
-    uint32_t base, p0, cnt, m;
+    uint32_t base, p0, cnt, m, lg;   // lg: log2 lanes per row of a wave task
     bool wave;
     uint32_t row, len;
     uint32_t idx[kDfK];
@@ -1076,7 +1085,8 @@ __device__ __forceinline__ void df2_issue(const DfDev &D, const float *__restric
     t.base = d.x;
     t.p0 = d.y;
     t.wave = (d.z & kLongTask) != 0;
-    t.cnt = d.z & ~kLongTask;
+    t.cnt = d.z & 0xffffffu;
+    t.lg = (d.z >> 24) & 0x7fu;
     t.m = d.w;
     t.dval = 1.f;
     if (!t.wave) {
@@ -1085,21 +1095,22 @@ __device__ __forceinline__ void df2_issue(const DfDev &D, const float *__restric
         t.row = rl.x;
         t.len = rl.y;
 #pragma unroll
-        for (uint32_t e = 0; e < kDfK; ++e)
-            if (e < t.m) {
-                t.idx[e] = D.idx[t.base + e * kWave + lane];
-                t.val[e] = sv[t.base + e * kWave + lane];
-            }
+        for (uint32_t e = 0; e < kDfK; ++e) {   // unused entries read row 0's ready bit
+            t.idx[e] = e < t.m ? D.idx[t.base + e * kWave + lane] : 0u;
+            t.val[e] = e < t.m ? sv[t.base + e * kWave + lane] : 0.f;
+        }
         if (D.div && on) t.dval = sv[D.nslots + t.p0 + lane];
     } else {
-        const uint2 rl = D.rl[t.p0];
+        // every lane's first slot exists (groups and 64-lane rows are padded),
+        // so the entry loads need not wait for the row lengths
+        const uint32_t g = lane >> t.lg;
+        const bool on = g < t.cnt;
+        const uint2 rl = on ? D.rl[t.p0 + g] : make_uint2(0u, 0u);
         t.row = rl.x;
         t.len = rl.y;
-        if (lane < t.len) {
-            t.idx[0] = D.idx[t.base + lane];
-            t.val[0] = sv[t.base + lane];
-        }
-        if (D.div) t.dval = sv[D.nslots + t.p0];
+        t.idx[0] = D.idx[t.base + lane];
+        t.val[0] = sv[t.base + lane];
+        if (D.div && on) t.dval = sv[D.nslots + t.p0 + g];
     }
 }
 
@@ -1113,15 +1124,25 @@ __device__ __forceinline__ void df2_run(const DfDev &D, const float *__restrict_
         while (__ballot(pending)) {
             bool progressed = false;
             if (pending) {
+                // every entry's ready bit and x value read without branches
+                // (independent LDS reads, one wait each), then the ready
+                // prefix consumed in order
+                uint32_t fw[kDfK];
+#pragma unroll
+                for (uint32_t e = 0; e < kDfK; ++e)
+                    fw[e] = __hip_atomic_load(flags + (t.idx[e] >> 5), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                 uint32_t mask = 0;
 #pragma unroll
                 for (uint32_t e = 0; e < kDfK; ++e)
-                    if (e >= ci && e < t.len && df_peek(flags, t.idx[e])) mask |= 1u << e;
+                    mask |= (((fw[e] >> (t.idx[e] & 31u)) & 1u) & (uint32_t)(e >= ci && e < t.len)) << e;
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                float xv[kDfK];
+#pragma unroll
+                for (uint32_t e = 0; e < kDfK; ++e) xv[e] = x[t.idx[e]];
 #pragma unroll
                 for (uint32_t e = 0; e < kDfK; ++e)
                     if (e == ci && ((mask >> e) & 1u)) {
-                        acc = acc - t.val[e] * x[t.idx[e]];
+                        acc = acc - t.val[e] * xv[e];
                         ++ci;
                         progressed = true;
                     }
@@ -1137,18 +1158,33 @@ __device__ __forceinline__ void df2_run(const DfDev &D, const float *__restrict_
         }
         return;
     }
+    // groups of gw lanes, a row each (no row depends on another of the
+    // task); only a 64-lane row may need more than one pass, and then loads
+    // four passes at a time
+    const uint32_t gw = 1u << t.lg, j = lane & (gw - 1u);
     float sum = 0.f;
-    for (uint32_t c0 = 0; c0 < t.len; c0 += kWave) {
-        const uint32_t e = c0 + lane;
-        if (e < t.len) {
-            const uint32_t col = c0 ? D.idx[t.base + e] : t.idx[0];
-            const float v = c0 ? sv[t.base + e] : t.val[0];
-            while (!df_ready(flags, col)) __builtin_amdgcn_s_sleep(1);
-            sum += v * x[col];
-        }
+    if (j < t.len) {
+        while (!df_ready(flags, t.idx[0])) __builtin_amdgcn_s_sleep(1);
+        sum = t.val[0] * x[t.idx[0]];
     }
-    sum = wave_sum(sum);
-    if (lane == 0) {
+    for (uint32_t c0 = kWave; c0 < t.len; c0 += 4 * kWave) {   // gw == 64 here
+        uint32_t col[4];
+        float v[4];
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u)
+            if (c0 + u * kWave + j < t.len) {
+                col[u] = D.idx[t.base + c0 + u * kWave + j];
+                v[u] = sv[t.base + c0 + u * kWave + j];
+            }
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u)
+            if (c0 + u * kWave + j < t.len) {
+                while (!df_ready(flags, col[u])) __builtin_amdgcn_s_sleep(1);
+                sum += v[u] * x[col[u]];
+            }
+    }
+    sum = group_sum(sum, gw);
+    if (j == 0 && (lane >> t.lg) < t.cnt) {
         float acc = x[t.row] - sum;
         if (D.div) acc = acc / t.dval;
         x[t.row] = acc;
@@ -1172,16 +1208,23 @@ __device__ __forceinline__ void df2_sweep(const DfDev &D, const float *__restric
         return make_uint4((uint32_t)__builtin_amdgcn_readlane((int)dl.x, l), (uint32_t)__builtin_amdgcn_readlane((int)dl.y, l),
                           (uint32_t)__builtin_amdgcn_readlane((int)dl.z, l), (uint32_t)__builtin_amdgcn_readlane((int)dl.w, l));
     };
+    // two tasks ahead, in three rotating register sets (no copies, which
+    // would wait for loads still in flight)
     descs(0);
-    DfTask cur, nxt;
-    df2_issue(D, sv, desc(0), lane, cur);
-    for (uint32_t j = 0; j < mine; ++j) {
-        if (j + 1 < mine) {
-            if (((j + 1) & (kWave - 1)) == 0) descs(j + 1);
-            df2_issue(D, sv, desc(j + 1), lane, nxt);
+    DfTask ta, tb, tc;
+    df2_issue(D, sv, desc(0), lane, ta);
+    if (1 < mine) df2_issue(D, sv, desc(1), lane, tb);
+    auto step = [&](uint32_t j, DfTask &cur, DfTask &slot) {
+        if (j + 2 < mine) {
+            if (((j + 2) & (kWave - 1)) == 0) descs(j + 2);
+            df2_issue(D, sv, desc(j + 2), lane, slot);
         }
-        df2_run(D, sv, x, flags, cur, lane);
-        cur = nxt;
+        if (j < mine) df2_run(D, sv, x, flags, cur, lane);
+    };
+    for (uint32_t j = 0; j < mine; j += 3) {
+        step(j, ta, tc);
+        step(j + 1, tb, ta);
+        step(j + 2, tc, tb);
     }
 }
 
@@ -1192,15 +1235,14 @@ enum : int {
     kSweepLds = 2,      // dataflow sweeps, ready bits and vector in LDS
 };
 
-// One workgroup runs one matrix end to end: value refresh, GESP value,
-// factorization, then refinement sweeps until eta <= tol or maxit.  xs is
-// the FP32 correction vector; flags the sweeps' ready bits (kSweepLevels:
-// unused).  Md is the matrix's descriptor in device memory: each phase
-// takes the (globalized) tables it needs from it, so the descriptor is not
-// held in registers as a whole.
-template <int kMode>
-__device__ __forceinline__ void wg_pipeline(const MatDesc *Md, const MatIO &io, double tol, int maxit, int perturb,
-                                            double *sh, float *xs, uint32_t *flags, float *wlds)
+// One workgroup runs one matrix: value refresh, GESP value, factorization
+// and the copy of the factors for the sweeps (wg_factor), then, in a second
+// kernel with its own register budget, refinement sweeps until eta <= tol
+// or maxit (wg_refine).  Md is the matrix's descriptor in device memory:
+// each phase takes the (globalized) tables it needs from it, so the
+// descriptor is not held in registers as a whole.
+template <bool kCopy>
+__device__ __forceinline__ void wg_factor(const MatDesc *Md, const MatIO &io, int perturb, double *sh, float *wlds)
 {
     const uint32_t n = Md->n, nnz = Md->nnz;
     const uint32_t tid = threadIdx.x, nt = blockDim.x;
@@ -1267,7 +1309,7 @@ __device__ __forceinline__ void wg_pipeline(const MatDesc *Md, const MatIO &io, 
     }
 
     // 4. the factors copied into the sweeps' task-contiguous layout
-    if (kMode != kSweepLevels) {
+    if (kCopy) {
         const uint32_t *vpos = glb(Md->s.df_vpos);
         const uint32_t nval = Md->s.df_nval;
         constexpr int U = 8;
@@ -1284,8 +1326,22 @@ __device__ __forceinline__ void wg_pipeline(const MatDesc *Md, const MatIO &io, 
         }
         __syncthreads();
     }
+    if (tid == 0) {
+        io.norms[0] = anorm;
+        io.norms[1] = bnorm;
+    }
+}
 
-    // 5. refinement: x += LU^-1 r (FP32), r = b - A x (FP64)
+// Refinement of one matrix by one workgroup: x += LU^-1 r (FP32 sweeps),
+// r = b - A x (FP64), until eta <= tol or maxit.  xs is the FP32
+// correction vector; flags the sweeps' ready bits (kSweepLevels: unused).
+template <int kMode>
+__device__ __forceinline__ void wg_refine(const MatDesc *Md, const MatIO &io, double tol, int maxit, double *sh,
+                                          float *xs, uint32_t *flags)
+{
+    const uint32_t n = Md->n;
+    const uint32_t tid = threadIdx.x, nt = blockDim.x;
+    const double anorm = io.norms[0], bnorm = io.norms[1];
     const uint32_t nwords = (n + 31) / 32;
     int it = 0;
     double eta = 0.0;
@@ -1357,15 +1413,10 @@ __device__ __forceinline__ void batch_lds(uint32_t n, float *t32, uint32_t *&fla
 }
 
 // Same pattern: matrix b = blockIdx.x uses desc[0] and contiguous buffers.
-template <int kMode>
-__global__ void __launch_bounds__(1024) k_batch_same(const MatDesc *__restrict__ desc, const double *__restrict__ ax,
-                             const double *__restrict__ b, double *__restrict__ x, float *__restrict__ lu,
-                             float *__restrict__ lucsr, float *__restrict__ t32, double *__restrict__ r64,
-                             int32_t *__restrict__ iters, double *__restrict__ eta, double tol, int maxit,
-                             int perturb)
+__device__ __forceinline__ MatIO batch_io(const MatDesc *desc, const double *ax, const double *b, double *x,
+                                          float *lu, float *lucsr, float *t32, double *r64, double *norms,
+                                          int32_t *iters, double *eta)
 {
-    __shared__ double sh[1024 / kWave];
-    __shared__ float wl[1024 / kWave][kWave];
     const uint32_t n = desc->n, nnz = desc->nnz, nnz_a = desc->r.nnz_a, nval = desc->s.df_nval;
     const uint64_t m = blockIdx.x;
     MatIO io;
@@ -1376,29 +1427,74 @@ __global__ void __launch_bounds__(1024) k_batch_same(const MatDesc *__restrict__
     io.t32 = t32 + m * n;
     io.r64 = r64 + m * n;
     io.lucsr = lucsr ? lucsr + m * nval : nullptr;
+    io.norms = norms + 2 * m;
     io.iters = iters + m;
     io.eta = eta + m;
     globalize(io);
+    return io;
+}
+
+// Dynamic LDS: the factor tail's slots.
+template <bool kCopy>
+__global__ void __launch_bounds__(1024) k_batch_same_factor(const MatDesc *__restrict__ desc,
+                                                            const double *__restrict__ ax, const double *__restrict__ b,
+                                                            double *__restrict__ x, float *__restrict__ lu,
+                                                            float *__restrict__ lucsr, float *__restrict__ t32,
+                                                            double *__restrict__ r64, double *__restrict__ norms,
+                                                            int32_t *__restrict__ iters, double *__restrict__ eta,
+                                                            int perturb)
+{
+    __shared__ double sh[1024 / kWave];
+    __shared__ float wl[1024 / kWave][kWave];
+    const MatIO io = batch_io(desc, ax, b, x, lu, lucsr, t32, r64, norms, iters, eta);
+    wg_factor<kCopy>(desc, io, perturb, sh, wl[threadIdx.x / kWave]);
+}
+
+// Dynamic LDS: the sweeps' ready bits (and correction vector).
+template <int kMode>
+__global__ void __launch_bounds__(1024) k_batch_same_refine(const MatDesc *__restrict__ desc,
+                                                            const double *__restrict__ ax, const double *__restrict__ b,
+                                                            double *__restrict__ x, float *__restrict__ lu,
+                                                            float *__restrict__ lucsr, float *__restrict__ t32,
+                                                            double *__restrict__ r64, double *__restrict__ norms,
+                                                            int32_t *__restrict__ iters, double *__restrict__ eta,
+                                                            double tol, int maxit)
+{
+    __shared__ double sh[1024 / kWave];
+    const MatIO io = batch_io(desc, ax, b, x, lu, lucsr, t32, r64, norms, iters, eta);
     uint32_t *flags;
     float *xs;
-    batch_lds<kMode>(n, io.t32, flags, xs);
-    wg_pipeline<kMode>(desc, io, tol, maxit, perturb, sh, xs, flags, wl[threadIdx.x / kWave]);
+    batch_lds<kMode>(desc->n, io.t32, flags, xs);
+    wg_refine<kMode>(desc, io, tol, maxit, sh, xs, flags);
 }
 
 // Distinct patterns: workgroup g runs matrix order[g] with its own tables.
-template <int kMode>
-__global__ void __launch_bounds__(1024) k_batch_group(const MatDesc *__restrict__ descs, const MatIO *__restrict__ ios,
-                              const uint32_t *__restrict__ order, double tol, int maxit, int perturb)
+template <bool kCopy>
+__global__ void __launch_bounds__(1024) k_batch_group_factor(const MatDesc *__restrict__ descs,
+                                                             const MatIO *__restrict__ ios,
+                                                             const uint32_t *__restrict__ order, int perturb)
 {
     __shared__ double sh[1024 / kWave];
     __shared__ float wl[1024 / kWave][kWave];
     const uint32_t m = order[blockIdx.x];
     MatIO io = ios[m];
     globalize(io);
+    wg_factor<kCopy>(descs + m, io, perturb, sh, wl[threadIdx.x / kWave]);
+}
+
+template <int kMode>
+__global__ void __launch_bounds__(1024) k_batch_group_refine(const MatDesc *__restrict__ descs,
+                                                             const MatIO *__restrict__ ios,
+                                                             const uint32_t *__restrict__ order, double tol, int maxit)
+{
+    __shared__ double sh[1024 / kWave];
+    const uint32_t m = order[blockIdx.x];
+    MatIO io = ios[m];
+    globalize(io);
     uint32_t *flags;
     float *xs;
     batch_lds<kMode>(descs[m].n, io.t32, flags, xs);
-    wg_pipeline<kMode>(descs + m, io, tol, maxit, perturb, sh, xs, flags, wl[threadIdx.x / kWave]);
+    wg_refine<kMode>(descs + m, io, tol, maxit, sh, xs, flags);
 }
 
 // ---------------------------------------------------------------------------
@@ -1477,31 +1573,43 @@ size_t batch_lds_bytes(int mode, uint32_t n)
     return mode == kSweepLevels ? 0 : 4 * words + (mode == kSweepLds ? 4 * (size_t)n : 0);
 }
 
+// Two launches: factor (dynamic LDS fact_lds: the tail's slots), then
+// refine in sweep mode `mode` (dynamic LDS ref_lds).
 void batch_same(const MatDesc *desc, int batch, const double *ax, const double *b, double *x, float *lu,
-                float *lucsr, float *t32, double *r64, int32_t *iters, double *eta, double tol, int maxit,
-                int perturb, int threads, int mode, size_t lds_bytes, hipStream_t st)
+                float *lucsr, float *t32, double *r64, double *norms, int32_t *iters, double *eta, double tol,
+                int maxit, int perturb, int threads, int mode, size_t fact_lds, size_t ref_lds, hipStream_t st)
 {
-    if (mode == kSweepLds)
-        k_batch_same<kSweepLds><<<batch, threads, lds_bytes, st>>>(desc, ax, b, x, lu, lucsr, t32, r64, iters, eta,
-                                                                  tol, maxit, perturb);
-    else if (mode == kSweepFlags)
-        k_batch_same<kSweepFlags><<<batch, threads, lds_bytes, st>>>(desc, ax, b, x, lu, lucsr, t32, r64, iters,
-                                                                    eta, tol, maxit, perturb);
+    if (mode == kSweepLevels)
+        k_batch_same_factor<false><<<batch, threads, fact_lds, st>>>(desc, ax, b, x, lu, lucsr, t32, r64, norms,
+                                                                      iters, eta, perturb);
     else
-        k_batch_same<kSweepLevels><<<batch, threads, lds_bytes, st>>>(desc, ax, b, x, lu, lucsr, t32, r64, iters,
-                                                                     eta, tol, maxit, perturb);
+        k_batch_same_factor<true><<<batch, threads, fact_lds, st>>>(desc, ax, b, x, lu, lucsr, t32, r64, norms,
+                                                                     iters, eta, perturb);
+    if (mode == kSweepLds)
+        k_batch_same_refine<kSweepLds><<<batch, threads, ref_lds, st>>>(desc, ax, b, x, lu, lucsr, t32, r64, norms,
+                                                                         iters, eta, tol, maxit);
+    else if (mode == kSweepFlags)
+        k_batch_same_refine<kSweepFlags><<<batch, threads, ref_lds, st>>>(desc, ax, b, x, lu, lucsr, t32, r64,
+                                                                           norms, iters, eta, tol, maxit);
+    else
+        k_batch_same_refine<kSweepLevels><<<batch, threads, ref_lds, st>>>(desc, ax, b, x, lu, lucsr, t32, r64,
+                                                                            norms, iters, eta, tol, maxit);
 }
 
 void batch_group(const MatDesc *descs, const void *ios, const uint32_t *order, int count, double tol,
-                 int maxit, int perturb, int threads, int mode, size_t lds_bytes, hipStream_t st)
+                 int maxit, int perturb, int threads, int mode, size_t fact_lds, size_t ref_lds, hipStream_t st)
 {
     const MatIO *io = static_cast<const MatIO *>(ios);
-    if (mode == kSweepLds)
-        k_batch_group<kSweepLds><<<count, threads, lds_bytes, st>>>(descs, io, order, tol, maxit, perturb);
-    else if (mode == kSweepFlags)
-        k_batch_group<kSweepFlags><<<count, threads, lds_bytes, st>>>(descs, io, order, tol, maxit, perturb);
+    if (mode == kSweepLevels)
+        k_batch_group_factor<false><<<count, threads, fact_lds, st>>>(descs, io, order, perturb);
     else
-        k_batch_group<kSweepLevels><<<count, threads, lds_bytes, st>>>(descs, io, order, tol, maxit, perturb);
+        k_batch_group_factor<true><<<count, threads, fact_lds, st>>>(descs, io, order, perturb);
+    if (mode == kSweepLds)
+        k_batch_group_refine<kSweepLds><<<count, threads, ref_lds, st>>>(descs, io, order, tol, maxit);
+    else if (mode == kSweepFlags)
+        k_batch_group_refine<kSweepFlags><<<count, threads, ref_lds, st>>>(descs, io, order, tol, maxit);
+    else
+        k_batch_group_refine<kSweepLevels><<<count, threads, ref_lds, st>>>(descs, io, order, tol, maxit);
 }
 
 void factor_tail(const FactorDev &F, float *val, const float *d_pert, int perturb, int threads, hipStream_t st)
@@ -1527,12 +1635,16 @@ size_t enable_lds()
     if (hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return 0;
     const size_t avail = (size_t)lim > 8192 ? (size_t)lim - 8192 : 0;  // static LDS of the kernels (< 8 KiB)
     const void *fns[] = {
-        reinterpret_cast<const void *>(&k_batch_same<kSweepLds>),
-        reinterpret_cast<const void *>(&k_batch_same<kSweepFlags>),
-        reinterpret_cast<const void *>(&k_batch_same<kSweepLevels>),
-        reinterpret_cast<const void *>(&k_batch_group<kSweepLds>),
-        reinterpret_cast<const void *>(&k_batch_group<kSweepFlags>),
-        reinterpret_cast<const void *>(&k_batch_group<kSweepLevels>),
+        reinterpret_cast<const void *>(&k_batch_same_factor<true>),
+        reinterpret_cast<const void *>(&k_batch_same_factor<false>),
+        reinterpret_cast<const void *>(&k_batch_same_refine<kSweepLds>),
+        reinterpret_cast<const void *>(&k_batch_same_refine<kSweepFlags>),
+        reinterpret_cast<const void *>(&k_batch_same_refine<kSweepLevels>),
+        reinterpret_cast<const void *>(&k_batch_group_factor<true>),
+        reinterpret_cast<const void *>(&k_batch_group_factor<false>),
+        reinterpret_cast<const void *>(&k_batch_group_refine<kSweepLds>),
+        reinterpret_cast<const void *>(&k_batch_group_refine<kSweepFlags>),
+        reinterpret_cast<const void *>(&k_batch_group_refine<kSweepLevels>),
         reinterpret_cast<const void *>(&k_tri_df),
         reinterpret_cast<const void *>(&k_factor_tail),
     };

@@ -43,10 +43,10 @@ void residual(const RefineDev &R, const double *ax, const double *b, const doubl
               int transpose, int with_anorm, unsigned long long *norms, uint32_t lanes, hipStream_t st);
 size_t batch_lds_bytes(int mode, uint32_t n);
 void batch_same(const MatDesc *desc, int batch, const double *ax, const double *b, double *x, float *lu,
-                float *lucsr, float *t32, double *r64, int32_t *iters, double *eta, double tol, int maxit,
-                int perturb, int threads, int mode, size_t lds_bytes, hipStream_t st);
+                float *lucsr, float *t32, double *r64, double *norms, int32_t *iters, double *eta, double tol,
+                int maxit, int perturb, int threads, int mode, size_t fact_lds, size_t ref_lds, hipStream_t st);
 void batch_group(const MatDesc *descs, const void *ios, const uint32_t *order, int count, double tol,
-                 int maxit, int perturb, int threads, int mode, size_t lds_bytes, hipStream_t st);
+                 int maxit, int perturb, int threads, int mode, size_t fact_lds, size_t ref_lds, hipStream_t st);
 void tri_df(const TriDev &T, const float *lu, float *x, int l0, int l1, uint32_t k0, uint32_t k1, uint32_t n,
             int threads, hipStream_t st);
 size_t enable_lds();
@@ -282,6 +282,7 @@ struct glu_plan {
     float *d_bt32 = nullptr;
     double *d_br64 = nullptr;
     float *d_blucsr = nullptr;   // sweep value copies, df_nval per matrix
+    double *d_bnorms = nullptr;  // |A|_inf, |b|_inf per matrix
     int batch_cap = 0;
 
     ~glu_plan()
@@ -291,6 +292,7 @@ struct glu_plan {
         if (d_bt32) (void)hipFree(d_bt32);
         if (d_br64) (void)hipFree(d_br64);
         if (d_blucsr) (void)hipFree(d_blucsr);
+        if (d_bnorms) (void)hipFree(d_bnorms);
         for (GraphSlot *g : {&g_factor, &g_solve[0], &g_solve[1]})
             if (g->exec) (void)hipGraphExecDestroy(g->exec);
         if (cap_stream) (void)hipStreamDestroy(cap_stream);
@@ -407,10 +409,8 @@ bool validate_pattern(const glu_pattern &P, std::string &err)
 struct Tables {
     std::vector<uint32_t> upos;        // CSR position -> CSC position
     // column tasks, by level: task t has lanes 64 t .. 64 t + 63 in cln
-    // ({L position, pivot position, U count, 0}); lane's e-th update reads
-    // U value cup[ctask[t].x + 64 e + lane] and targets ctgt[same index]
-    // for e < ctask[t].y
-    std::vector<uint2> ctask;
+    // ({L position, pivot position, update count <= 4, 0}); the lane's e-th
+    // update reads U value cup[256 t + 64 e + lane] and targets ctgt[same]
     std::vector<uint4> cln;
     std::vector<uint32_t> ctgt, cup;
     std::vector<uint32_t> ctask_ptr;   // [num_lev + 2]
@@ -492,9 +492,8 @@ bool build_factor_tables(const glu_pattern &P, Tables &T, int64_t &updates, std:
     T.dvd_ptr.assign(nl + 1, 0);
     T.msrc_ptr.assign(1, 0);
     std::vector<uint32_t> stamp(nnz, UINT32_MAX), cnt(nnz, 0), slot(nnz, 0), touched;
-    std::vector<std::pair<uint32_t, uint32_t>> cols;   // (|U_k|, k) of the level
     for (int l = 0; l < nl; ++l) {
-        T.ctask_ptr[l] = (uint32_t)T.ctask.size();
+        T.ctask_ptr[l] = (uint32_t)(T.cln.size() / 64);
         T.ment_ptr[l] = (uint32_t)T.ment.size();
         T.dvd_ptr[l] = (uint32_t)T.dvd.size();
         auto each_update = [&](auto f) {
@@ -537,46 +536,36 @@ bool build_factor_tables(const glu_pattern &P, Tables &T, int64_t &updates, std:
         each_update([&](uint32_t tgt, uint32_t lpos, uint32_t up, uint32_t dpos) {
             if (cnt[tgt] >= 2) T.msrc[fillpos[slot[tgt] - first]++] = MergeSrc{lpos, up, dpos, 0};
         });
-        // Column tasks: the level's columns with updates, longest U list
-        // first, cut into pieces of <= 64 L entries and packed into tasks of
-        // 64 lanes; merge targets are left to their merge entries (~0u).
-        cols.clear();
+        // Column tasks: a lane takes one L entry A(i,k) of a pivot column
+        // and up to four consecutive entries of its U list; lanes sharing a
+        // U chunk are adjacent (they read the same U values), and tasks are
+        // 64 lanes.  Merge targets are left to their merge entries (~0u).
+        uint32_t used = 64;   // lanes used in the open task
         for (int t = P.level_ptr[l]; t < P.level_ptr[l + 1]; ++t) {
             const uint32_t k = (uint32_t)P.level_idx[t];
-            const uint32_t L = P.sym_c_ptr[k + 1] - P.l_col_ptr[k] - 1;
-            const uint32_t U = P.csr_r_ptr[k + 1] - P.csr_diag_ptr[k] - 1;
-            if (L && U) cols.push_back({U, k});
-        }
-        std::stable_sort(cols.begin(), cols.end(), [](auto a, auto b) { return a.first > b.first; });
-        uint32_t used = 64;   // lanes used in the open task
-        for (const auto &cu : cols) {
-            const uint32_t k = cu.second, U = cu.first;
             const uint32_t dpos = P.l_col_ptr[k];
             const uint32_t L = P.sym_c_ptr[k + 1] - dpos - 1;
+            const uint32_t U = P.csr_r_ptr[k + 1] - P.csr_diag_ptr[k] - 1;
+            if (!L || !U) continue;
             const uint32_t uf = P.csr_diag_ptr[k] + 1;
-            for (uint32_t i0 = 0; i0 < L; i0 += 64) {
-                const uint32_t pc = std::min<uint32_t>(64, L - i0);
-                if (used + pc > 64) {   // open a new task, its lanes idle until filled
-                    T.ctask.push_back(make_uint2((uint32_t)T.ctgt.size(), 0u));
-                    T.cln.resize(T.cln.size() + 64, make_uint4(0u, 0u, 0u, 0u));
-                    used = 0;
-                }
-                uint2 &th = T.ctask.back();
-                if (U > th.y) {   // widen the task's block (tasks fill longest U first)
-                    T.ctgt.resize(th.x + 64 * (size_t)U, UINT32_MAX);
-                    T.cup.resize(th.x + 64 * (size_t)U, 0u);
-                    th.y = U;
-                }
-                const size_t lane0 = T.cln.size() - 64 + used;
-                for (uint32_t ii = 0; ii < pc; ++ii) {
-                    T.cln[lane0 + ii] = make_uint4(dpos + 1 + i0 + ii, dpos, U, 0u);
-                    for (uint32_t jj = 0; jj < U; ++jj) {
-                        const uint32_t tgt = umap[moff[k] + (uint64_t)jj * L + i0 + ii];
-                        T.ctgt[th.x + 64 * (size_t)jj + used + ii] = cnt[tgt] >= 2 ? UINT32_MAX : tgt;
-                        T.cup[th.x + 64 * (size_t)jj + used + ii] = T.upos[uf + jj];
+            for (uint32_t c0 = 0; c0 < U; c0 += 4) {
+                const uint32_t nu = std::min<uint32_t>(4, U - c0);
+                for (uint32_t ii = 0; ii < L; ++ii) {
+                    if (used == 64) {
+                        T.cln.resize(T.cln.size() + 64, make_uint4(0u, 0u, 0u, 0u));
+                        T.ctgt.resize(T.ctgt.size() + 256, UINT32_MAX);
+                        T.cup.resize(T.cup.size() + 256, 0u);
+                        used = 0;
                     }
+                    const size_t lane = T.cln.size() - 64 + used, slot0 = T.ctgt.size() - 256 + used;
+                    T.cln[lane] = make_uint4(dpos + 1 + ii, dpos, nu, 0u);
+                    for (uint32_t e = 0; e < nu; ++e) {
+                        const uint32_t tgt = umap[moff[k] + (uint64_t)(c0 + e) * L + ii];
+                        T.ctgt[slot0 + 64 * e] = cnt[tgt] >= 2 ? UINT32_MAX : tgt;
+                        T.cup[slot0 + 64 * e] = T.upos[uf + c0 + e];
+                    }
+                    ++used;
                 }
-                used += pc;
             }
         }
         for (int t = P.level_ptr[l]; t < P.level_ptr[l + 1]; ++t) {
@@ -586,7 +575,7 @@ bool build_factor_tables(const glu_pattern &P, Tables &T, int64_t &updates, std:
                 T.dvd.push_back(make_uint2(p, dpos | (p == dpos + 1 ? kFirstBit : 0u)));
         }
     }
-    T.ctask_ptr[nl] = T.ctask_ptr[nl + 1] = (uint32_t)T.ctask.size();
+    T.ctask_ptr[nl] = T.ctask_ptr[nl + 1] = (uint32_t)(T.cln.size() / 64);
     T.ment_ptr[nl] = T.ment_ptr[nl + 1] = (uint32_t)T.ment.size();
     T.dvd_ptr[nl] = (uint32_t)T.dvd.size();
     if (T.ctgt.size() >= UINT32_MAX || T.msrc.size() >= UINT32_MAX) {
@@ -708,7 +697,7 @@ void choose_tail(const Tables &T, int num_lev, uint32_t nnz, uint32_t budget, Ta
                 if (ln.z == 0) continue;
                 touch(ln.x), touch(ln.y);
                 for (uint32_t e = 0; e < ln.z; ++e) {
-                    const size_t q = T.ctask[t].x + 64 * (size_t)e + lane;
+                    const size_t q = 256 * (size_t)t + 64 * e + lane;
                     touch(T.cup[q]);
                     if (T.ctgt[q] != UINT32_MAX) touch(T.ctgt[q]);
                 }
@@ -741,7 +730,7 @@ void choose_tail(const Tables &T, int num_lev, uint32_t nnz, uint32_t budget, Ta
     const int L = t.lev;
     t.c0 = T.ctask_ptr[L];
     const uint32_t c1 = T.ctask_ptr[num_lev + 1];
-    t.g0 = t.c0 < c1 ? T.ctask[t.c0].x : 0;
+    t.g0 = 256 * t.c0;
     // padding entries (no update) keep their values: ~0u targets, 0 U positions
     for (uint32_t q = t.g0; q < T.ctgt.size(); ++q) {
         const bool live = T.ctgt[q] != UINT32_MAX || slot[T.cup[q]] != UINT32_MAX;
@@ -816,44 +805,81 @@ struct DfHost {
     std::vector<uint32_t> pvpos;       // per position: pivot CSC position (div)
 };
 
+// Lanes per row of a wave row: 16, 32 or 64 (longer rows take several
+// passes of 64).
+inline uint32_t df_group(uint32_t len) { return len <= 16 ? 16u : len <= 32 ? 32u : 64u; }
+
 template <typename Entries, typename Pivot>
 void build_df(const TriHost &H, Entries entries, bool div, Pivot pivot, DfHost &D)
 {
     const uint32_t n = (uint32_t)H.rows.size();
+    const int nlev = (int)H.lvl_ptr.size() - 1;
+    std::vector<uint32_t> len(n);
+    for (uint32_t r = 0; r < n; ++r) {
+        std::vector<uint2> tmp;
+        entries(r, tmp);
+        len[r] = (uint32_t)tmp.size();
+    }
+    // Within each level (whose rows are independent) the rows taken a lane
+    // each are kept together, after the wave rows on even levels and before
+    // them on odd ones, so that lane runs of adjacent levels meet and pack
+    // into full tasks; wave rows are sorted by length so that rows sharing
+    // a wave (16 or 32 lanes each) have the same lane count.
+    std::vector<uint32_t> order(n);
+    for (int l = 0; l < nlev; ++l) {
+        std::vector<uint32_t> a, b;
+        for (uint32_t q = H.lvl_ptr[l]; q < H.lvl_ptr[l + 1]; ++q) (len[H.rows[q]] <= kDfK ? a : b).push_back(H.rows[q]);
+        std::stable_sort(b.begin(), b.end(), [&](uint32_t x, uint32_t y) { return len[x] < len[y]; });
+        if (!(l & 1)) std::swap(a, b);
+        std::copy(a.begin(), a.end(), order.begin() + H.lvl_ptr[l]);
+        std::copy(b.begin(), b.end(), order.begin() + H.lvl_ptr[l] + a.size());
+    }
+    std::vector<int> lev_of(n);
+    for (int l = 0; l < nlev; ++l)
+        for (uint32_t q = H.lvl_ptr[l]; q < H.lvl_ptr[l + 1]; ++q) lev_of[q] = l;
     std::vector<std::vector<uint2>> ent(n);
     D.rl.resize(n);
     for (uint32_t p = 0; p < n; ++p) {
-        const uint32_t r = H.rows[p];
-        entries(r, ent[p]);
-        D.rl[p] = make_uint2(r, (uint32_t)ent[p].size());
+        entries(order[p], ent[p]);
+        D.rl[p] = make_uint2(order[p], (uint32_t)ent[p].size());
     }
     auto put = [&](const uint2 &e) {
         D.idx.push_back(e.x);
         D.vpos.push_back(e.y);
     };
+    const uint2 pad = make_uint2(0u, UINT32_MAX);
     uint32_t p = 0;
     while (p < n) {
         const uint32_t base = (uint32_t)D.idx.size();
+        const uint32_t p0 = p;
         if (ent[p].size() > kDfK) {
-            for (const uint2 &e : ent[p]) put(e);
-            D.task.push_back(make_uint4(base, p, kLongTask | 1u, (uint32_t)ent[p].size()));
-            ++p;
+            // rows of one lane count and one level (no dependencies inside)
+            const uint32_t gw = df_group((uint32_t)ent[p].size()), per = 64 / gw;
+            uint32_t m = 0;
+            while (p < n && p - p0 < per && lev_of[p] == lev_of[p0] && ent[p].size() > kDfK &&
+                   df_group((uint32_t)ent[p].size()) == gw)
+                m = std::max(m, (uint32_t)ent[p++].size());
+            if (gw == 64) {   // padded to whole passes of 64 (every lane loads its slot)
+                for (const uint2 &e : ent[p0]) put(e);
+                while ((D.idx.size() - base) % 64) put(pad);
+            } else {
+                for (uint32_t g = 0; g < per; ++g)
+                    for (uint32_t j = 0; j < gw; ++j)
+                        put(p0 + g < p && j < ent[p0 + g].size() ? ent[p0 + g][j] : pad);
+            }
+            const uint32_t lg = gw == 16 ? 4u : gw == 32 ? 5u : 6u;
+            D.task.push_back(make_uint4(base, p0, kLongTask | (lg << 24) | (p - p0), m));
             continue;
         }
-        const uint32_t p0 = p;
         uint32_t m = 0;
         while (p < n && p - p0 < 64 && ent[p].size() <= kDfK) m = std::max(m, (uint32_t)ent[p++].size());
         for (uint32_t e = 0; e < m; ++e)
-            for (uint32_t l = 0; l < 64; ++l) {
-                const uint32_t q = p0 + l;
-                if (q < p && e < ent[q].size()) put(ent[q][e]);
-                else put(make_uint2(0u, UINT32_MAX));
-            }
+            for (uint32_t l = 0; l < 64; ++l) put(p0 + l < p && e < ent[p0 + l].size() ? ent[p0 + l][e] : pad);
         D.task.push_back(make_uint4(base, p0, p - p0, m));
     }
     if (div) {
         D.pvpos.resize(n);
-        for (uint32_t q = 0; q < n; ++q) D.pvpos[q] = pivot(H.rows[q]);
+        for (uint32_t q = 0; q < n; ++q) D.pvpos[q] = pivot(order[q]);
     }
 }
 
@@ -918,7 +944,7 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
     p->tail_lev = tail.lev;
     p->tail_nslot = (uint32_t)tail.pos.size();
     factor_segments(*p, T);
-    p->units = (int64_t)T.ctask.size();
+    p->units = (int64_t)(T.cln.size() / 64);
 
     const uint32_t n = P.n;
     // Solve schedules.
@@ -1021,7 +1047,6 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
     const size_t o_ccidx = U.put(P.csr_c_idx, P.nnz);
     const size_t o_cdiag = U.put(P.csr_diag_ptr, n);
     const size_t o_upos = U.put(T.upos);
-    const size_t o_ctask = U.put(T.ctask);
     const size_t o_cptr = U.put(T.ctask_ptr);
     const size_t o_cln = U.put(T.cln);
     const size_t o_ctgt = U.put(T.ctgt);
@@ -1091,7 +1116,6 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
     auto u32p = [&](size_t o) { return reinterpret_cast<const uint32_t *>(A + o); };
     FactorDev &F = p->F;
     F.col_ptr = u32p(o_colptr);
-    F.ctask = reinterpret_cast<const uint2 *>(A + o_ctask);
     F.ctask_ptr = u32p(o_cptr);
     F.cln = reinterpret_cast<const uint4 *>(A + o_cln);
     F.ctgt = u32p(o_ctgt);
@@ -1279,13 +1303,16 @@ glu_status ensure_batch(glu_plan *p, int batch)
     if (p->d_bt32) (void)hipFree(p->d_bt32);
     if (p->d_br64) (void)hipFree(p->d_br64);
     if (p->d_blucsr) (void)hipFree(p->d_blucsr);
+    if (p->d_bnorms) (void)hipFree(p->d_bnorms);
     p->d_bt32 = nullptr;
     p->d_br64 = nullptr;
     p->d_blucsr = nullptr;
+    p->d_bnorms = nullptr;
     p->batch_cap = 0;
     HIP_TRY(hipMalloc(&p->d_bt32, sizeof(float) * (size_t)p->n * batch));
     HIP_TRY(hipMalloc(&p->d_br64, sizeof(double) * (size_t)p->n * batch));
     HIP_TRY(hipMalloc(&p->d_blucsr, sizeof(float) * (size_t)p->df_nval * batch));
+    HIP_TRY(hipMalloc(&p->d_bnorms, 2 * sizeof(double) * (size_t)batch));
     p->batch_cap = batch;
     return GLU_OK;
 }
@@ -1503,9 +1530,10 @@ extern "C" glu_status glu_batch_solve(glu_plan *p, int batch, const double *d_ax
     if (s != GLU_OK) return s;
     const size_t cap = launch::enable_lds();
     const int mode = sweep_mode(p->n, cap);
-    const size_t lds = std::max(launch::batch_lds_bytes(mode, p->n), sizeof(float) * p->tail_nslot);
-    launch::batch_same(p->d_desc, batch, d_ax, d_b, d_x, d_lu, p->d_blucsr, p->d_bt32, p->d_br64, d_iters, d_eta,
-                       tol, maxit, (int)(flags & 1u), p->fused_threads, mode, lds, static_cast<hipStream_t>(stream));
+    launch::batch_same(p->d_desc, batch, d_ax, d_b, d_x, d_lu, p->d_blucsr, p->d_bt32, p->d_br64, p->d_bnorms,
+                       d_iters, d_eta, tol, maxit, (int)(flags & 1u), p->fused_threads, mode,
+                       sizeof(float) * p->tail_nslot, launch::batch_lds_bytes(mode, p->n),
+                       static_cast<hipStream_t>(stream));
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(GLU_ERR_DEVICE, std::string("batch launch: ") + hipGetErrorString(e));
     return GLU_OK;
@@ -1523,6 +1551,7 @@ struct MatIOHost {            // mirrors the device-side MatIO layout
     float *t32;
     double *r64;
     float *lucsr;
+    double *norms;
     int32_t *iters;
     double *eta;
 };
@@ -1535,12 +1564,14 @@ struct glu_group {
     float *d_t32 = nullptr;
     double *d_r64 = nullptr;
     float *d_lucsr = nullptr;
+    double *d_norms = nullptr;
     std::vector<size_t> voff;   // per-matrix offset into the scratch vectors
     std::vector<size_t> foff;   // per-matrix offset into d_lucsr
     int threads = kDefaultFusedThreads;
     ~glu_group()
     {
         if (d_lucsr) (void)hipFree(d_lucsr);
+        if (d_norms) (void)hipFree(d_norms);
         if (d_descs) (void)hipFree(d_descs);
         if (d_ios) (void)hipFree(d_ios);
         if (d_order) (void)hipFree(d_order);
@@ -1585,6 +1616,7 @@ extern "C" glu_status glu_group_create(glu_plan *const *plans, int count, glu_gr
     HIP_TRY(hipMalloc(&g->d_t32, sizeof(float) * vtot));
     HIP_TRY(hipMalloc(&g->d_r64, sizeof(double) * vtot));
     HIP_TRY(hipMalloc(&g->d_lucsr, sizeof(float) * ftot));
+    HIP_TRY(hipMalloc(&g->d_norms, 2 * sizeof(double) * count));
     HIP_TRY(hipMemcpy(g->d_descs, descs.data(), sizeof(MatDesc) * count, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(g->d_order, order.data(), sizeof(uint32_t) * count, hipMemcpyHostToDevice));
     *out = g.release();
@@ -1604,7 +1636,7 @@ extern "C" glu_status glu_group_solve(glu_group *g, const double *const *d_ax, c
     for (int i = 0; i < count; ++i) {
         if (!d_ax[i] || !d_b[i] || !d_x[i] || !d_lu[i]) return fail(GLU_ERR_ARGUMENT, "null matrix pointer");
         ios[i] = MatIOHost{d_ax[i], d_b[i], d_x[i], d_lu[i], g->d_t32 + g->voff[i], g->d_r64 + g->voff[i],
-                           g->d_lucsr + g->foff[i], d_iters + i, d_eta + i};
+                           g->d_lucsr + g->foff[i], g->d_norms + 2 * i, d_iters + i, d_eta + i};
     }
     hipStream_t st = static_cast<hipStream_t>(stream);
     HIP_TRY(hipMemcpyAsync(g->d_ios, ios.data(), sizeof(MatIOHost) * count, hipMemcpyHostToDevice, st));
@@ -1615,9 +1647,8 @@ extern "C" glu_status glu_group_solve(glu_group *g, const double *const *d_ax, c
     }
     const size_t cap = launch::enable_lds();
     const int mode = sweep_mode((uint32_t)nmax, cap);
-    const size_t lds = std::max(launch::batch_lds_bytes(mode, (uint32_t)nmax), tmax);
-    launch::batch_group(g->d_descs, g->d_ios, g->d_order, count, tol, maxit, (int)(flags & 1u), g->threads, mode, lds,
-                        st);
+    launch::batch_group(g->d_descs, g->d_ios, g->d_order, count, tol, maxit, (int)(flags & 1u), g->threads, mode, tmax,
+                        launch::batch_lds_bytes(mode, (uint32_t)nmax), st);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(GLU_ERR_DEVICE, std::string("group launch: ") + hipGetErrorString(e));
     // ios is read by the kernel from device memory; the host copy may go.
