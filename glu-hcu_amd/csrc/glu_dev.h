@@ -42,13 +42,31 @@ constexpr uint32_t kLongMerge = 32;
 // stores the GESP pivot.
 constexpr uint32_t kFirstBit = 0x80000000u;
 
-// Per-level lane items of the right-looking factorization (one lane each):
-//   update  A(i,j) += -(A(i,k) / A(k,k) * A(k,j))   {tgt, lpos, upos, dpos}
-//           for entries exactly one column of the level updates;
-//   merge   all updates of an entry several columns of the level update,
-//           in increasing column order;
-//   divide  A(i,k) /= A(k,k) for the columns of the previous level
-//           {lpos, dpos | kFirstBit} (deferred so a level needs one barrier).
+// Per-level items of the right-looking factorization:
+//   column task  (a wave) A(i,j) += -(A(i,k) / A(k,k) * A(k,j)) for targets
+//                exactly one column of the level updates;
+//   merge        all updates of an entry several columns of the level
+//                update, in increasing column order (a lane, or a wave);
+//   divide       A(i,k) /= A(k,k) for the columns of the previous level
+//                {lpos, dpos | kFirstBit} (deferred so a level needs one
+//                barrier).
+
+// One LDS window: steps [l0, l1) with the nslot values they touch staged in
+// LDS.  Its items (cln / ctgt / cup / ment / msrc / dvd, indexed like the
+// factor's tables) address LDS slots instead of CSC positions; pos[slot] is
+// the slot's CSC position.
+struct WinDev {
+    int32_t l0, l1;
+    uint32_t nslot, pad;
+    const uint32_t *pos;
+    const uint4 *cln;
+    const uint32_t *ctgt;
+    const uint32_t *cup;
+    const uint32_t *ment;
+    const MergeSrc *msrc;
+    const uint2 *dvd;
+};
+
 struct FactorDev {
     const uint32_t *col_ptr;   // sym_c_ptr [n+1] (GESP column sums)
     // Column tasks, by level (tasks ctask_ptr[l] .. ctask_ptr[l+1]).  Lane
@@ -71,20 +89,10 @@ struct FactorDev {
     const uint32_t *dvd_ptr;   // [num_lev + 1]
     int32_t num_lev;
     uint32_t n, nnz;
-    // LDS-resident tail: steps [tail_lev, num_lev] work on tail_nslot values
-    // staged in LDS.  Their items, in t_cln / t_ctgt / t_cup / t_ment /
-    // t_msrc / t_dvd (same indexing as cln / ctgt / cup / ment / msrc /
-    // dvd), address LDS slots instead of CSC positions; tail_pos[slot] is
-    // the slot's CSC position.
-    int32_t tail_lev;          // num_lev + 1 when there is no tail
-    uint32_t tail_nslot;
-    const uint32_t *tail_pos;
-    const uint4 *t_cln;
-    const uint32_t *t_ctgt;
-    const uint32_t *t_cup;
-    const uint32_t *t_ment;
-    const MergeSrc *t_msrc;
-    const uint2 *t_dvd;
+    // LDS windows: runs of steps that work on values staged in LDS
+    const WinDev *win;         // [nwin], in step order
+    int32_t nwin;
+    uint32_t win_nslot;        // the largest window's values
 };
 
 // One level-scheduled triangular sweep, row-pull form:

@@ -31,6 +31,19 @@
 
 namespace glu {
 
+// Pointers read from device memory (the matrix descriptors) are generic to
+// the compiler, which then emits flat loads: these count against the LDS
+// wait counter too, so every LDS wait would also wait for global loads in
+// flight.  The tables and buffers are all in global memory; rebuilding each
+// pointer as a global-address-space pointer (through its integer value, a
+// cast the optimizer keeps, unlike a generic->global->generic round trip)
+// lets address-space inference turn its accesses into global loads.
+template <typename T>
+__device__ __forceinline__ T *glb(T *p)
+{
+    return (T *)(__attribute__((address_space(1))) T *)(uintptr_t)p;
+}
+
 constexpr int kWave = 64;
 constexpr float kPertFactor = 3.45e-4f;   // numeric.cu:503
 
@@ -170,6 +183,43 @@ __device__ __forceinline__ void factor_merge(const FactorDev &F, float *__restri
     val[t] = v;
 }
 
+// acc - p[0] - p[1] - ... - p[cnt-1] (desc: from p[cnt-1] down to p[0]), each
+// subtraction rounded, for a wave's 64-float LDS slice p (256-byte
+// aligned).  A full slice is read as sixteen 16-byte loads issued together,
+// so only the subtraction chain is sequential.
+__device__ __forceinline__ float serial_sub(float acc, const float *p, uint32_t cnt, bool desc)
+{
+    if (cnt == kWave) {
+        const float4 *p4 = reinterpret_cast<const float4 *>(p);
+        float4 v[kWave / 4];
+#pragma unroll
+        for (uint32_t k = 0; k < kWave / 4; ++k) v[k] = p4[k];
+        if (!desc) {
+#pragma unroll
+            for (uint32_t k = 0; k < kWave / 4; ++k) {
+                acc = acc - v[k].x;
+                acc = acc - v[k].y;
+                acc = acc - v[k].z;
+                acc = acc - v[k].w;
+            }
+        } else {
+#pragma unroll
+            for (uint32_t k = kWave / 4; k > 0; --k) {
+                acc = acc - v[k - 1].w;
+                acc = acc - v[k - 1].z;
+                acc = acc - v[k - 1].y;
+                acc = acc - v[k - 1].x;
+            }
+        }
+        return acc;
+    }
+    if (!desc)
+        for (uint32_t i = 0; i < cnt; ++i) acc = acc - p[i];
+    else
+        for (uint32_t i = cnt; i > 0; --i) acc = acc - p[i - 1];
+    return acc;
+}
+
 // The same for an entry with many sources, by one wave: the products of 64
 // sources at a time in parallel, then lane 0 adds them in source order from
 // the wave's LDS slice (64 floats) — the same arithmetic, sequence included.
@@ -191,7 +241,7 @@ __device__ __forceinline__ void factor_merge_wave(const FactorDev &F, float *__r
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         if (lane == 0) {
             const uint32_t cnt = min((uint32_t)kWave, s1 - base);
-            for (uint32_t i = 0; i < cnt; ++i) v = v + (-lds[i]);
+            v = serial_sub(v, lds, cnt, false);   // v + (-p) is v - p exactly
         }
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -316,7 +366,7 @@ __global__ void k_zero_u64(unsigned long long *__restrict__ p, int count)
 __global__ void __launch_bounds__(256) k_factor_grid(FactorDev F, float *__restrict__ val,
                                                      const float *__restrict__ d_pert, int lvl, int perturb)
 {
-    __shared__ float lds[256 / kWave][kWave];
+    __shared__ __attribute__((aligned(16))) float lds[256 / kWave][kWave];
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     const float pert = perturb ? *d_pert : 0.f;
     factor_level<__HIP_MEMORY_SCOPE_AGENT>(F, val, level_span(F, lvl), g, gridDim.x * blockDim.x, pert, perturb,
@@ -327,48 +377,77 @@ __global__ void __launch_bounds__(256) k_factor_grid(FactorDev F, float *__restr
 __global__ void __launch_bounds__(1024) k_factor_fused(FactorDev F, float *__restrict__ val,
                                                        const float *__restrict__ d_pert, int l0, int l1, int perturb)
 {
-    __shared__ float lds[1024 / kWave][kWave];
+    __shared__ __attribute__((aligned(16))) float lds[1024 / kWave][kWave];
     const float pert = perturb ? *d_pert : 0.f;
     factor_levels_wg<__HIP_MEMORY_SCOPE_AGENT>(F, val, l0, l1, pert, perturb, lds[threadIdx.x / kWave]);
 }
 
-// The tail's view of the tables: the same level spans, items that address
+// A window's view of the tables: the same level spans, items that address
 // LDS slots.
-__device__ __forceinline__ FactorDev tail_view(const FactorDev &F)
+__device__ __forceinline__ FactorDev window_view(const FactorDev &F, const WinDev &W)
 {
     FactorDev T = F;
-    T.cln = F.t_cln;
-    T.ctgt = F.t_ctgt;
-    T.cup = F.t_cup;
-    T.ment = F.t_ment;
-    T.msrc = F.t_msrc;
-    T.dvd = F.t_dvd;
+    T.cln = W.cln;
+    T.ctgt = W.ctgt;
+    T.cup = W.cup;
+    T.ment = W.ment;
+    T.msrc = W.msrc;
+    T.dvd = W.dvd;
     return T;
 }
 
-// Steps [tail_lev, num_lev] inside one workgroup with the values they touch
-// staged in LDS (slots): load, one barrier per level, store back.
-__device__ __forceinline__ void factor_tail(const FactorDev &F, float *__restrict__ val, float *slots, float pert,
-                                            int perturb, float *wlds)
+__device__ __forceinline__ WinDev load_window(const FactorDev &F, int w)
+{
+    WinDev W = F.win[w];
+    W.pos = glb(W.pos);
+    W.cln = glb(W.cln);
+    W.ctgt = glb(W.ctgt);
+    W.cup = glb(W.cup);
+    W.ment = glb(W.ment);
+    W.msrc = glb(W.msrc);
+    W.dvd = glb(W.dvd);
+    return W;
+}
+
+// Steps [W.l0, W.l1) inside one workgroup with the values they touch staged
+// in LDS (slots): load, one barrier per step, store back.
+__device__ __forceinline__ void factor_window(const FactorDev &F, const WinDev &W, float *__restrict__ val,
+                                              float *slots, float pert, int perturb, float *wlds)
 {
     const uint32_t tid = threadIdx.x, nt = blockDim.x;
-    for (uint32_t s = tid; s < F.tail_nslot; s += nt) slots[s] = val[F.tail_pos[s]];
+    for (uint32_t s = tid; s < W.nslot; s += nt) slots[s] = val[W.pos[s]];
     __syncthreads();
-    const FactorDev T = tail_view(F);
-    factor_levels_wg<__HIP_MEMORY_SCOPE_WORKGROUP>(T, slots, F.tail_lev, F.num_lev + 1, pert, perturb, wlds);
-    for (uint32_t s = tid; s < F.tail_nslot; s += nt) val[F.tail_pos[s]] = slots[s];
+    const FactorDev T = window_view(F, W);
+    factor_levels_wg<__HIP_MEMORY_SCOPE_WORKGROUP>(T, slots, W.l0, W.l1, pert, perturb, wlds);
+    for (uint32_t s = tid; s < W.nslot; s += nt) val[W.pos[s]] = slots[s];
     __syncthreads();
+}
+
+// All steps of one factorization by one workgroup: windows in LDS, the
+// steps between them in global memory.
+__device__ __forceinline__ void factor_all_wg(const FactorDev &F, float *__restrict__ val, float *slots, float pert,
+                                              int perturb, float *wlds)
+{
+    int l = 0;
+    for (int w = 0; w < F.nwin; ++w) {
+        const WinDev W = load_window(F, w);
+        factor_levels_wg<__HIP_MEMORY_SCOPE_WORKGROUP>(F, val, l, W.l0, pert, perturb, wlds);
+        factor_window(F, W, val, slots, pert, perturb, wlds);
+        l = W.l1;
+    }
+    factor_levels_wg<__HIP_MEMORY_SCOPE_WORKGROUP>(F, val, l, F.num_lev + 1, pert, perturb, wlds);
 }
 
 extern __shared__ float g_lds_vec[];   // dynamic LDS of the kernels below
 
-// The tail of one factorization in one workgroup (dynamic LDS: 4 * tail_nslot).
-__global__ void __launch_bounds__(1024) k_factor_tail(FactorDev F, float *__restrict__ val,
-                                                      const float *__restrict__ d_pert, int perturb)
+// Window w of one factorization in one workgroup (dynamic LDS: its values).
+__global__ void __launch_bounds__(1024) k_factor_window(FactorDev F, float *__restrict__ val,
+                                                        const float *__restrict__ d_pert, int w, int perturb)
 {
-    __shared__ float lds[1024 / kWave][kWave];
+    __shared__ __attribute__((aligned(16))) float lds[1024 / kWave][kWave];
     const float pert = perturb ? *d_pert : 0.f;
-    factor_tail(F, val, g_lds_vec, pert, perturb, lds[threadIdx.x / kWave]);
+    const WinDev W = load_window(F, w);
+    factor_window(F, W, val, g_lds_vec, pert, perturb, lds[threadIdx.x / kWave]);
 }
 
 // ---------------------------------------------------------------------------
@@ -415,13 +494,7 @@ __device__ __forceinline__ void tri_row_wave(const TriDev &T, const float *__res
         lds[lane] = p;
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        if (lane == 0) {
-            const uint32_t cnt = hi - lo;
-            if (!T.desc)
-                for (uint32_t i = 0; i < cnt; ++i) acc = acc - lds[i];
-            else
-                for (uint32_t i = cnt; i > 0; --i) acc = acc - lds[i - 1];
-        }
+        if (lane == 0) acc = serial_sub(acc, lds, hi - lo, T.desc != 0);
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     }
@@ -436,7 +509,7 @@ __device__ __forceinline__ void tri_row_wave(const TriDev &T, const float *__res
 __global__ void __launch_bounds__(256) k_tri_grid(TriDev T, const float *__restrict__ lu, float *__restrict__ x,
                                                   int lvl)
 {
-    __shared__ float lds[256 / kWave][kWave];
+    __shared__ __attribute__((aligned(16))) float lds[256 / kWave][kWave];
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t r0 = T.lvl_ptr[lvl], nlong = T.lvl_long[lvl];
     const uint32_t w = g / kWave;
@@ -451,7 +524,7 @@ __global__ void __launch_bounds__(256) k_tri_grid(TriDev T, const float *__restr
 __global__ void __launch_bounds__(1024) k_tri_fused(TriDev T, const float *__restrict__ lu, float *__restrict__ x,
                                                     int l0, int l1)
 {
-    __shared__ float lds[1024 / kWave][kWave];
+    __shared__ __attribute__((aligned(16))) float lds[1024 / kWave][kWave];
     const uint32_t lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave, nw = blockDim.x / kWave;
     for (int lvl = l0; lvl < l1; ++lvl) {
         const uint32_t r0 = T.lvl_ptr[lvl], nlong = T.lvl_long[lvl];
@@ -606,13 +679,7 @@ __device__ __forceinline__ void df_long_task(const TriDev &T, const float *__res
         lds[lane] = p;
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        if (lane == 0) {
-            const uint32_t cnt = hi - lo;
-            if (!T.desc)
-                for (uint32_t i = 0; i < cnt; ++i) acc = acc - lds[i];
-            else
-                for (uint32_t i = cnt; i > 0; --i) acc = acc - lds[i - 1];
-        }
+        if (lane == 0) acc = serial_sub(acc, lds, hi - lo, T.desc != 0);
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     }
@@ -645,7 +712,7 @@ __device__ __forceinline__ void df_sweep(const TriDev &T, const float *__restric
 __global__ void __launch_bounds__(1024) k_tri_df(TriDev T, const float *__restrict__ lu, float *__restrict__ x,
                                                  int l0, int l1, uint32_t k0, uint32_t k1, uint32_t n)
 {
-    __shared__ float lds[1024 / kWave][kWave];
+    __shared__ __attribute__((aligned(16))) float lds[1024 / kWave][kWave];
     uint32_t *flags = reinterpret_cast<uint32_t *>(g_lds_vec);
     const uint32_t nwords = (n + 31) / 32;
     for (uint32_t i = threadIdx.x; i < nwords; i += blockDim.x) flags[i] = 0xffffffffu;
@@ -952,18 +1019,6 @@ __device__ __forceinline__ void wg_tri_refine(const TriDev &T, const float *__re
     }
 }
 
-// Pointers read from device memory (the matrix descriptors) are generic to
-// the compiler, which then emits flat loads: these count against the LDS
-// wait counter too, so every LDS wait would also wait for global loads in
-// flight.  The tables and buffers are all in global memory; rebuilding each
-// pointer as a global-address-space pointer (through its integer value, a
-// cast the optimizer keeps, unlike a generic->global->generic round trip)
-// lets address-space inference turn its accesses into global loads.
-template <typename T>
-__device__ __forceinline__ T *glb(T *p)
-{
-    return (T *)(__attribute__((address_space(1))) T *)(uintptr_t)p;
-}
 
 __device__ __forceinline__ void globalize(FactorDev &F)
 {
@@ -979,13 +1034,7 @@ __device__ __forceinline__ void globalize(FactorDev &F)
     F.msrc = glb(F.msrc);
     F.dvd = glb(F.dvd);
     F.dvd_ptr = glb(F.dvd_ptr);
-    F.tail_pos = glb(F.tail_pos);
-    F.t_cln = glb(F.t_cln);
-    F.t_ctgt = glb(F.t_ctgt);
-    F.t_cup = glb(F.t_cup);
-    F.t_ment = glb(F.t_ment);
-    F.t_msrc = glb(F.t_msrc);
-    F.t_dvd = glb(F.t_dvd);
+    F.win = glb(F.win);
 }
 
 __device__ __forceinline__ void globalize(TriDev &T)
@@ -1089,28 +1138,32 @@ __device__ __forceinline__ void df2_issue(const DfDev &D, const float *__restric
     t.lg = (d.z >> 24) & 0x7fu;
     t.m = d.w;
     t.dval = 1.f;
+    // All loads unconditional (the tables are padded past their last task
+    // and row), so they need no branches; what a lane does not own is
+    // masked after loading.  Entries past a row's length are never read.
     if (!t.wave) {
         const bool on = lane < t.cnt;
-        const uint2 rl = on ? D.rl[t.p0 + lane] : make_uint2(0u, 0u);
-        t.row = rl.x;
-        t.len = rl.y;
+        const uint2 rl = D.rl[t.p0 + lane];
+        t.row = on ? rl.x : 0u;
+        t.len = on ? rl.y : 0u;
 #pragma unroll
-        for (uint32_t e = 0; e < kDfK; ++e) {   // unused entries read row 0's ready bit
-            t.idx[e] = e < t.m ? D.idx[t.base + e * kWave + lane] : 0u;
-            t.val[e] = e < t.m ? sv[t.base + e * kWave + lane] : 0.f;
+        for (uint32_t e = 0; e < kDfK; ++e) {
+            t.idx[e] = D.idx[t.base + e * kWave + lane];
+            t.val[e] = sv[t.base + e * kWave + lane];
         }
-        if (D.div && on) t.dval = sv[D.nslots + t.p0 + lane];
+        const float dv = sv[D.nslots + t.p0 + lane];
+        if (D.div) t.dval = dv;
     } else {
-        // every lane's first slot exists (groups and 64-lane rows are padded),
-        // so the entry loads need not wait for the row lengths
+        // every lane's first slot exists (groups and 64-lane rows are padded)
         const uint32_t g = lane >> t.lg;
         const bool on = g < t.cnt;
-        const uint2 rl = on ? D.rl[t.p0 + g] : make_uint2(0u, 0u);
-        t.row = rl.x;
-        t.len = rl.y;
+        const uint2 rl = D.rl[t.p0 + g];
+        t.row = on ? rl.x : 0u;
+        t.len = on ? rl.y : 0u;
         t.idx[0] = D.idx[t.base + lane];
         t.val[0] = sv[t.base + lane];
-        if (D.div && on) t.dval = sv[D.nslots + t.p0 + g];
+        const float dv = sv[D.nslots + t.p0 + g];
+        if (D.div) t.dval = dv;
     }
 }
 
@@ -1193,38 +1246,36 @@ __device__ __forceinline__ void df2_run(const DfDev &D, const float *__restrict_
 }
 
 // All tasks of sweep D for the calling workgroup; sv: this sweep's values.
-// The caller has cleared every ready bit.
+// The caller has cleared every ready bit.  A wave runs tasks w, w + nw, ...
+// Descriptors are scalar loads (uniform per wave; they count against the
+// scalar/LDS wait counter, so waiting for one never waits for the vector
+// loads in flight), fetched two tasks ahead; a task's rows, columns and
+// values are vector loads issued one task ahead.
 __device__ __forceinline__ void df2_sweep(const DfDev &D, const float *__restrict__ sv, float *x, uint32_t *flags)
 {
-    const uint32_t lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave, nw = blockDim.x / kWave;
+    const uint32_t lane = threadIdx.x & (kWave - 1), nw = blockDim.x / kWave;
+    const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);   // uniform: scalar loads below
     const uint32_t mine = D.ntask > w ? (D.ntask - w + nw - 1) / nw : 0;   // tasks w, w + nw, ...
     if (mine == 0) return;
-    uint4 dl = make_uint4(0u, 0u, 0u, 0u);
-    auto descs = [&](uint32_t j0) {
-        if (j0 + lane < mine) dl = D.task[w + (j0 + lane) * nw];
-    };
+    // an empty task (nothing to load or run) past the wave's last one
+    // through the constant address space: the index is uniform, so this is
+    // a scalar load, which never waits for the vector loads in flight
+    const __attribute__((address_space(4))) uint32_t *tasks =
+        (const __attribute__((address_space(4))) uint32_t *)(uintptr_t)D.task;
     auto desc = [&](uint32_t j) {
-        const int l = (int)(j & (kWave - 1));
-        return make_uint4((uint32_t)__builtin_amdgcn_readlane((int)dl.x, l), (uint32_t)__builtin_amdgcn_readlane((int)dl.y, l),
-                          (uint32_t)__builtin_amdgcn_readlane((int)dl.z, l), (uint32_t)__builtin_amdgcn_readlane((int)dl.w, l));
+        const uint32_t q = 4 * (w + (j < mine ? j : mine - 1) * nw);
+        const uint4 d = make_uint4(tasks[q], tasks[q + 1], tasks[q + 2], tasks[q + 3]);
+        return j < mine ? d : make_uint4(0u, 0u, 0u, 0u);
     };
-    // two tasks ahead, in three rotating register sets (no copies, which
-    // would wait for loads still in flight)
-    descs(0);
-    DfTask ta, tb, tc;
-    df2_issue(D, sv, desc(0), lane, ta);
-    if (1 < mine) df2_issue(D, sv, desc(1), lane, tb);
-    auto step = [&](uint32_t j, DfTask &cur, DfTask &slot) {
-        if (j + 2 < mine) {
-            if (((j + 2) & (kWave - 1)) == 0) descs(j + 2);
-            df2_issue(D, sv, desc(j + 2), lane, slot);
-        }
-        if (j < mine) df2_run(D, sv, x, flags, cur, lane);
-    };
-    for (uint32_t j = 0; j < mine; j += 3) {
-        step(j, ta, tc);
-        step(j + 1, tb, ta);
-        step(j + 2, tc, tb);
+    uint4 dn = desc(0), dn2 = desc(1);
+    DfTask cur, nxt;
+    df2_issue(D, sv, dn, lane, nxt);
+    for (uint32_t j = 0; j < mine; ++j) {
+        cur = nxt;
+        dn = dn2;
+        dn2 = desc(j + 2);
+        df2_issue(D, sv, dn, lane, nxt);
+        df2_run(D, sv, x, flags, cur, lane);
     }
 }
 
@@ -1299,13 +1350,12 @@ __device__ __forceinline__ void wg_factor(const MatDesc *Md, const MatIO &io, in
         pert = kPertFactor * (float)wg_max(cm, sh);
     }
 
-    // 3. factorization, level by level; the tail with its values in LDS
+    // 3. factorization, level by level; LDS windows with their values in LDS
     //    (the sweeps' LDS is free until step 5)
     {
         FactorDev F = Md->f;
         globalize(F);
-        factor_levels_wg<__HIP_MEMORY_SCOPE_WORKGROUP>(F, io.lu, 0, F.tail_lev, pert, perturb, wlds);
-        if (F.tail_lev <= F.num_lev) factor_tail(F, io.lu, g_lds_vec, pert, perturb, wlds);
+        factor_all_wg(F, io.lu, g_lds_vec, pert, perturb, wlds);
     }
 
     // 4. the factors copied into the sweeps' task-contiguous layout
@@ -1434,7 +1484,7 @@ __device__ __forceinline__ MatIO batch_io(const MatDesc *desc, const double *ax,
     return io;
 }
 
-// Dynamic LDS: the factor tail's slots.
+// Dynamic LDS: the largest LDS window's values.
 template <bool kCopy>
 __global__ void __launch_bounds__(1024) k_batch_same_factor(const MatDesc *__restrict__ desc,
                                                             const double *__restrict__ ax, const double *__restrict__ b,
@@ -1445,7 +1495,7 @@ __global__ void __launch_bounds__(1024) k_batch_same_factor(const MatDesc *__res
                                                             int perturb)
 {
     __shared__ double sh[1024 / kWave];
-    __shared__ float wl[1024 / kWave][kWave];
+    __shared__ __attribute__((aligned(16))) float wl[1024 / kWave][kWave];
     const MatIO io = batch_io(desc, ax, b, x, lu, lucsr, t32, r64, norms, iters, eta);
     wg_factor<kCopy>(desc, io, perturb, sh, wl[threadIdx.x / kWave]);
 }
@@ -1475,7 +1525,7 @@ __global__ void __launch_bounds__(1024) k_batch_group_factor(const MatDesc *__re
                                                              const uint32_t *__restrict__ order, int perturb)
 {
     __shared__ double sh[1024 / kWave];
-    __shared__ float wl[1024 / kWave][kWave];
+    __shared__ __attribute__((aligned(16))) float wl[1024 / kWave][kWave];
     const uint32_t m = order[blockIdx.x];
     MatIO io = ios[m];
     globalize(io);
@@ -1573,7 +1623,7 @@ size_t batch_lds_bytes(int mode, uint32_t n)
     return mode == kSweepLevels ? 0 : 4 * words + (mode == kSweepLds ? 4 * (size_t)n : 0);
 }
 
-// Two launches: factor (dynamic LDS fact_lds: the tail's slots), then
+// Two launches: factor (dynamic LDS fact_lds: the largest window's values), then
 // refine in sweep mode `mode` (dynamic LDS ref_lds).
 void batch_same(const MatDesc *desc, int batch, const double *ax, const double *b, double *x, float *lu,
                 float *lucsr, float *t32, double *r64, double *norms, int32_t *iters, double *eta, double tol,
@@ -1612,9 +1662,10 @@ void batch_group(const MatDesc *descs, const void *ios, const uint32_t *order, i
         k_batch_group_refine<kSweepLevels><<<count, threads, ref_lds, st>>>(descs, io, order, tol, maxit);
 }
 
-void factor_tail(const FactorDev &F, float *val, const float *d_pert, int perturb, int threads, hipStream_t st)
+void factor_window(const FactorDev &F, int w, uint32_t nslot, float *val, const float *d_pert, int perturb,
+                   int threads, hipStream_t st)
 {
-    k_factor_tail<<<1, threads, 4 * (size_t)F.tail_nslot, st>>>(F, val, d_pert, perturb);
+    k_factor_window<<<1, threads, 4 * (size_t)nslot, st>>>(F, val, d_pert, w, perturb);
 }
 
 // Dataflow sweep of fused levels [l0, l1) (tasks [k0, k1)) in one workgroup.
@@ -1624,7 +1675,7 @@ void tri_df(const TriDev &T, const float *lu, float *x, int l0, int l1, uint32_t
     k_tri_df<<<1, threads, 4 * (size_t)((n + 31) / 32), st>>>(T, lu, x, l0, l1, k0, k1, n);
 }
 
-// Raises the dynamic-LDS cap of the batch, dataflow and tail kernels to the
+// Raises the dynamic-LDS cap of the batch, dataflow and window kernels to the
 // device's per-workgroup limit (less their static LDS) and returns that
 // limit in bytes (0 if no device).  Launches then size their own dynamic LDS
 // up to it.
@@ -1646,7 +1697,7 @@ size_t enable_lds()
         reinterpret_cast<const void *>(&k_batch_group_refine<kSweepFlags>),
         reinterpret_cast<const void *>(&k_batch_group_refine<kSweepLevels>),
         reinterpret_cast<const void *>(&k_tri_df),
-        reinterpret_cast<const void *>(&k_factor_tail),
+        reinterpret_cast<const void *>(&k_factor_window),
     };
     for (const void *f : fns) (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)avail);
     return avail;

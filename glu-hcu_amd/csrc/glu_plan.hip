@@ -50,7 +50,8 @@ void batch_group(const MatDesc *descs, const void *ios, const uint32_t *order, i
 void tri_df(const TriDev &T, const float *lu, float *x, int l0, int l1, uint32_t k0, uint32_t k1, uint32_t n,
             int threads, hipStream_t st);
 size_t enable_lds();
-void factor_tail(const FactorDev &F, float *val, const float *d_pert, int perturb, int threads, hipStream_t st);
+void factor_window(const FactorDev &F, int w, uint32_t nslot, float *val, const float *d_pert, int perturb,
+                   int threads, hipStream_t st);
 void zero_u64(unsigned long long *p, int count, hipStream_t st);
 void adjoint_values(const RefineDev &R, const double *lam, const double *x, double *grad, uint32_t n,
                     hipStream_t st);
@@ -209,6 +210,7 @@ struct Segment {
     bool fused;
     uint32_t items;  // grid segments: items (factor) or rows (solve)
     uint32_t k0 = 0, k1 = 0;   // solve segments: dataflow tasks [k0, k1)
+    int win = -1;              // factor segments: an LDS window
 };
 
 struct TriHost {
@@ -249,8 +251,11 @@ struct glu_plan {
     bool tri_dataflow = false;   // fused solve segments run barrier-free
     size_t lds_cap = 0;          // dynamic LDS per workgroup (bytes)
     uint32_t df_nval = 0;        // values of the per-matrix sweep copy
-    int tail_lev = 0;            // first step of the LDS-resident tail
-    uint32_t tail_nslot = 0;
+    int max_windows = 1;         // LDS windows allowed (from the last level)
+    int nwin = 0;                // LDS windows of the factorization
+    int win_steps = 0;           // steps they cover
+    uint32_t win_nslot = 0;      // the largest window's values
+    std::vector<uint32_t> win_slots;   // values of each window
 
     // hipGraph replay of the launch sequences (use_graph): one executable
     // graph per entry point, keyed on the buffers it was captured with
@@ -652,109 +657,177 @@ uint32_t level_lanes(const Tables &T, int l)
     return std::max({nc * 64, nm - ml, nd, ml * 64});
 }
 
-// Segments of the steps before the LDS tail.
-void factor_segments(glu_plan &p, const Tables &T)
-{
-    std::vector<uint32_t> work(p.tail_lev);
-    for (int l = 0; l < p.tail_lev; ++l) work[l] = level_lanes(T, l);
-    make_segments(work, (uint32_t)p.fuse_units, p.fsegs);
-}
+// LDS windows: runs of consecutive steps whose items touch at most `budget`
+// distinct CSC positions run inside one workgroup with those values staged
+// in LDS (slots, in increasing position order) and the run's items rewritten
+// to address slots.  A run shorter than kMinWindow steps stays in global
+// memory (staging would cost more than it saves).
+constexpr int kMinWindow = 6;
 
-// The LDS-resident tail: the longest run of final steps whose items touch
-// at most `budget` distinct CSC positions.  Those positions become LDS slots
-// (in increasing position order) and the run's items are rewritten to
-// address slots.
-struct Tail {
-    int lev = 0;
-    std::vector<uint32_t> pos;   // slot -> CSC position
+struct Window {
+    int l0 = 0, l1 = 0;            // steps [l0, l1)
+    std::vector<uint32_t> pos;     // slot -> CSC position
     std::vector<uint4> cln;
     std::vector<uint32_t> ctgt, cup;
     std::vector<uint32_t> ment;
     std::vector<MergeSrc> msrc;
     std::vector<uint2> dvd;
-    uint32_t c0 = 0, g0 = 0, m0 = 0, s0 = 0, d0 = 0;   // first tail task / target / item of each kind
+    uint32_t c0 = 0, g0 = 0, m0 = 0, s0 = 0, d0 = 0;   // first task / target / item of each kind
 };
 
-void choose_tail(const Tables &T, int num_lev, uint32_t nnz, uint32_t budget, Tail &t)
+// The CSC positions the items of step l read or write, each once per call
+// of f (duplicates possible).
+template <typename F>
+void step_positions(const Tables &T, int l, F f)
 {
-    t = Tail();
-    t.lev = num_lev + 1;
-    if (budget == 0) return;
-    std::vector<uint32_t> slot(nnz, UINT32_MAX);
+    for (uint32_t t = T.ctask_ptr[l]; t < T.ctask_ptr[l + 1]; ++t)
+        for (uint32_t lane = 0; lane < 64; ++lane) {
+            const uint4 &ln = T.cln[(size_t)t * 64 + lane];
+            if (ln.z == 0) continue;
+            f(ln.x), f(ln.y);
+            for (uint32_t e = 0; e < ln.z; ++e) {
+                const size_t q = 256 * (size_t)t + 64 * e + lane;
+                f(T.cup[q]);
+                if (T.ctgt[q] != UINT32_MAX) f(T.ctgt[q]);
+            }
+        }
+    for (uint32_t e = T.ment_ptr[l]; e < T.ment_ptr[l + 1]; ++e) {
+        f(T.ment[e]);
+        for (uint32_t k = T.msrc_ptr[e]; k < T.msrc_ptr[e + 1]; ++k) {
+            const MergeSrc &m = T.msrc[k];
+            f(m.lpos), f(m.upos), f(m.dpos);
+        }
+    }
+    if (l > 0)
+        for (uint32_t q = T.dvd_ptr[l - 1]; q < T.dvd_ptr[l]; ++q) {
+            f(T.dvd[q].x);
+            f(T.dvd[q].y & ~kFirstBit);
+        }
+}
+
+void make_window(const Tables &T, int a, int b, const std::vector<uint32_t> &slot_of_pos,
+                 const std::vector<uint32_t> &pos, Window &w)
+{
+    w.l0 = a;
+    w.l1 = b;
+    w.pos = pos;
+    auto slot = [&](uint32_t q) { return slot_of_pos[q]; };
+    w.c0 = T.ctask_ptr[a];
+    const uint32_t c1 = T.ctask_ptr[b];
+    w.g0 = 256 * w.c0;
+    // padding entries (no update): ~0u targets, U position slot 0
+    for (size_t q = w.g0; q < 256 * (size_t)c1; ++q) {
+        w.ctgt.push_back(T.ctgt[q] == UINT32_MAX ? UINT32_MAX : slot(T.ctgt[q]));
+        w.cup.push_back(slot(T.cup[q]) != UINT32_MAX ? slot(T.cup[q]) : 0u);
+    }
+    for (size_t q = (size_t)w.c0 * 64; q < (size_t)c1 * 64; ++q) {
+        uint4 ln = T.cln[q];
+        if (ln.z) ln = make_uint4(slot(ln.x), slot(ln.y), ln.z, 0u);
+        w.cln.push_back(ln);
+    }
+    w.m0 = T.ment_ptr[a];
+    const uint32_t m1 = T.ment_ptr[b];
+    w.s0 = T.msrc_ptr[w.m0];
+    for (uint32_t e = w.m0; e < m1; ++e) w.ment.push_back(slot(T.ment[e]));
+    for (uint32_t k = w.s0; k < T.msrc_ptr[m1]; ++k) {
+        const MergeSrc &m = T.msrc[k];
+        w.msrc.push_back(MergeSrc{slot(m.lpos), slot(m.upos), slot(m.dpos), 0});
+    }
+    w.d0 = a > 0 ? T.dvd_ptr[a - 1] : 0;
+    const uint32_t d1 = b > 0 ? T.dvd_ptr[b - 1] : 0;
+    for (uint32_t q = w.d0; q < d1; ++q) {
+        const uint2 &d = T.dvd[q];
+        w.dvd.push_back(make_uint2(slot(d.x), slot(d.y & ~kFirstBit) | (d.y & kFirstBit)));
+    }
+}
+
+// Windows chosen backward from the last step, so that the final run (the
+// longest chain of short steps) is one window.  Apart from that final
+// window, windows hold narrow steps only (at most `narrow` lanes): a window
+// runs in one workgroup, so a wide step inside it would give up the grid's
+// parallelism.
+void choose_windows(const Tables &T, int num_lev, uint32_t nnz, uint32_t budget, uint32_t narrow, int max_windows,
+                    std::vector<Window> &wins)
+{
+    wins.clear();
+    if (budget == 0 || max_windows <= 0) return;
+    std::vector<uint32_t> stamp(nnz, UINT32_MAX), slot_of_pos(nnz, UINT32_MAX);
     std::vector<uint32_t> fresh;
-    uint32_t count = 0;
-    auto touch = [&](uint32_t q) {
-        if (slot[q] == UINT32_MAX) {
-            slot[q] = 0;
-            fresh.push_back(q);
+    int b = num_lev + 1;
+    uint32_t gen = 0;
+    while (b > 0 && (int)wins.size() < max_windows) {
+        const uint32_t lim = b == num_lev + 1 ? UINT32_MAX : narrow;
+        if (level_lanes(T, b - 1) > lim) {
+            --b;
+            continue;
+        }
+        ++gen;
+        uint32_t count = 0;
+        int a = b;
+        while (a > 0 && level_lanes(T, a - 1) <= lim) {
+            fresh.clear();
+            step_positions(T, a - 1, [&](uint32_t q) {
+                if (stamp[q] != gen) {
+                    stamp[q] = gen;
+                    fresh.push_back(q);
+                }
+            });
+            if (count + fresh.size() > budget) {
+                for (uint32_t q : fresh) stamp[q] = UINT32_MAX;
+                break;
+            }
+            count += (uint32_t)fresh.size();
+            --a;
+        }
+        if (b - a < kMinWindow) {
+            b = a < b ? a : b - 1;   // too short: leave these steps in global memory
+            continue;
+        }
+        std::vector<uint32_t> pos;
+        for (int l = a; l < b; ++l)
+            step_positions(T, l, [&](uint32_t q) {
+                if (slot_of_pos[q] == UINT32_MAX) {
+                    slot_of_pos[q] = 0;
+                    pos.push_back(q);
+                }
+            });
+        std::sort(pos.begin(), pos.end());
+        for (uint32_t i = 0; i < pos.size(); ++i) slot_of_pos[pos[i]] = i;
+        wins.emplace_back();
+        make_window(T, a, b, slot_of_pos, pos, wins.back());
+        for (uint32_t q : pos) slot_of_pos[q] = UINT32_MAX;
+        b = a;
+    }
+    std::reverse(wins.begin(), wins.end());
+}
+
+// Segments of the factorization schedule: windows (one workgroup each) and,
+// between them, runs of global-memory steps split into fused and grid
+// segments.
+void factor_segments(glu_plan &p, const Tables &T, const std::vector<Window> &wins)
+{
+    p.fsegs.clear();
+    int l = 0;
+    auto global_run = [&](int a, int b) {
+        if (a >= b) return;
+        std::vector<uint32_t> work(b - a);
+        for (int q = a; q < b; ++q) work[q - a] = level_lanes(T, q);
+        std::vector<Segment> segs;
+        make_segments(work, (uint32_t)p.fuse_units, segs);
+        for (Segment s : segs) {
+            s.l0 += a;
+            s.l1 += a;
+            p.fsegs.push_back(s);
         }
     };
-    for (int l = num_lev; l >= 0; --l) {
-        fresh.clear();
-        for (uint32_t t = T.ctask_ptr[l]; t < T.ctask_ptr[l + 1]; ++t)
-            for (uint32_t lane = 0; lane < 64; ++lane) {
-                const uint4 &ln = T.cln[(size_t)t * 64 + lane];
-                if (ln.z == 0) continue;
-                touch(ln.x), touch(ln.y);
-                for (uint32_t e = 0; e < ln.z; ++e) {
-                    const size_t q = 256 * (size_t)t + 64 * e + lane;
-                    touch(T.cup[q]);
-                    if (T.ctgt[q] != UINT32_MAX) touch(T.ctgt[q]);
-                }
-            }
-        for (uint32_t e = T.ment_ptr[l]; e < T.ment_ptr[l + 1]; ++e) {
-            touch(T.ment[e]);
-            for (uint32_t k = T.msrc_ptr[e]; k < T.msrc_ptr[e + 1]; ++k) {
-                const MergeSrc &m = T.msrc[k];
-                touch(m.lpos), touch(m.upos), touch(m.dpos);
-            }
-        }
-        if (l > 0)
-            for (uint32_t q = T.dvd_ptr[l - 1]; q < T.dvd_ptr[l]; ++q) {
-                touch(T.dvd[q].x);
-                touch(T.dvd[q].y & ~kFirstBit);
-            }
-        if (count + fresh.size() > budget) {
-            for (uint32_t q : fresh) slot[q] = UINT32_MAX;
-            break;
-        }
-        count += (uint32_t)fresh.size();
-        t.lev = l;
+    for (size_t w = 0; w < wins.size(); ++w) {
+        global_run(l, wins[w].l0);
+        Segment s{wins[w].l0, wins[w].l1, true, 0};
+        s.win = (int)w;
+        p.fsegs.push_back(s);
+        l = wins[w].l1;
     }
-    if (t.lev > num_lev) return;
-    for (uint32_t q = 0; q < nnz; ++q)
-        if (slot[q] != UINT32_MAX) {
-            slot[q] = (uint32_t)t.pos.size();
-            t.pos.push_back(q);
-        }
-    const int L = t.lev;
-    t.c0 = T.ctask_ptr[L];
-    const uint32_t c1 = T.ctask_ptr[num_lev + 1];
-    t.g0 = 256 * t.c0;
-    // padding entries (no update) keep their values: ~0u targets, 0 U positions
-    for (uint32_t q = t.g0; q < T.ctgt.size(); ++q) {
-        const bool live = T.ctgt[q] != UINT32_MAX || slot[T.cup[q]] != UINT32_MAX;
-        t.ctgt.push_back(T.ctgt[q] == UINT32_MAX ? UINT32_MAX : slot[T.ctgt[q]]);
-        t.cup.push_back(live && slot[T.cup[q]] != UINT32_MAX ? slot[T.cup[q]] : 0u);
-    }
-    for (size_t q = (size_t)t.c0 * 64; q < (size_t)c1 * 64; ++q) {
-        uint4 ln = T.cln[q];
-        if (ln.z) ln = make_uint4(slot[ln.x], slot[ln.y], ln.z, 0u);
-        t.cln.push_back(ln);
-    }
-    t.m0 = T.ment_ptr[L];
-    const uint32_t m1 = T.ment_ptr[num_lev + 1];
-    t.s0 = T.msrc_ptr[t.m0];
-    for (uint32_t e = t.m0; e < m1; ++e) t.ment.push_back(slot[T.ment[e]]);
-    for (uint32_t k = t.s0; k < T.msrc_ptr[m1]; ++k) {
-        const MergeSrc &m = T.msrc[k];
-        t.msrc.push_back(MergeSrc{slot[m.lpos], slot[m.upos], slot[m.dpos], 0});
-    }
-    t.d0 = L > 0 ? T.dvd_ptr[L - 1] : 0;
-    for (uint32_t q = t.d0; q < T.dvd_ptr[num_lev]; ++q) {
-        const uint2 &d = T.dvd[q];
-        t.dvd.push_back(make_uint2(slot[d.x], slot[d.y & ~kFirstBit] | (d.y & kFirstBit)));
-    }
+    global_run(l, p.num_lev + 1);
 }
 
 void tri_segments(glu_plan &p, TriHost &H)
@@ -877,6 +950,10 @@ void build_df(const TriHost &H, Entries entries, bool div, Pivot pivot, DfHost &
             for (uint32_t l = 0; l < 64; ++l) put(p0 + l < p && e < ent[p0 + l].size() ? ent[p0 + l][e] : pad);
         D.task.push_back(make_uint4(base, p0, p - p0, m));
     }
+    // padding, so that the kernels' loads need no bounds tests: kDfK passes
+    // of 64 slots after the last task, 64 positions after the last row
+    for (uint32_t q = 0; q < kDfK * 64; ++q) put(pad);
+    D.rl.resize(n + 64, make_uint2(0u, 0u));
     if (div) {
         D.pvpos.resize(n);
         for (uint32_t q = 0; q < n; ++q) D.pvpos[q] = pivot(order[q]);
@@ -924,6 +1001,8 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
         if (opts->fuse_units < 0) return fail(GLU_ERR_ARGUMENT, "fuse_units must be >= 0");
         if (opts->fuse_units) p->fuse_units = opts->fuse_units;
         p->use_graph = opts->use_graph != 0;
+        if (opts->lds_windows < -1) return fail(GLU_ERR_ARGUMENT, "lds_windows must be >= -1");
+        if (opts->lds_windows) p->max_windows = opts->lds_windows < 0 ? 0 : opts->lds_windows;
     }
     p->n = P.n;
     p->nnz = P.nnz;
@@ -939,11 +1018,16 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(GLU_ERR_DEVICE, "no HIP device");
     HIP_TRY(hipGetDevice(&p->device));
     p->lds_cap = launch::enable_lds();
-    Tail tail;
-    choose_tail(T, P.num_lev, P.nnz, (uint32_t)(p->lds_cap / sizeof(float)), tail);
-    p->tail_lev = tail.lev;
-    p->tail_nslot = (uint32_t)tail.pos.size();
-    factor_segments(*p, T);
+    std::vector<Window> wins;
+    choose_windows(T, P.num_lev, P.nnz, (uint32_t)(p->lds_cap / sizeof(float)), (uint32_t)p->fuse_units,
+                   p->max_windows, wins);
+    p->nwin = (int)wins.size();
+    for (const Window &w : wins) {
+        p->win_steps += w.l1 - w.l0;
+        p->win_nslot = std::max<uint32_t>(p->win_nslot, (uint32_t)w.pos.size());
+        p->win_slots.push_back((uint32_t)w.pos.size());
+    }
+    factor_segments(*p, T, wins);
     p->units = (int64_t)(T.cln.size() / 64);
 
     const uint32_t n = P.n;
@@ -976,6 +1060,7 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
     std::vector<uint32_t> df_vpos(dlo.vpos);
     df_vpos.insert(df_vpos.end(), dup.vpos.begin(), dup.vpos.end());
     df_vpos.insert(df_vpos.end(), dup.pvpos.begin(), dup.pvpos.end());
+    df_vpos.resize(df_vpos.size() + 64, UINT32_MAX);   // pivot reads of the last positions' lanes
     p->df_nval = (uint32_t)df_vpos.size();
 
     // Refinement / refresh tables from the analysis.
@@ -1058,16 +1143,21 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
     const size_t o_msrc = U.put(T.msrc);
     const size_t o_dvd = U.put(T.dvd);
     const size_t o_dptr = U.put(T.dvd_ptr);
-    const size_t o_tpos = U.put(tail.pos);
+    std::vector<size_t> o_wpos(wins.size()), o_wcln(wins.size()), o_wctgt(wins.size()), o_wcup(wins.size()),
+        o_wment(wins.size()), o_wmsrc(wins.size()), o_wdvd(wins.size());
+    for (size_t w = 0; w < wins.size(); ++w) {
+        o_wpos[w] = U.put(wins[w].pos);
+        o_wcln[w] = U.put(wins[w].cln);
+        o_wctgt[w] = U.put(wins[w].ctgt);
+        o_wcup[w] = U.put(wins[w].cup);
+        o_wment[w] = U.put(wins[w].ment);
+        o_wmsrc[w] = U.put(wins[w].msrc);
+        o_wdvd[w] = U.put(wins[w].dvd);
+    }
+    const size_t o_wdev = U.reserve(sizeof(WinDev) * std::max<size_t>(wins.size(), 1));
     const size_t o_dlo_task = U.put(dlo.task), o_dlo_rl = U.put(dlo.rl), o_dlo_idx = U.put(dlo.idx);
     const size_t o_dup_task = U.put(dup.task), o_dup_rl = U.put(dup.rl), o_dup_idx = U.put(dup.idx);
     const size_t o_dfvpos = U.put(df_vpos);
-    const size_t o_tcln = U.put(tail.cln);
-    const size_t o_tctgt = U.put(tail.ctgt);
-    const size_t o_tcup = U.put(tail.cup);
-    const size_t o_tment = U.put(tail.ment);
-    const size_t o_tmsrc = U.put(tail.msrc);
-    const size_t o_tdvd = U.put(tail.dvd);
     size_t o_trows[4], o_tptr[4], o_tlong[4], o_ttask[4];
     for (int s = 0; s < 4; ++s) {
         o_trows[s] = U.put(p->tri[s].rows);
@@ -1132,18 +1222,31 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
     F.num_lev = P.num_lev;
     F.n = n;
     F.nnz = P.nnz;
-    // tail items are indexed like the full tables: shift each base back by
-    // the index of its first tail item (only indices >= that are used)
-    F.tail_lev = tail.lev;
-    F.tail_nslot = p->tail_nslot;
-    F.tail_pos = u32p(o_tpos);
+    // window items are indexed like the full tables: shift each base back
+    // by the index of its first item (only indices >= that are used)
     auto shifted = [&](size_t off, size_t first, size_t item) { return (uintptr_t)(A + off) - first * item; };
-    F.t_cln = reinterpret_cast<const uint4 *>(shifted(o_tcln, (size_t)tail.c0 * 64, sizeof(uint4)));
-    F.t_ctgt = reinterpret_cast<const uint32_t *>(shifted(o_tctgt, tail.g0, sizeof(uint32_t)));
-    F.t_cup = reinterpret_cast<const uint32_t *>(shifted(o_tcup, tail.g0, sizeof(uint32_t)));
-    F.t_ment = reinterpret_cast<const uint32_t *>(shifted(o_tment, tail.m0, sizeof(uint32_t)));
-    F.t_msrc = reinterpret_cast<const MergeSrc *>(shifted(o_tmsrc, tail.s0, sizeof(MergeSrc)));
-    F.t_dvd = reinterpret_cast<const uint2 *>(shifted(o_tdvd, tail.d0, sizeof(uint2)));
+    {
+        std::vector<WinDev> wd(wins.size());
+        for (size_t w = 0; w < wins.size(); ++w) {
+            const Window &x = wins[w];
+            WinDev &d = wd[w];
+            d.l0 = x.l0;
+            d.l1 = x.l1;
+            d.nslot = (uint32_t)x.pos.size();
+            d.pos = u32p(o_wpos[w]);
+            d.cln = reinterpret_cast<const uint4 *>(shifted(o_wcln[w], (size_t)x.c0 * 64, sizeof(uint4)));
+            d.ctgt = reinterpret_cast<const uint32_t *>(shifted(o_wctgt[w], x.g0, sizeof(uint32_t)));
+            d.cup = reinterpret_cast<const uint32_t *>(shifted(o_wcup[w], x.g0, sizeof(uint32_t)));
+            d.ment = reinterpret_cast<const uint32_t *>(shifted(o_wment[w], x.m0, sizeof(uint32_t)));
+            d.msrc = reinterpret_cast<const MergeSrc *>(shifted(o_wmsrc[w], x.s0, sizeof(MergeSrc)));
+            d.dvd = reinterpret_cast<const uint2 *>(shifted(o_wdvd[w], x.d0, sizeof(uint2)));
+        }
+        if (!wd.empty())
+            HIP_TRY(hipMemcpy(A + o_wdev, wd.data(), sizeof(WinDev) * wd.size(), hipMemcpyHostToDevice));
+        F.win = reinterpret_cast<const WinDev *>(A + o_wdev);
+        F.nwin = (int32_t)wins.size();
+        F.win_nslot = p->win_nslot;
+    }
 
     auto tri = [&](int s, const uint32_t *beg, uint32_t begoff, const uint32_t *end, const uint32_t *idx,
                    const uint32_t *vpos, const uint32_t *dpos, int desc) {
@@ -1252,12 +1355,13 @@ glu_status run_factor(glu_plan *p, float *d_val, int perturb, hipStream_t st)
 {
     if (perturb) launch::pert(p->F, d_val, p->d_norm_bits, p->d_pert, st);
     for (const Segment &s : p->fsegs) {
-        if (s.fused)
+        if (s.win >= 0)
+            launch::factor_window(p->F, s.win, p->win_slots[s.win], d_val, p->d_pert, perturb, p->fused_threads, st);
+        else if (s.fused)
             launch::factor_fused(p->F, d_val, p->d_pert, s.l0, s.l1, perturb, p->fused_threads, st);
         else
             launch::factor_grid(p->F, d_val, p->d_pert, s.l0, s.items, perturb, kGridThreads, st);
     }
-    if (p->tail_lev <= p->num_lev) launch::factor_tail(p->F, d_val, p->d_pert, perturb, p->fused_threads, st);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(GLU_ERR_DEVICE, std::string("factor launch: ") + hipGetErrorString(e));
     return GLU_OK;
@@ -1354,8 +1458,8 @@ extern "C" glu_status glu_plan_info(const glu_plan *p, int64_t info[10])
     info[5] = (int64_t)p->fsegs.size();
     info[6] = (int64_t)p->arena_bytes;
     info[7] = p->units;
-    info[8] = p->tail_lev;
-    info[9] = p->tail_nslot;
+    info[8] = p->win_steps;
+    info[9] = p->win_nslot;
     return GLU_OK;
 }
 
@@ -1532,7 +1636,7 @@ extern "C" glu_status glu_batch_solve(glu_plan *p, int batch, const double *d_ax
     const int mode = sweep_mode(p->n, cap);
     launch::batch_same(p->d_desc, batch, d_ax, d_b, d_x, d_lu, p->d_blucsr, p->d_bt32, p->d_br64, p->d_bnorms,
                        d_iters, d_eta, tol, maxit, (int)(flags & 1u), p->fused_threads, mode,
-                       sizeof(float) * p->tail_nslot, launch::batch_lds_bytes(mode, p->n),
+                       sizeof(float) * p->win_nslot, launch::batch_lds_bytes(mode, p->n),
                        static_cast<hipStream_t>(stream));
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(GLU_ERR_DEVICE, std::string("batch launch: ") + hipGetErrorString(e));
@@ -1643,7 +1747,7 @@ extern "C" glu_status glu_group_solve(glu_group *g, const double *const *d_ax, c
     size_t nmax = 0, tmax = 0;
     for (glu_plan *p : g->plans) {
         nmax = std::max<size_t>(nmax, p->n);
-        tmax = std::max<size_t>(tmax, sizeof(float) * p->tail_nslot);
+        tmax = std::max<size_t>(tmax, sizeof(float) * p->win_nslot);
     }
     const size_t cap = launch::enable_lds();
     const int mode = sweep_mode((uint32_t)nmax, cap);

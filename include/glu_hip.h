@@ -112,7 +112,10 @@ typedef struct glu_plan_opts {
     int32_t block_threads;  /* threads per workgroup of the fused and batched
                                kernels (64..1024, multiple of 64); 0 = default */
     int32_t use_graph;      /* 1 = replay the level schedule from a hipGraph */
-    int32_t reserved[5];
+    int32_t lds_windows;    /* runs of narrow levels factored with their values
+                               staged in LDS, counted from the last level:
+                               0 = default (1), -1 = none, k > 0 = at most k */
+    int32_t reserved[4];
 } glu_plan_opts;
 
 glu_status glu_plan_create(const glu_pattern *pat, const glu_plan_opts *opts, glu_plan **out);
