@@ -249,6 +249,7 @@ struct glu_plan {
     std::vector<Segment> fsegs;
     TriHost tri[4];          // lo, up, tup, tlo
     bool tri_dataflow = false;   // fused solve segments run barrier-free
+    int solve_df_opt = 0;        // glu_plan_opts.solve_dataflow
     size_t lds_cap = 0;          // dynamic LDS per workgroup (bytes)
     uint32_t df_nval = 0;        // values of the per-matrix sweep copy
     int max_windows = 1;         // LDS windows allowed (from the last level)
@@ -1003,6 +1004,9 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
         p->use_graph = opts->use_graph != 0;
         if (opts->lds_windows < -1) return fail(GLU_ERR_ARGUMENT, "lds_windows must be >= -1");
         if (opts->lds_windows) p->max_windows = opts->lds_windows < 0 ? 0 : opts->lds_windows;
+        if (opts->solve_dataflow < -1 || opts->solve_dataflow > 1)
+            return fail(GLU_ERR_ARGUMENT, "solve_dataflow must be -1, 0 or 1");
+        p->solve_df_opt = opts->solve_dataflow;
     }
     p->n = P.n;
     p->nnz = P.nnz;
@@ -1289,7 +1293,7 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
     S.up_df = df(dup, o_dup_task, o_dup_rl, o_dup_idx, 1, (uint32_t)dlo.idx.size());
     S.df_vpos = u32p(o_dfvpos);
     S.df_nval = p->df_nval;
-    p->tri_dataflow = p->lds_cap >= 4 * (size_t)((n + 31) / 32);
+    p->tri_dataflow = p->solve_df_opt >= 0 && p->lds_cap >= 4 * (size_t)((n + 31) / 32);
     if (an) {
         S.row_perm = u32p(o_rp);
         S.row_perm_inv = u32p(o_rpi);

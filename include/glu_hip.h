@@ -115,7 +115,10 @@ typedef struct glu_plan_opts {
     int32_t lds_windows;    /* runs of narrow levels factored with their values
                                staged in LDS, counted from the last level:
                                0 = default (1), -1 = none, k > 0 = at most k */
-    int32_t reserved[4];
+    int32_t solve_dataflow; /* narrow levels of the single-matrix solves run
+                               barrier-free in one workgroup: 0 = default (on
+                               when LDS holds the ready bits), 1 = on, -1 = off */
+    int32_t reserved[3];
 } glu_plan_opts;
 
 glu_status glu_plan_create(const glu_pattern *pat, const glu_plan_opts *opts, glu_plan **out);
