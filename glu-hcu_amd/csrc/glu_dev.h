@@ -36,6 +36,7 @@ constexpr uint32_t kDfLaneRow = 16;
 
 // Rows / merge entries longer than this are processed by a whole wave.
 constexpr uint32_t kLongRow = 64;
+constexpr int kMaxKrylov = 4;     // largest restart length of the batch refinement
 constexpr uint32_t kLongMerge = 32;
 
 // Flag of a division item: the first L entry of its column, whose lane also
@@ -182,6 +183,8 @@ struct MatDesc {
     SolveDev s;
     RefineDev r;
     uint32_t n, nnz;
+    uint32_t krylov;   // Arnoldi vectors per refinement cycle (0: plain refinement)
+    uint32_t pad;
 };
 
 }  // namespace glu

@@ -983,6 +983,7 @@ struct MatIO {
     double *r64;        // n scratch
     float *lucsr;       // df_nval scratch: the sweeps' value copy
     double *norms;      // 2 scratch: |A|_inf, |b|_inf (factor -> refine kernel)
+    double *kry;        // (3 krylov + 1) n scratch: Arnoldi vectors V, Z = M V, W = A Z (null: plain refinement)
     int32_t *iters;
     double *eta;
 };
@@ -1086,6 +1087,7 @@ __device__ __forceinline__ void globalize(MatIO &io)
     io.r64 = glb(io.r64);
     if (io.lucsr) io.lucsr = glb(io.lucsr);
     io.norms = glb(io.norms);
+    if (io.kry) io.kry = glb(io.kry);
     io.iters = glb(io.iters);
     io.eta = glb(io.eta);
 }
@@ -1382,70 +1384,229 @@ __device__ __forceinline__ void wg_factor(const MatDesc *Md, const MatIO &io, in
     }
 }
 
-// Refinement of one matrix by one workgroup: x += LU^-1 r (FP32 sweeps),
-// r = b - A x (FP64), until eta <= tol or maxit.  xs is the FP32
-// correction vector; flags the sweeps' ready bits (kSweepLevels: unused).
+// Workgroup sum in a fixed order (deterministic): each wave's butterfly,
+// then the waves' sums in wave order.  Ends with a barrier.
+__device__ double wg_sum(double v, double *sh)
+{
+    v = wave_sum64(v);
+    const int nw = blockDim.x / kWave;
+    if ((threadIdx.x & (kWave - 1)) == 0) sh[threadIdx.x / kWave] = v;
+    __syncthreads();
+    double r = sh[0];
+    for (int i = 1; i < nw; ++i) r += sh[i];
+    __syncthreads();
+    return r;
+}
+
+// xs = (LU)^-1 applied to v in the factors' frame: xs[j] = v[col_perm[j]] *
+// col_scale_perm[j] rounded to FP32, then the FP32 L and U sweeps in place
+// (Symbolic_Matrix::solve's permutation and scaling, symbolic.cc:215-239).
+// The caller maps xs back with row_perm_inv / row_scale.
+template <int kMode>
+__device__ __forceinline__ void wg_precond(const MatDesc *Md, const MatIO &io, const double *v, float *xs,
+                                           uint32_t *flags)
+{
+    const uint32_t n = Md->n;
+    const uint32_t tid = threadIdx.x, nt = blockDim.x;
+    const uint32_t nwords = (n + 31) / 32;
+    {
+        const uint32_t *cp = glb(Md->s.col_perm);
+        const double *csp = glb(Md->s.col_scale_perm);
+        for (uint32_t j = tid; j < n; j += nt) xs[j] = (float)(v[cp[j]] * csp[j]);
+    }
+    if (kMode == kSweepLevels) {
+        __syncthreads();
+        TriDev lo = Md->s.lo, up = Md->s.up;
+        globalize(lo);
+        globalize(up);
+        wg_tri_refine(lo, io.lu, xs);
+        wg_tri_refine(up, io.lu, xs);
+    } else {
+        for (uint32_t i = tid; i < nwords; i += nt) flags[i] = 0u;
+        __syncthreads();
+        {
+            DfDev lo = Md->s.lo_df;
+            globalize(lo);
+            df2_sweep(lo, io.lucsr + lo.voff, xs, flags);
+        }
+        __syncthreads();
+        for (uint32_t i = tid; i < nwords; i += nt) flags[i] = 0u;
+        __syncthreads();
+        {
+            DfDev up = Md->s.up_df;
+            globalize(up);
+            df2_sweep(up, io.lucsr + up.voff, xs, flags);
+        }
+        __syncthreads();
+    }
+}
+
+// r = b - A x in FP64 (into io.r64); returns the backward error
+// eta = |r|_inf / (|A|_inf |x|_inf + |b|_inf) to every thread.
+__device__ __forceinline__ double wg_residual(const MatDesc *Md, const MatIO &io, double anorm, double bnorm,
+                                              double *sh)
+{
+    const uint32_t n = Md->n;
+    const uint32_t tid = threadIdx.x, nt = blockDim.x;
+    double rmax = 0.0, xmax = 0.0;
+    RefineDev R = Md->r;
+    globalize(R);
+    wg_rows_of_a(R, io.ax, io.x, n, [&](uint32_t i, double dot, double) {
+        const double ri = io.b[i] - dot;
+        io.r64[i] = ri;
+        rmax = nmax(rmax, fabs(ri));
+    });
+    for (uint32_t i = tid; i < n; i += nt) xmax = nmax(xmax, fabs(io.x[i]));
+    const double rn = wg_max(rmax, sh);
+    const double xn = wg_max(xmax, sh);
+    return rn / (anorm * xn + bnorm);
+}
+
+// LDS of the Arnoldi cycle (doubles): the (kMaxKrylov + 1) x kMaxKrylov
+// Hessenberg matrix H (rotated in place into R), the least-squares
+// coefficients y, the rotated right-hand side g, the Givens cosines and
+// sines, and the change of y since the previous step.
+constexpr int kKrylovY = (kMaxKrylov + 1) * kMaxKrylov;
+constexpr int kKrylovG = kKrylovY + kMaxKrylov;
+constexpr int kKrylovC = kKrylovG + kMaxKrylov + 1;
+constexpr int kKrylovS = kKrylovC + kMaxKrylov;
+constexpr int kKrylovD = kKrylovS + kMaxKrylov;
+constexpr int kKrylovLds = kKrylovD + kMaxKrylov;
+
+// Refinement of one matrix by one workgroup until eta <= tol or maxit
+// preconditioner solves, from x = 0, r = b.  M = (LU)^-1 is the FP32 sweep
+// pair; every other quantity is FP64.
+//  * Md->krylov = 0 (or no workspace): plain refinement, x += M r, then
+//    r = b - A x and eta from it.
+//  * Md->krylov = m > 0: restarted flexible GMRES(m), right-preconditioned
+//    by M.  A cycle starts from the current residual, v_0 = r / |r|_2; step
+//    j stores z_j = M v_j and w_j = A z_j, orthogonalizes w_j against V by
+//    modified Gram-Schmidt into v_{j+1}, extends the Givens QR of the
+//    Hessenberg matrix and solves min |beta e_1 - H y|_2; x and r move by
+//    the change of y: x += sum dy_c z_c, r -= sum dy_c w_c (r stays b - A x
+//    up to FP64 rounding, as every w_c is an FP64 product A z_c).  eta is
+//    checked after every solve, so a matrix stops as soon as it meets tol.
+// Every workgroup-wide sum has a fixed order: results do not depend on
+// timing.  xs is the FP32 sweep vector; flags the sweeps' ready bits.
 template <int kMode>
 __device__ __forceinline__ void wg_refine(const MatDesc *Md, const MatIO &io, double tol, int maxit, double *sh,
-                                          float *xs, uint32_t *flags)
+                                          double *hs, float *xs, uint32_t *flags)
 {
     const uint32_t n = Md->n;
     const uint32_t tid = threadIdx.x, nt = blockDim.x;
     const double anorm = io.norms[0], bnorm = io.norms[1];
-    const uint32_t nwords = (n + 31) / 32;
+    const int m = io.kry ? (int)Md->krylov : 0;
+    const uint32_t *rpi = glb(Md->s.row_perm_inv);
+    const double *rs = glb(Md->s.row_scale);
     int it = 0;
     double eta = 0.0;
-    for (;;) {
-        {
-            const uint32_t *cp = glb(Md->s.col_perm);
-            const double *csp = glb(Md->s.col_scale_perm);
-            for (uint32_t j = tid; j < n; j += nt) xs[j] = (float)(io.r64[cp[j]] * csp[j]);
+    if (m == 0) {
+        for (;;) {
+            wg_precond<kMode>(Md, io, io.r64, xs, flags);
+            ++it;
+            for (uint32_t i = tid; i < n; i += nt) io.x[i] = io.x[i] + (double)xs[rpi[i]] * rs[i];
+            __syncthreads();
+            eta = wg_residual(Md, io, anorm, bnorm, sh);
+            if (!(eta > tol) || it >= maxit) break;   // also stops on NaN
         }
-        if (kMode == kSweepLevels) {
-            __syncthreads();
-            TriDev lo = Md->s.lo, up = Md->s.up;
-            globalize(lo);
-            globalize(up);
-            wg_tri_refine(lo, io.lu, xs);
-            wg_tri_refine(up, io.lu, xs);
-        } else {
-            for (uint32_t i = tid; i < nwords; i += nt) flags[i] = 0u;
-            __syncthreads();
-            {
-                DfDev lo = Md->s.lo_df;
-                globalize(lo);
-                df2_sweep(lo, io.lucsr + lo.voff, xs, flags);
+    } else {
+        double *const V = io.kry;                            // m + 1 vectors
+        double *const Z = V + (size_t)(m + 1) * n;           // m
+        double *const W = Z + (size_t)m * n;                 // m
+        double *const H = hs, *const y = hs + kKrylovY, *const g = hs + kKrylovG;
+        double *const cs = hs + kKrylovC, *const sn = hs + kKrylovS, *const dy = hs + kKrylovD;
+        bool done = false;
+        while (!done) {
+            double d = 0.0;
+            for (uint32_t i = tid; i < n; i += nt) d = fma(io.r64[i], io.r64[i], d);
+            const double beta = sqrt(wg_sum(d, sh));
+            if (!(beta > 0.0)) {                  // r = 0 (x exact) or NaN
+                eta = beta == 0.0 ? 0.0 : beta;
+                break;
+            }
+            for (uint32_t i = tid; i < n; i += nt) V[i] = io.r64[i] / beta;
+            if (tid == 0) {
+                g[0] = beta;
+                for (int c = 0; c < kMaxKrylov; ++c) y[c] = 0.0;
             }
             __syncthreads();
-            for (uint32_t i = tid; i < nwords; i += nt) flags[i] = 0u;
-            __syncthreads();
-            {
-                DfDev up = Md->s.up_df;
-                globalize(up);
-                df2_sweep(up, io.lucsr + up.voff, xs, flags);
+            for (int j = 0; j < m; ++j) {
+                double *zj = Z + (size_t)j * n, *wj = W + (size_t)j * n, *vn = V + (size_t)(j + 1) * n;
+                wg_precond<kMode>(Md, io, V + (size_t)j * n, xs, flags);
+                ++it;
+                for (uint32_t i = tid; i < n; i += nt) zj[i] = (double)xs[rpi[i]] * rs[i];
+                __syncthreads();
+                {
+                    RefineDev R = Md->r;
+                    globalize(R);
+                    wg_rows_of_a(R, io.ax, zj, n, [&](uint32_t i, double dot, double) {
+                        wj[i] = dot;
+                        vn[i] = dot;
+                    });
+                }
+                __syncthreads();
+                for (int q = 0; q <= j; ++q) {
+                    const double *vq = V + (size_t)q * n;
+                    double dq = 0.0;
+                    for (uint32_t i = tid; i < n; i += nt) dq = fma(vn[i], vq[i], dq);
+                    const double h = wg_sum(dq, sh);
+                    for (uint32_t i = tid; i < n; i += nt) vn[i] = fma(-h, vq[i], vn[i]);
+                    if (tid == 0) H[q * kMaxKrylov + j] = h;
+                }
+                double dn = 0.0;
+                for (uint32_t i = tid; i < n; i += nt) dn = fma(vn[i], vn[i], dn);
+                const double hn = sqrt(wg_sum(dn, sh));
+                if (hn > 0.0)
+                    for (uint32_t i = tid; i < n; i += nt) vn[i] = vn[i] / hn;
+                if (tid == 0) {
+                    // previous rotations on column j, a new one zeroing hn
+                    for (int c = 0; c < j; ++c) {
+                        const double t1 = H[c * kMaxKrylov + j], t2 = H[(c + 1) * kMaxKrylov + j];
+                        H[c * kMaxKrylov + j] = cs[c] * t1 + sn[c] * t2;
+                        H[(c + 1) * kMaxKrylov + j] = cs[c] * t2 - sn[c] * t1;
+                    }
+                    const double a = H[j * kMaxKrylov + j];
+                    const double rr = hypot(a, hn);
+                    cs[j] = rr > 0.0 ? a / rr : 1.0;
+                    sn[j] = rr > 0.0 ? hn / rr : 0.0;
+                    H[j * kMaxKrylov + j] = rr;
+                    g[j + 1] = -sn[j] * g[j];
+                    g[j] = cs[j] * g[j];
+                    for (int c = j; c >= 0; --c) {
+                        double t = g[c];
+                        for (int cc = c + 1; cc <= j; ++cc) t -= H[c * kMaxKrylov + cc] * dy[cc];
+                        dy[c] = t / H[c * kMaxKrylov + c];   // the new y, for now
+                    }
+                    for (int c = 0; c <= j; ++c) {
+                        const double yn = dy[c];
+                        dy[c] = yn - y[c];
+                        y[c] = yn;
+                    }
+                }
+                __syncthreads();
+                double rmax = 0.0, xmax = 0.0;
+                for (uint32_t i = tid; i < n; i += nt) {
+                    double xi = io.x[i], ri = io.r64[i];
+                    for (int c = 0; c <= j; ++c) {
+                        const double dc = dy[c];
+                        xi = fma(dc, Z[(size_t)c * n + i], xi);
+                        ri = fma(-dc, W[(size_t)c * n + i], ri);
+                    }
+                    io.x[i] = xi;
+                    io.r64[i] = ri;
+                    rmax = nmax(rmax, fabs(ri));
+                    xmax = nmax(xmax, fabs(xi));
+                }
+                const double rn = wg_max(rmax, sh);
+                const double xn = wg_max(xmax, sh);
+                eta = rn / (anorm * xn + bnorm);
+                if (!(eta > tol) || it >= maxit) {   // also stops on NaN
+                    done = true;
+                    break;
+                }
+                if (!(hn > 0.0)) break;   // breakdown short of tol: restart from r
             }
-            __syncthreads();
         }
-        {
-            const uint32_t *rpi = glb(Md->s.row_perm_inv);
-            const double *rs = glb(Md->s.row_scale);
-            for (uint32_t j = tid; j < n; j += nt) io.x[j] = io.x[j] + (double)xs[rpi[j]] * rs[j];
-        }
-        __syncthreads();
-        double rmax = 0.0, xmax = 0.0;
-        RefineDev R = Md->r;
-        globalize(R);
-        wg_rows_of_a(R, io.ax, io.x, n, [&](uint32_t i, double dot, double) {
-            const double ri = io.b[i] - dot;
-            io.r64[i] = ri;
-            rmax = nmax(rmax, fabs(ri));
-        });
-        for (uint32_t i = tid; i < n; i += nt) xmax = nmax(xmax, fabs(io.x[i]));
-        const double rn = wg_max(rmax, sh);
-        const double xn = wg_max(xmax, sh);
-        eta = rn / (anorm * xn + bnorm);
-        ++it;
-        if (!(eta > tol) || it >= maxit) break;   // also stops on NaN
     }
     if (tid == 0) {
         *io.iters = it;
@@ -1465,7 +1626,7 @@ __device__ __forceinline__ void batch_lds(uint32_t n, float *t32, uint32_t *&fla
 // Same pattern: matrix b = blockIdx.x uses desc[0] and contiguous buffers.
 __device__ __forceinline__ MatIO batch_io(const MatDesc *desc, const double *ax, const double *b, double *x,
                                           float *lu, float *lucsr, float *t32, double *r64, double *norms,
-                                          int32_t *iters, double *eta)
+                                          double *kry, int32_t *iters, double *eta)
 {
     const uint32_t n = desc->n, nnz = desc->nnz, nnz_a = desc->r.nnz_a, nval = desc->s.df_nval;
     const uint64_t m = blockIdx.x;
@@ -1478,6 +1639,7 @@ __device__ __forceinline__ MatIO batch_io(const MatDesc *desc, const double *ax,
     io.r64 = r64 + m * n;
     io.lucsr = lucsr ? lucsr + m * nval : nullptr;
     io.norms = norms + 2 * m;
+    io.kry = kry ? kry + m * (3 * desc->krylov + 1) * n : nullptr;
     io.iters = iters + m;
     io.eta = eta + m;
     globalize(io);
@@ -1491,12 +1653,12 @@ __global__ void __launch_bounds__(1024) k_batch_same_factor(const MatDesc *__res
                                                             double *__restrict__ x, float *__restrict__ lu,
                                                             float *__restrict__ lucsr, float *__restrict__ t32,
                                                             double *__restrict__ r64, double *__restrict__ norms,
-                                                            int32_t *__restrict__ iters, double *__restrict__ eta,
-                                                            int perturb)
+                                                            double *__restrict__ kry, int32_t *__restrict__ iters,
+                                                            double *__restrict__ eta, int perturb)
 {
     __shared__ double sh[1024 / kWave];
     __shared__ __attribute__((aligned(16))) float wl[1024 / kWave][kWave];
-    const MatIO io = batch_io(desc, ax, b, x, lu, lucsr, t32, r64, norms, iters, eta);
+    const MatIO io = batch_io(desc, ax, b, x, lu, lucsr, t32, r64, norms, kry, iters, eta);
     wg_factor<kCopy>(desc, io, perturb, sh, wl[threadIdx.x / kWave]);
 }
 
@@ -1507,15 +1669,16 @@ __global__ void __launch_bounds__(1024) k_batch_same_refine(const MatDesc *__res
                                                             double *__restrict__ x, float *__restrict__ lu,
                                                             float *__restrict__ lucsr, float *__restrict__ t32,
                                                             double *__restrict__ r64, double *__restrict__ norms,
-                                                            int32_t *__restrict__ iters, double *__restrict__ eta,
-                                                            double tol, int maxit)
+                                                            double *__restrict__ kry, int32_t *__restrict__ iters,
+                                                            double *__restrict__ eta, double tol, int maxit)
 {
     __shared__ double sh[1024 / kWave];
-    const MatIO io = batch_io(desc, ax, b, x, lu, lucsr, t32, r64, norms, iters, eta);
+    __shared__ double hs[kKrylovLds];
+    const MatIO io = batch_io(desc, ax, b, x, lu, lucsr, t32, r64, norms, kry, iters, eta);
     uint32_t *flags;
     float *xs;
     batch_lds<kMode>(desc->n, io.t32, flags, xs);
-    wg_refine<kMode>(desc, io, tol, maxit, sh, xs, flags);
+    wg_refine<kMode>(desc, io, tol, maxit, sh, hs, xs, flags);
 }
 
 // Distinct patterns: workgroup g runs matrix order[g] with its own tables.
@@ -1538,13 +1701,14 @@ __global__ void __launch_bounds__(1024) k_batch_group_refine(const MatDesc *__re
                                                              const uint32_t *__restrict__ order, double tol, int maxit)
 {
     __shared__ double sh[1024 / kWave];
+    __shared__ double hs[kKrylovLds];
     const uint32_t m = order[blockIdx.x];
     MatIO io = ios[m];
     globalize(io);
     uint32_t *flags;
     float *xs;
     batch_lds<kMode>(descs[m].n, io.t32, flags, xs);
-    wg_refine<kMode>(descs + m, io, tol, maxit, sh, xs, flags);
+    wg_refine<kMode>(descs + m, io, tol, maxit, sh, hs, xs, flags);
 }
 
 // ---------------------------------------------------------------------------
@@ -1626,24 +1790,25 @@ size_t batch_lds_bytes(int mode, uint32_t n)
 // Two launches: factor (dynamic LDS fact_lds: the largest window's values), then
 // refine in sweep mode `mode` (dynamic LDS ref_lds).
 void batch_same(const MatDesc *desc, int batch, const double *ax, const double *b, double *x, float *lu,
-                float *lucsr, float *t32, double *r64, double *norms, int32_t *iters, double *eta, double tol,
-                int maxit, int perturb, int threads, int mode, size_t fact_lds, size_t ref_lds, hipStream_t st)
+                float *lucsr, float *t32, double *r64, double *norms, double *kry, int32_t *iters, double *eta,
+                double tol, int maxit, int perturb, int threads, int mode, size_t fact_lds, size_t ref_lds,
+                hipStream_t st)
 {
     if (mode == kSweepLevels)
         k_batch_same_factor<false><<<batch, threads, fact_lds, st>>>(desc, ax, b, x, lu, lucsr, t32, r64, norms,
-                                                                      iters, eta, perturb);
+                                                                      kry, iters, eta, perturb);
     else
         k_batch_same_factor<true><<<batch, threads, fact_lds, st>>>(desc, ax, b, x, lu, lucsr, t32, r64, norms,
-                                                                     iters, eta, perturb);
+                                                                     kry, iters, eta, perturb);
     if (mode == kSweepLds)
         k_batch_same_refine<kSweepLds><<<batch, threads, ref_lds, st>>>(desc, ax, b, x, lu, lucsr, t32, r64, norms,
-                                                                         iters, eta, tol, maxit);
+                                                                         kry, iters, eta, tol, maxit);
     else if (mode == kSweepFlags)
         k_batch_same_refine<kSweepFlags><<<batch, threads, ref_lds, st>>>(desc, ax, b, x, lu, lucsr, t32, r64,
-                                                                           norms, iters, eta, tol, maxit);
+                                                                           norms, kry, iters, eta, tol, maxit);
     else
         k_batch_same_refine<kSweepLevels><<<batch, threads, ref_lds, st>>>(desc, ax, b, x, lu, lucsr, t32, r64,
-                                                                            norms, iters, eta, tol, maxit);
+                                                                            norms, kry, iters, eta, tol, maxit);
 }
 
 void batch_group(const MatDesc *descs, const void *ios, const uint32_t *order, int count, double tol,

@@ -43,8 +43,9 @@ void residual(const RefineDev &R, const double *ax, const double *b, const doubl
               int transpose, int with_anorm, unsigned long long *norms, uint32_t lanes, hipStream_t st);
 size_t batch_lds_bytes(int mode, uint32_t n);
 void batch_same(const MatDesc *desc, int batch, const double *ax, const double *b, double *x, float *lu,
-                float *lucsr, float *t32, double *r64, double *norms, int32_t *iters, double *eta, double tol,
-                int maxit, int perturb, int threads, int mode, size_t fact_lds, size_t ref_lds, hipStream_t st);
+                float *lucsr, float *t32, double *r64, double *norms, double *kry, int32_t *iters, double *eta,
+                double tol, int maxit, int perturb, int threads, int mode, size_t fact_lds, size_t ref_lds,
+                hipStream_t st);
 void batch_group(const MatDesc *descs, const void *ios, const uint32_t *order, int count, double tol,
                  int maxit, int perturb, int threads, int mode, size_t fact_lds, size_t ref_lds, hipStream_t st);
 void tri_df(const TriDev &T, const float *lu, float *x, int l0, int l1, uint32_t k0, uint32_t k1, uint32_t n,
@@ -203,6 +204,7 @@ namespace {
 
 constexpr int kGridThreads = 256;
 constexpr int kDefaultFusedThreads = 1024;
+constexpr int kDefaultKrylov = 2;         // FGMRES(2) cycles after the first solve
 constexpr int kDefaultFuseUnits = 2048;   // lanes
 
 struct Segment {
@@ -253,6 +255,7 @@ struct glu_plan {
     size_t lds_cap = 0;          // dynamic LDS per workgroup (bytes)
     uint32_t df_nval = 0;        // values of the per-matrix sweep copy
     int max_windows = 1;         // LDS windows allowed (from the last level)
+    int krylov = kDefaultKrylov;  // batch refinement: FGMRES restart length (0: plain)
     int nwin = 0;                // LDS windows of the factorization
     int win_steps = 0;           // steps they cover
     uint32_t win_nslot = 0;      // the largest window's values
@@ -289,6 +292,7 @@ struct glu_plan {
     double *d_br64 = nullptr;
     float *d_blucsr = nullptr;   // sweep value copies, df_nval per matrix
     double *d_bnorms = nullptr;  // |A|_inf, |b|_inf per matrix
+    double *d_bkry = nullptr;    // Arnoldi vectors V, Z = M V, W = A Z: (3 krylov + 1) n per matrix
     int batch_cap = 0;
 
     ~glu_plan()
@@ -298,6 +302,7 @@ struct glu_plan {
         if (d_bt32) (void)hipFree(d_bt32);
         if (d_br64) (void)hipFree(d_br64);
         if (d_blucsr) (void)hipFree(d_blucsr);
+        if (d_bkry) (void)hipFree(d_bkry);
         if (d_bnorms) (void)hipFree(d_bnorms);
         for (GraphSlot *g : {&g_factor, &g_solve[0], &g_solve[1]})
             if (g->exec) (void)hipGraphExecDestroy(g->exec);
@@ -1007,6 +1012,9 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
         if (opts->solve_dataflow < -1 || opts->solve_dataflow > 1)
             return fail(GLU_ERR_ARGUMENT, "solve_dataflow must be -1, 0 or 1");
         p->solve_df_opt = opts->solve_dataflow;
+        if (opts->krylov < -1 || opts->krylov > kMaxKrylov)
+            return fail(GLU_ERR_ARGUMENT, "krylov must be in [-1, 4]");
+        if (opts->krylov) p->krylov = opts->krylov < 0 ? 0 : opts->krylov;
     }
     p->n = P.n;
     p->nnz = P.nnz;
@@ -1327,6 +1335,7 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
     md.r = p->R;
     md.n = n;
     md.nnz = P.nnz;
+    md.krylov = (uint32_t)p->krylov;
     p->d_desc = reinterpret_cast<MatDesc *>(A + o_desc);
     HIP_TRY(hipMemcpy(p->d_desc, &md, sizeof(MatDesc), hipMemcpyHostToDevice));
     p->d_norm_bits = reinterpret_cast<unsigned int *>(A + o_normbits);
@@ -1412,6 +1421,8 @@ glu_status ensure_batch(glu_plan *p, int batch)
     if (p->d_br64) (void)hipFree(p->d_br64);
     if (p->d_blucsr) (void)hipFree(p->d_blucsr);
     if (p->d_bnorms) (void)hipFree(p->d_bnorms);
+    if (p->d_bkry) (void)hipFree(p->d_bkry);
+    p->d_bkry = nullptr;
     p->d_bt32 = nullptr;
     p->d_br64 = nullptr;
     p->d_blucsr = nullptr;
@@ -1421,6 +1432,8 @@ glu_status ensure_batch(glu_plan *p, int batch)
     HIP_TRY(hipMalloc(&p->d_br64, sizeof(double) * (size_t)p->n * batch));
     HIP_TRY(hipMalloc(&p->d_blucsr, sizeof(float) * (size_t)p->df_nval * batch));
     HIP_TRY(hipMalloc(&p->d_bnorms, 2 * sizeof(double) * (size_t)batch));
+    if (p->krylov)
+        HIP_TRY(hipMalloc(&p->d_bkry, sizeof(double) * (3 * (size_t)p->krylov + 1) * p->n * (size_t)batch));
     p->batch_cap = batch;
     return GLU_OK;
 }
@@ -1639,7 +1652,7 @@ extern "C" glu_status glu_batch_solve(glu_plan *p, int batch, const double *d_ax
     const size_t cap = launch::enable_lds();
     const int mode = sweep_mode(p->n, cap);
     launch::batch_same(p->d_desc, batch, d_ax, d_b, d_x, d_lu, p->d_blucsr, p->d_bt32, p->d_br64, p->d_bnorms,
-                       d_iters, d_eta, tol, maxit, (int)(flags & 1u), p->fused_threads, mode,
+                       p->d_bkry, d_iters, d_eta, tol, maxit, (int)(flags & 1u), p->fused_threads, mode,
                        sizeof(float) * p->win_nslot, launch::batch_lds_bytes(mode, p->n),
                        static_cast<hipStream_t>(stream));
     hipError_t e = hipGetLastError();
@@ -1660,6 +1673,7 @@ struct MatIOHost {            // mirrors the device-side MatIO layout
     double *r64;
     float *lucsr;
     double *norms;
+    double *kry;
     int32_t *iters;
     double *eta;
 };
@@ -1673,13 +1687,16 @@ struct glu_group {
     double *d_r64 = nullptr;
     float *d_lucsr = nullptr;
     double *d_norms = nullptr;
+    double *d_kry = nullptr;
     std::vector<size_t> voff;   // per-matrix offset into the scratch vectors
+    std::vector<size_t> koff;   // per-matrix offset into d_kry (~0: plain refinement)
     std::vector<size_t> foff;   // per-matrix offset into d_lucsr
     int threads = kDefaultFusedThreads;
     ~glu_group()
     {
         if (d_lucsr) (void)hipFree(d_lucsr);
         if (d_norms) (void)hipFree(d_norms);
+        if (d_kry) (void)hipFree(d_kry);
         if (d_descs) (void)hipFree(d_descs);
         if (d_ios) (void)hipFree(d_ios);
         if (d_order) (void)hipFree(d_order);
@@ -1696,7 +1713,7 @@ extern "C" glu_status glu_group_create(glu_plan *const *plans, int count, glu_gr
     std::unique_ptr<glu_group> g(new glu_group());
     std::vector<MatDesc> descs(count);
     std::vector<std::pair<int64_t, uint32_t>> work(count);
-    size_t vtot = 0, ftot = 0;
+    size_t vtot = 0, ftot = 0, ktot = 0;
     for (int i = 0; i < count; ++i) {
         glu_plan *p = plans[i];
         if (!p || !p->has_refine) return fail(GLU_ERR_ARGUMENT, "every plan must come from an analysis");
@@ -1706,6 +1723,9 @@ extern "C" glu_status glu_group_create(glu_plan *const *plans, int count, glu_gr
         descs[i].r = p->R;
         descs[i].n = p->n;
         descs[i].nnz = p->nnz;
+        descs[i].krylov = (uint32_t)p->krylov;
+        g->koff.push_back(p->krylov ? ktot : ~(size_t)0);
+        ktot += (3 * (size_t)p->krylov + 1) * p->n * (p->krylov != 0);
         g->voff.push_back(vtot);
         vtot += p->n;
         g->foff.push_back(ftot);
@@ -1725,6 +1745,7 @@ extern "C" glu_status glu_group_create(glu_plan *const *plans, int count, glu_gr
     HIP_TRY(hipMalloc(&g->d_r64, sizeof(double) * vtot));
     HIP_TRY(hipMalloc(&g->d_lucsr, sizeof(float) * ftot));
     HIP_TRY(hipMalloc(&g->d_norms, 2 * sizeof(double) * count));
+    if (ktot) HIP_TRY(hipMalloc(&g->d_kry, sizeof(double) * ktot));
     HIP_TRY(hipMemcpy(g->d_descs, descs.data(), sizeof(MatDesc) * count, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(g->d_order, order.data(), sizeof(uint32_t) * count, hipMemcpyHostToDevice));
     *out = g.release();
@@ -1744,7 +1765,8 @@ extern "C" glu_status glu_group_solve(glu_group *g, const double *const *d_ax, c
     for (int i = 0; i < count; ++i) {
         if (!d_ax[i] || !d_b[i] || !d_x[i] || !d_lu[i]) return fail(GLU_ERR_ARGUMENT, "null matrix pointer");
         ios[i] = MatIOHost{d_ax[i], d_b[i], d_x[i], d_lu[i], g->d_t32 + g->voff[i], g->d_r64 + g->voff[i],
-                           g->d_lucsr + g->foff[i], g->d_norms + 2 * i, d_iters + i, d_eta + i};
+                           g->d_lucsr + g->foff[i], g->d_norms + 2 * i,
+                           g->koff[i] == ~(size_t)0 ? nullptr : g->d_kry + g->koff[i], d_iters + i, d_eta + i};
     }
     hipStream_t st = static_cast<hipStream_t>(stream);
     HIP_TRY(hipMemcpyAsync(g->d_ios, ios.data(), sizeof(MatIOHost) * count, hipMemcpyHostToDevice, st));

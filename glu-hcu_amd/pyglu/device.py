@@ -52,15 +52,17 @@ class Plan:
     narrow levels, counted from the last, are factored with their values
     staged in LDS (0 = default, 1; -1 = none).  ``solve_dataflow``: narrow
     levels of the single-matrix solves barrier-free (0 = default, 1 = on,
-    -1 = off).
+    -1 = off).  ``krylov``: refinement of the batched and grouped solves
+    after the first solve (0 = default, restarted FGMRES(2); -1 = plain
+    refinement; 1..4 = FGMRES(m)).
     """
 
     def __init__(self, source, fuse_units: int = 0, block_threads: int = 0, use_graph: bool = False,
-                 lds_windows: int = 0, solve_dataflow: int = 0):
+                 lds_windows: int = 0, solve_dataflow: int = 0, krylov: int = 0):
         _require_device()
         L = capi.lib()
         opts = capi.PlanOpts(fuse_units, block_threads, int(bool(use_graph)), int(lds_windows),
-                             int(solve_dataflow))
+                             int(solve_dataflow), int(krylov))
         h = C.c_void_p()
         if isinstance(source, Analysis):
             capi.check(L.glu_plan_create_from_analysis(source.handle, C.byref(opts), C.byref(h)),

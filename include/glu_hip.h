@@ -118,7 +118,12 @@ typedef struct glu_plan_opts {
     int32_t solve_dataflow; /* narrow levels of the single-matrix solves run
                                barrier-free in one workgroup: 0 = default (on
                                when LDS holds the ready bits), 1 = on, -1 = off */
-    int32_t reserved[3];
+    int32_t krylov;         /* refinement of glu_batch_solve/glu_group_solve after
+                               the first solve: 0 = default (2), -1 = plain
+                               (x += LU^-1 r per step), m in 1..4 = restarted
+                               flexible GMRES(m) in FP64 preconditioned by the
+                               FP32 factors (m solves per restart cycle) */
+    int32_t reserved[2];
 } glu_plan_opts;
 
 glu_status glu_plan_create(const glu_pattern *pat, const glu_plan_opts *opts, glu_plan **out);

@@ -257,3 +257,39 @@ def test_graph_replay_matches_eager(torch_mod):
         xt_g = pg.solve(lu_g, b_g, torch.empty_like(x_g), transpose=True)
         torch.cuda.synchronize()
         assert torch.equal(xt_g.view(torch.int32), xt_e.view(torch.int32)), rep
+
+
+def test_batch_krylov_modes(torch_mod):
+    """Every refinement mode of the batch (plain, FGMRES(1..4)) reaches the
+    backward-error target on the same matrices; the factors do not depend on
+    the mode, and the FGMRES results are bit-reproducible (fixed-order sums)."""
+    from pyglu import mna
+    from pyglu.device import Plan
+    torch = torch_mod
+    data, indices, indptr, n, an, plan, pat = _setup("mna2k")
+    B = 5
+    ax = np.stack([mna.perturb_values(data, 3100 + b) for b in range(B)])
+    bb = np.stack([mna.rhs(n, 3600 + b) for b in range(B)])
+    axd, bd = torch.from_numpy(ax).cuda(), torch.from_numpy(bb).cuda()
+    lus = {}
+    for kry in (-1, 1, 2, 3, 4):
+        pk = Plan(an, krylov=kry)
+        x, lu, iters, eta = pk.batch_solve(axd, bd, tol=1e-10, maxit=20)
+        x2, _, iters2, _ = pk.batch_solve(axd, bd, tol=1e-10, maxit=20)
+        torch.cuda.synchronize()
+        print(f"  krylov={kry}: iters {iters.cpu().numpy().tolist()}", flush=True)
+        assert np.all(eta.cpu().numpy() <= 1e-10)
+        assert torch.equal(x.view(torch.int64), x2.view(torch.int64)) and torch.equal(iters, iters2)
+        for b in range(B):
+            assert backward_error(ax[b], indices, indptr, n, x[b].cpu().numpy(), bb[b]) <= 2e-10
+        lus[kry] = lu.cpu().numpy().view(np.uint32)
+    for kry in lus:
+        assert np.array_equal(lus[kry], lus[-1])
+
+
+def test_plan_rejects_bad_krylov(torch_mod):
+    from pyglu.device import Plan
+    data, indices, indptr, n, an, plan, pat = _setup("small")
+    for bad in (-2, 5):
+        with pytest.raises(RuntimeError):
+            Plan(an, krylov=bad)
