@@ -185,30 +185,35 @@ __device__ __forceinline__ void factor_merge(const FactorDev &F, float *__restri
 
 // acc - p[0] - p[1] - ... - p[cnt-1] (desc: from p[cnt-1] down to p[0]), each
 // subtraction rounded, for a wave's 64-float LDS slice p (256-byte
-// aligned).  A full slice is read as sixteen 16-byte loads issued together,
-// so only the subtraction chain is sequential.
+// aligned).  A full slice is read as two halves of eight 16-byte loads
+// issued together, so only the subtraction chain is sequential.
 __device__ __forceinline__ float serial_sub(float acc, const float *p, uint32_t cnt, bool desc)
 {
     if (cnt == kWave) {
         const float4 *p4 = reinterpret_cast<const float4 *>(p);
-        float4 v[kWave / 4];
+        constexpr uint32_t kHalf = kWave / 8;
 #pragma unroll
-        for (uint32_t k = 0; k < kWave / 4; ++k) v[k] = p4[k];
-        if (!desc) {
+        for (uint32_t h = 0; h < 2; ++h) {
+            float4 v[kHalf];
+            const uint32_t base = desc ? (1 - h) * kHalf : h * kHalf;
 #pragma unroll
-            for (uint32_t k = 0; k < kWave / 4; ++k) {
-                acc = acc - v[k].x;
-                acc = acc - v[k].y;
-                acc = acc - v[k].z;
-                acc = acc - v[k].w;
-            }
-        } else {
+            for (uint32_t k = 0; k < kHalf; ++k) v[k] = p4[base + k];
+            if (!desc) {
 #pragma unroll
-            for (uint32_t k = kWave / 4; k > 0; --k) {
-                acc = acc - v[k - 1].w;
-                acc = acc - v[k - 1].z;
-                acc = acc - v[k - 1].y;
-                acc = acc - v[k - 1].x;
+                for (uint32_t k = 0; k < kHalf; ++k) {
+                    acc = acc - v[k].x;
+                    acc = acc - v[k].y;
+                    acc = acc - v[k].z;
+                    acc = acc - v[k].w;
+                }
+            } else {
+#pragma unroll
+                for (uint32_t k = kHalf; k > 0; --k) {
+                    acc = acc - v[k - 1].w;
+                    acc = acc - v[k - 1].z;
+                    acc = acc - v[k - 1].y;
+                    acc = acc - v[k - 1].x;
+                }
             }
         }
         return acc;
