@@ -11,18 +11,18 @@
 // times a slice of U_k); targets come from a precomputed scatter map instead
 // of a dense n-length scratch column; the L(:,k) /= U(k,k) write-back of a
 // level is deferred to the next level's launch so a level costs one barrier;
-// updates are fire-and-forget FP32 atomics (the reference's own update
-// operator, val += -(l*u)) where an entry gets one update per level, and
-// lane-owned sequential merges where several columns of a level update the
-// same entry, so the factors are deterministic and equal a sequential pass
+// an entry that gets one update per level takes it as a plain load + store
+// of the reference's own update operator, val += -(l*u) (no other item of
+// the level reads or writes it, so no atomic is needed), and lane-owned
+// sequential merges take entries that several columns of a level update,
+// so the factors are deterministic and equal a sequential pass
 // over each level's columns in increasing order; solves pull along CSR/CSC
 // rows so each unknown is owned by one lane and accumulates in the
 // reference's order.
 //
 // Build flags (see the package Makefile): -ffp-contract=off so products are
 // rounded before they are subtracted, as the reference's FP32 kernels and
-// host solve do; packed FP32 disabled; -munsafe-fp-atomics so FP32 atomic
-// adds lower to global_atomic_add_f32.
+// host solve do; packed FP32 disabled.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -50,12 +50,6 @@ constexpr float kPertFactor = 3.45e-4f;   // numeric.cu:503
 // ---------------------------------------------------------------------------
 // small helpers
 // ---------------------------------------------------------------------------
-
-template <int Scope>
-__device__ __forceinline__ void atomic_sub(float *p, float v)
-{
-    __hip_atomic_fetch_add(p, -v, __ATOMIC_RELAXED, Scope);
-}
 
 __device__ __forceinline__ uint32_t pow2_ceil(uint32_t x)
 {
@@ -100,8 +94,11 @@ __device__ __forceinline__ float gesp(float d, float pert, int perturb)
 // pivot column k of the level and up to four entries u = A(k,j) of its U
 // list, forms l = A(i,k) / A(k,k) (A(k,k) GESP-perturbed) and applies
 //   A(i,j) += -(l * u)
-// Each target here receives exactly this one update in this level, so the
-// fire-and-forget atomic add gives an order-independent result.
+// Each target here receives exactly this one update in this level and no
+// other item of the level reads it (a target A(i,j) of pivot column k lies
+// in a column of a later level), so a plain load + store is race-free and
+// equals the sequential pass.  Plain stores beat L2 atomics here: the
+// target's line is usually already in flight with the task's other values.
 //
 // CTab holds a task's table data for this lane (independent of the values,
 // so it can be loaded ahead): the L and pivot positions, the update count,
@@ -125,25 +122,25 @@ __device__ __forceinline__ void ctab_load(const FactorDev &F, uint32_t t, uint32
     }
 }
 
-template <int Scope>
 __device__ __forceinline__ void ctab_run(const FactorDev &F, float *__restrict__ val, const CTab &c, float pert,
                                          int perturb)
 {
     if (c.cnt == 0) return;
     const float d = val[c.dpos], a = val[c.lpos];
-    float uv[4];
+    float uv[4], tv[4];
 #pragma unroll
-    for (uint32_t u = 0; u < 4; ++u)
+    for (uint32_t u = 0; u < 4; ++u) {
         if (u < c.cnt) uv[u] = val[c.up[u]];
+        if (u < c.cnt && c.tt[u] != ~0u) tv[u] = val[c.tt[u]];
+    }
     const float l = a / gesp(d, pert, perturb);
 #pragma unroll
     for (uint32_t u = 0; u < 4; ++u)
-        if (u < c.cnt && c.tt[u] != ~0u) atomic_sub<Scope>(val + c.tt[u], l * uv[u]);
+        if (u < c.cnt && c.tt[u] != ~0u) val[c.tt[u]] = tv[u] + (-(l * uv[u]));
 }
 
 // Column tasks [c0, c0 + nc) of one level for a group of cooperating
 // waves (w of nw), one at a time.
-template <int Scope>
 __device__ __forceinline__ void factor_ctasks(const FactorDev &F, float *__restrict__ val, uint32_t c0, uint32_t nc,
                                               uint32_t tid, uint32_t nthreads, float pert, int perturb)
 {
@@ -151,7 +148,7 @@ __device__ __forceinline__ void factor_ctasks(const FactorDev &F, float *__restr
     for (uint32_t t = c0 + w; t < c0 + nc; t += nw) {
         CTab c;
         ctab_load(F, t, lane, c);
-        ctab_run<Scope>(F, val, c, pert, perturb);
+        ctab_run(F, val, c, pert, perturb);
     }
 }
 
@@ -292,13 +289,12 @@ __device__ __forceinline__ LevelSpan level_span(const FactorDev &F, int lvl)
 // workgroup or the whole grid), `tid` = this lane's index in the group:
 // long merges and column tasks one wave each, short merges and divisions
 // one lane each.  lds: this wave's 64-float slice.
-template <int Scope>
 __device__ __forceinline__ void factor_level(const FactorDev &F, float *__restrict__ val, const LevelSpan &s,
                                              uint32_t tid, uint32_t nthreads, float pert, int perturb, float *lds)
 {
     const uint32_t lane = tid & (kWave - 1), w = tid / kWave, nw = nthreads / kWave;
     for (uint32_t e = w; e < s.ml; e += nw) factor_merge_wave(F, val, s.m0 + e, pert, perturb, lane, lds);
-    factor_ctasks<Scope>(F, val, s.c0, s.nc, tid, nthreads, pert, perturb);
+    factor_ctasks(F, val, s.c0, s.nc, tid, nthreads, pert, perturb);
     for (uint32_t t = s.ml + tid; t < s.nm; t += nthreads) factor_merge(F, val, s.m0 + t, pert, perturb);
     for (uint32_t t = tid; t < s.nd; t += nthreads) factor_divide(val, F.dvd[s.d0 + t], pert, perturb);
 }
@@ -308,7 +304,6 @@ __device__ __forceinline__ void factor_level(const FactorDev &F, float *__restri
 // (the next one of the level, or its first one of the next level, issued
 // before the barrier) while it runs the current one, so a task waits only
 // for its values.  Same items and order as factor_level.
-template <int Scope>
 __device__ __forceinline__ void factor_levels_wg(const FactorDev &F, float *__restrict__ val, int l0, int l1,
                                                  float pert, int perturb, float *lds)
 {
@@ -328,7 +323,7 @@ __device__ __forceinline__ void factor_levels_wg(const FactorDev &F, float *__re
             const bool more = t + nw < s.c0 + s.nc;
             CTab nxt;
             if (more) ctab_load(F, t + nw, lane, nxt);
-            ctab_run<Scope>(F, val, cur, pert, perturb);
+            ctab_run(F, val, cur, pert, perturb);
             if (more) cur = nxt;
         }
         for (uint32_t t = s.ml + tid; t < s.nm; t += nt) factor_merge(F, val, s.m0 + t, pert, perturb);
@@ -374,7 +369,7 @@ __global__ void __launch_bounds__(256) k_factor_grid(FactorDev F, float *__restr
     __shared__ __attribute__((aligned(16))) float lds[256 / kWave][kWave];
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     const float pert = perturb ? *d_pert : 0.f;
-    factor_level<__HIP_MEMORY_SCOPE_AGENT>(F, val, level_span(F, lvl), g, gridDim.x * blockDim.x, pert, perturb,
+    factor_level(F, val, level_span(F, lvl), g, gridDim.x * blockDim.x, pert, perturb,
                                            lds[threadIdx.x / kWave]);
 }
 
@@ -384,7 +379,7 @@ __global__ void __launch_bounds__(1024) k_factor_fused(FactorDev F, float *__res
 {
     __shared__ __attribute__((aligned(16))) float lds[1024 / kWave][kWave];
     const float pert = perturb ? *d_pert : 0.f;
-    factor_levels_wg<__HIP_MEMORY_SCOPE_AGENT>(F, val, l0, l1, pert, perturb, lds[threadIdx.x / kWave]);
+    factor_levels_wg(F, val, l0, l1, pert, perturb, lds[threadIdx.x / kWave]);
 }
 
 // A window's view of the tables: the same level spans, items that address
@@ -423,7 +418,7 @@ __device__ __forceinline__ void factor_window(const FactorDev &F, const WinDev &
     for (uint32_t s = tid; s < W.nslot; s += nt) slots[s] = val[W.pos[s]];
     __syncthreads();
     const FactorDev T = window_view(F, W);
-    factor_levels_wg<__HIP_MEMORY_SCOPE_WORKGROUP>(T, slots, W.l0, W.l1, pert, perturb, wlds);
+    factor_levels_wg(T, slots, W.l0, W.l1, pert, perturb, wlds);
     for (uint32_t s = tid; s < W.nslot; s += nt) val[W.pos[s]] = slots[s];
     __syncthreads();
 }
@@ -436,11 +431,11 @@ __device__ __forceinline__ void factor_all_wg(const FactorDev &F, float *__restr
     int l = 0;
     for (int w = 0; w < F.nwin; ++w) {
         const WinDev W = load_window(F, w);
-        factor_levels_wg<__HIP_MEMORY_SCOPE_WORKGROUP>(F, val, l, W.l0, pert, perturb, wlds);
+        factor_levels_wg(F, val, l, W.l0, pert, perturb, wlds);
         factor_window(F, W, val, slots, pert, perturb, wlds);
         l = W.l1;
     }
-    factor_levels_wg<__HIP_MEMORY_SCOPE_WORKGROUP>(F, val, l, F.num_lev + 1, pert, perturb, wlds);
+    factor_levels_wg(F, val, l, F.num_lev + 1, pert, perturb, wlds);
 }
 
 extern __shared__ float g_lds_vec[];   // dynamic LDS of the kernels below
