@@ -982,7 +982,7 @@ struct MatIO {
     float *t32;         // n scratch
     double *r64;        // n scratch
     float *lucsr;       // df_nval scratch: the sweeps' value copy
-    double *norms;      // 2 scratch: |A|_inf, |b|_inf (factor -> refine kernel)
+    double *norms;      // 2 scratch: [1] = |b|_inf (factor -> refine kernel), [0] unused
     double *kry;        // (3 krylov + 1) n scratch: Arnoldi vectors V, Z = M V, W = A Z (null: plain refinement)
     int32_t *iters;
     double *eta;
@@ -1302,10 +1302,10 @@ __device__ __forceinline__ void wg_factor(const MatDesc *Md, const MatIO &io, in
 
     // 1. values: zero the pattern, then scatter the scaled entries of A
     //    (lu[a_dst[k]] = float((ax[k] * rs) * cs), the refactor order);
-    //    |A|_inf and |b|_inf
+    //    |b|_inf (|A|_inf comes with the refinement's first product with A)
     for (uint32_t p = tid; p < nnz; p += nt) io.lu[p] = 0.f;
     __syncthreads();
-    double anorm, bnorm;
+    double bnorm;
     {
         RefineDev R = Md->r;
         globalize(R);
@@ -1328,14 +1328,12 @@ __device__ __forceinline__ void wg_factor(const MatDesc *Md, const MatIO &io, in
             for (int u = 0; u < U; ++u)
                 if (k0 + u * nt < na) io.lu[dst[u]] = (float)((a[u] * s1[u]) * s2[u]);
         }
-        double amax = 0.0, bmax = 0.0;
-        wg_rows_of_a(R, io.ax, nullptr, n, [&](uint32_t, double, double asum) { amax = nmax(amax, asum); });
+        double bmax = 0.0;
         for (uint32_t c = tid; c < n; c += nt) {
             bmax = nmax(bmax, fabs(io.b[c]));
             io.x[c] = 0.0;
             io.r64[c] = io.b[c];
         }
-        anorm = wg_max(amax, sh);
         bnorm = wg_max(bmax, sh);
     }
 
@@ -1360,26 +1358,29 @@ __device__ __forceinline__ void wg_factor(const MatDesc *Md, const MatIO &io, in
         factor_all_wg(F, io.lu, g_lds_vec, pert, perturb, wlds);
     }
 
-    // 4. the factors copied into the sweeps' task-contiguous layout
+    // 4. the factors copied into the sweeps' task-contiguous layout, read in
+    //    CSC order (contiguous) and stored to their slots
     if (kCopy) {
-        const uint32_t *vpos = glb(Md->s.df_vpos);
-        const uint32_t nval = Md->s.df_nval;
+        const uint32_t *dst = glb(Md->s.df_dst);
         constexpr int U = 8;
-        for (uint32_t q0 = tid; q0 < nval; q0 += U * nt) {
-            uint32_t src[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) src[u] = q0 + u * nt < nval ? vpos[q0 + u * nt] : ~0u;
+        for (uint32_t p0 = tid; p0 < nnz; p0 += U * nt) {
+            uint32_t q[U];
             float v[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) v[u] = src[u] != ~0u ? io.lu[src[u]] : 0.f;
+            for (int u = 0; u < U; ++u) {
+                const uint32_t p = p0 + u * nt;
+                q[u] = p < nnz ? dst[p] : ~0u;
+                v[u] = p < nnz ? io.lu[p] : 0.f;
+            }
 #pragma unroll
             for (int u = 0; u < U; ++u)
-                if (q0 + u * nt < nval) io.lucsr[q0 + u * nt] = v[u];
+                if (q[u] != ~0u) io.lucsr[q[u]] = v[u];
         }
+        const uint2 *extra = glb(Md->s.df_extra);
+        for (uint32_t e = tid; e < Md->s.df_nextra; e += nt) io.lucsr[extra[e].x] = io.lu[extra[e].y];
         __syncthreads();
     }
     if (tid == 0) {
-        io.norms[0] = anorm;
         io.norms[1] = bnorm;
     }
 }
@@ -1443,20 +1444,24 @@ __device__ __forceinline__ void wg_precond(const MatDesc *Md, const MatIO &io, c
 
 // r = b - A x in FP64 (into io.r64); returns the backward error
 // eta = |r|_inf / (|A|_inf |x|_inf + |b|_inf) to every thread.
-__device__ __forceinline__ double wg_residual(const MatDesc *Md, const MatIO &io, double anorm, double bnorm,
+// anorm < 0: |A|_inf is taken from this product's row sums (and returned
+// in anorm).
+__device__ __forceinline__ double wg_residual(const MatDesc *Md, const MatIO &io, double &anorm, double bnorm,
                                               double *sh)
 {
     const uint32_t n = Md->n;
     const uint32_t tid = threadIdx.x, nt = blockDim.x;
-    double rmax = 0.0, xmax = 0.0;
+    double rmax = 0.0, xmax = 0.0, amax = 0.0;
     RefineDev R = Md->r;
     globalize(R);
-    wg_rows_of_a(R, io.ax, io.x, n, [&](uint32_t i, double dot, double) {
+    wg_rows_of_a(R, io.ax, io.x, n, [&](uint32_t i, double dot, double asum) {
         const double ri = io.b[i] - dot;
         io.r64[i] = ri;
         rmax = nmax(rmax, fabs(ri));
+        amax = nmax(amax, asum);
     });
     for (uint32_t i = tid; i < n; i += nt) xmax = nmax(xmax, fabs(io.x[i]));
+    if (anorm < 0.0) anorm = wg_max(amax, sh);
     const double rn = wg_max(rmax, sh);
     const double xn = wg_max(xmax, sh);
     return rn / (anorm * xn + bnorm);
@@ -1494,7 +1499,8 @@ __device__ __forceinline__ void wg_refine(const MatDesc *Md, const MatIO &io, do
 {
     const uint32_t n = Md->n;
     const uint32_t tid = threadIdx.x, nt = blockDim.x;
-    const double anorm = io.norms[0], bnorm = io.norms[1];
+    const double bnorm = io.norms[1];
+    double anorm = -1.0;   // |A|_inf: from the first product with A
     const int m = io.kry ? (int)Md->krylov : 0;
     const uint32_t *rpi = glb(Md->s.row_perm_inv);
     const double *rs = glb(Md->s.row_scale);
@@ -1539,10 +1545,13 @@ __device__ __forceinline__ void wg_refine(const MatDesc *Md, const MatIO &io, do
                 {
                     RefineDev R = Md->r;
                     globalize(R);
-                    wg_rows_of_a(R, io.ax, zj, n, [&](uint32_t i, double dot, double) {
+                    double amax = 0.0;
+                    wg_rows_of_a(R, io.ax, zj, n, [&](uint32_t i, double dot, double asum) {
                         wj[i] = dot;
                         vn[i] = dot;
+                        amax = nmax(amax, asum);
                     });
+                    if (anorm < 0.0) anorm = wg_max(amax, sh);
                 }
                 __syncthreads();
                 for (int q = 0; q <= j; ++q) {

@@ -254,6 +254,7 @@ struct glu_plan {
     int solve_df_opt = 0;        // glu_plan_opts.solve_dataflow
     size_t lds_cap = 0;          // dynamic LDS per workgroup (bytes)
     uint32_t df_nval = 0;        // values of the per-matrix sweep copy
+    uint32_t df_nextra = 0;      // its positions with a second slot
     int max_windows = 1;         // LDS windows allowed (from the last level)
     int krylov = kDefaultKrylov;  // batch refinement: FGMRES restart length (0: plain)
     int nwin = 0;                // LDS windows of the factorization
@@ -1074,6 +1075,19 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
     df_vpos.insert(df_vpos.end(), dup.pvpos.begin(), dup.pvpos.end());
     df_vpos.resize(df_vpos.size() + 64, UINT32_MAX);   // pivot reads of the last positions' lanes
     p->df_nval = (uint32_t)df_vpos.size();
+    // the copy runs in CSC order (contiguous reads, scattered stores): the
+    // slot of each position, plus pairs for positions with a second slot;
+    // padding slots are zeroed once when the copies are allocated
+    std::vector<uint32_t> df_dst(P.nnz, UINT32_MAX);
+    std::vector<uint2> df_extra;
+    for (uint32_t q = 0; q < p->df_nval; ++q) {
+        const uint32_t pos = df_vpos[q];
+        if (pos == UINT32_MAX) continue;
+        if (df_dst[pos] == UINT32_MAX) df_dst[pos] = q;
+        else df_extra.push_back(make_uint2(q, pos));
+    }
+    if (df_extra.empty()) df_extra.push_back(make_uint2(UINT32_MAX, 0u));   // keeps the table non-empty
+    p->df_nextra = (uint32_t)(df_extra.size() - (df_extra[0].x == UINT32_MAX));
 
     // Refinement / refresh tables from the analysis.
     std::vector<uint32_t> a_rptr, a_cidx, a_src, lu_src, lu_col, a_rows, a_cols;
@@ -1169,7 +1183,8 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
     const size_t o_wdev = U.reserve(sizeof(WinDev) * std::max<size_t>(wins.size(), 1));
     const size_t o_dlo_task = U.put(dlo.task), o_dlo_rl = U.put(dlo.rl), o_dlo_idx = U.put(dlo.idx);
     const size_t o_dup_task = U.put(dup.task), o_dup_rl = U.put(dup.rl), o_dup_idx = U.put(dup.idx);
-    const size_t o_dfvpos = U.put(df_vpos);
+    const size_t o_dfdst = U.put(df_dst);
+    const size_t o_dfextra = U.put(df_extra);
     size_t o_trows[4], o_tptr[4], o_tlong[4], o_ttask[4];
     for (int s = 0; s < 4; ++s) {
         o_trows[s] = U.put(p->tri[s].rows);
@@ -1299,8 +1314,10 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
     };
     S.lo_df = df(dlo, o_dlo_task, o_dlo_rl, o_dlo_idx, 0, 0);
     S.up_df = df(dup, o_dup_task, o_dup_rl, o_dup_idx, 1, (uint32_t)dlo.idx.size());
-    S.df_vpos = u32p(o_dfvpos);
+    S.df_dst = u32p(o_dfdst);
+    S.df_extra = reinterpret_cast<const uint2 *>(A + o_dfextra);
     S.df_nval = p->df_nval;
+    S.df_nextra = p->df_nextra;
     p->tri_dataflow = p->solve_df_opt >= 0 && p->lds_cap >= 4 * (size_t)((n + 31) / 32);
     if (an) {
         S.row_perm = u32p(o_rp);
@@ -1431,6 +1448,7 @@ glu_status ensure_batch(glu_plan *p, int batch)
     HIP_TRY(hipMalloc(&p->d_bt32, sizeof(float) * (size_t)p->n * batch));
     HIP_TRY(hipMalloc(&p->d_br64, sizeof(double) * (size_t)p->n * batch));
     HIP_TRY(hipMalloc(&p->d_blucsr, sizeof(float) * (size_t)p->df_nval * batch));
+    HIP_TRY(hipMemset(p->d_blucsr, 0, sizeof(float) * (size_t)p->df_nval * batch));   // padding slots
     HIP_TRY(hipMalloc(&p->d_bnorms, 2 * sizeof(double) * (size_t)batch));
     if (p->krylov)
         HIP_TRY(hipMalloc(&p->d_bkry, sizeof(double) * (3 * (size_t)p->krylov + 1) * p->n * (size_t)batch));
@@ -1744,6 +1762,7 @@ extern "C" glu_status glu_group_create(glu_plan *const *plans, int count, glu_gr
     HIP_TRY(hipMalloc(&g->d_t32, sizeof(float) * vtot));
     HIP_TRY(hipMalloc(&g->d_r64, sizeof(double) * vtot));
     HIP_TRY(hipMalloc(&g->d_lucsr, sizeof(float) * ftot));
+    HIP_TRY(hipMemset(g->d_lucsr, 0, sizeof(float) * ftot));   // padding slots
     HIP_TRY(hipMalloc(&g->d_norms, 2 * sizeof(double) * count));
     if (ktot) HIP_TRY(hipMalloc(&g->d_kry, sizeof(double) * ktot));
     HIP_TRY(hipMemcpy(g->d_descs, descs.data(), sizeof(MatDesc) * count, hipMemcpyHostToDevice));
