@@ -2,15 +2,17 @@
 """Benchmark: batched same-pattern refactorization + FP64 refinement.
 
 Workload: 256 MNA matrices sharing one n=20,000 (~112k nnz) pattern, values
-= base * (1 + U[-0.1, 0.1]) per entry, one RHS each; a step is one launch
-that refreshes the values into the LU pattern (frozen MC64 scales and
-permutation), factors in FP32, solves in FP32 and refines in FP64 until the
-normwise backward error is <= 1e-10.  Inputs are resident on the GPU before
-the timed region.
+= base * (1 + U[-0.1, 0.1]) per entry, one RHS each; a step is one batched
+solve (two dependent launches, one workgroup per matrix): refresh the values
+into the LU pattern (frozen MC64 scales and permutation), factor in FP32,
+then solve in FP32 and refine in FP64 (FGMRES preconditioned by the FP32
+factors) until the normwise backward error is <= 1e-10.  Inputs are
+resident on the GPU before the timed region.
 
 Reported beside it (same JSON line):
 * the rate of looping the plain FP32 drop-in path (single-matrix
   level-scheduled factor + refinement) over the same matrices;
+* the batched refactor alone (refresh + factor, one launch);
 * the batch rate including the host-to-device copy of values and RHS;
 * the plain FP32 factor + solve of the n=4,960 MNA matrix;
 * the single-matrix numeric refactor time of the n=500,000 matrix;
@@ -317,6 +319,10 @@ def main():
     log(f"[bench] plain loop: {loop_rate:.1f} refactors/s (max eta {loop_eta:.2e}); "
         f"single factor {single_factor_ms:.3f} ms; batch/loop = {value / loop_rate:.1f}x")
 
+    # ---------------- batched refactor alone -----------------------------
+    fb = _events(lambda: plan.batch_factor(ax, lu=lu), 11)
+    log(f"[bench] batched refactor (refresh + factor, no solve): {fb:.3f} ms -> {B / fb * 1e3:.1f} refactors/s")
+
     # ---------------- roofline of the batch kernel ----------------------
     per_mat = [alg_bytes_factor(an.nnz_lu, n, B) + int(it) * alg_bytes_refine(data.size, an.nnz_lu, n, B)
                for it in iters_h]
@@ -409,12 +415,14 @@ def main():
                             "single_factor_ms": round(single_factor_ms, 4),
                             "speedup": round(value / loop_rate, 2)},
         "pcie_inclusive": {"value": round(pcie_rate, 2), "unit": "refactors/s"},
+        "batch_refactor_only": {"value": round(B / fb * 1e3, 2), "unit": "refactors/s", "ms": round(fb, 4)},
         "small_factor_solve": small,
         "large_refactor": large,
         "uneven_group": uneven,
         "autograd": grad,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                     "kernel": "k_batch_same_factor + k_batch_same_refine (one batch step)",
                      "kernel_ms": round(kernel_ms, 4), "alg_bytes": round(traffic_alg)},
         "cpu_baseline": cpu,
     }

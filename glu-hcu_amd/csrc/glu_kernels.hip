@@ -1329,11 +1329,12 @@ __device__ __forceinline__ void wg_factor(const MatDesc *Md, const MatIO &io, in
                 if (k0 + u * nt < na) io.lu[dst[u]] = (float)((a[u] * s1[u]) * s2[u]);
         }
         double bmax = 0.0;
-        for (uint32_t c = tid; c < n; c += nt) {
-            bmax = nmax(bmax, fabs(io.b[c]));
-            io.x[c] = 0.0;
-            io.r64[c] = io.b[c];
-        }
+        if (io.b)   // null: refactor only (k_batch_factor)
+            for (uint32_t c = tid; c < n; c += nt) {
+                bmax = nmax(bmax, fabs(io.b[c]));
+                io.x[c] = 0.0;
+                io.r64[c] = io.b[c];
+            }
         bnorm = wg_max(bmax, sh);
     }
 
@@ -1380,9 +1381,7 @@ __device__ __forceinline__ void wg_factor(const MatDesc *Md, const MatIO &io, in
         for (uint32_t e = tid; e < Md->s.df_nextra; e += nt) io.lucsr[extra[e].x] = io.lu[extra[e].y];
         __syncthreads();
     }
-    if (tid == 0) {
-        io.norms[1] = bnorm;
-    }
+    if (tid == 0 && io.norms) io.norms[1] = bnorm;
 }
 
 // Workgroup sum in a fixed order (deterministic): each wave's butterfly,
@@ -1671,6 +1670,21 @@ __global__ void __launch_bounds__(1024) k_batch_same_factor(const MatDesc *__res
     wg_factor<kCopy>(desc, io, perturb, sh, wl[threadIdx.x / kWave]);
 }
 
+// Refresh + factor only (glu_batch_factor): matrix blockIdx.x of a
+// same-pattern batch, values ax + b nnz_a into lu + b nnz.  Dynamic LDS: the
+// largest LDS window's values.
+__global__ void __launch_bounds__(1024) k_batch_factor(const MatDesc *__restrict__ desc,
+                                                       const double *__restrict__ ax, float *__restrict__ lu,
+                                                       int perturb)
+{
+    __shared__ double sh[1024 / kWave];
+    __shared__ __attribute__((aligned(16))) float wl[1024 / kWave][kWave];
+    MatIO io{};
+    io.ax = glb(ax + (uint64_t)blockIdx.x * desc->r.nnz_a);
+    io.lu = glb(lu + (uint64_t)blockIdx.x * desc->nnz);
+    wg_factor<false>(desc, io, perturb, sh, wl[threadIdx.x / kWave]);
+}
+
 // Dynamic LDS: the sweeps' ready bits (and correction vector).
 template <int kMode>
 __global__ void __launch_bounds__(1024) k_batch_same_refine(const MatDesc *__restrict__ desc,
@@ -1820,6 +1834,12 @@ void batch_same(const MatDesc *desc, int batch, const double *ax, const double *
                                                                             norms, kry, iters, eta, tol, maxit);
 }
 
+void batch_factor(const MatDesc *desc, int batch, const double *ax, float *lu, int perturb, int threads,
+                  size_t fact_lds, hipStream_t st)
+{
+    k_batch_factor<<<batch, threads, fact_lds, st>>>(desc, ax, lu, perturb);
+}
+
 void batch_group(const MatDesc *descs, const void *ios, const uint32_t *order, int count, double tol,
                  int maxit, int perturb, int threads, int mode, size_t fact_lds, size_t ref_lds, hipStream_t st)
 {
@@ -1862,6 +1882,7 @@ size_t enable_lds()
     const void *fns[] = {
         reinterpret_cast<const void *>(&k_batch_same_factor<true>),
         reinterpret_cast<const void *>(&k_batch_same_factor<false>),
+        reinterpret_cast<const void *>(&k_batch_factor),
         reinterpret_cast<const void *>(&k_batch_same_refine<kSweepLds>),
         reinterpret_cast<const void *>(&k_batch_same_refine<kSweepFlags>),
         reinterpret_cast<const void *>(&k_batch_same_refine<kSweepLevels>),

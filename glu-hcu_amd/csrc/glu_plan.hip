@@ -46,6 +46,8 @@ void batch_same(const MatDesc *desc, int batch, const double *ax, const double *
                 float *lucsr, float *t32, double *r64, double *norms, double *kry, int32_t *iters, double *eta,
                 double tol, int maxit, int perturb, int threads, int mode, size_t fact_lds, size_t ref_lds,
                 hipStream_t st);
+void batch_factor(const MatDesc *desc, int batch, const double *ax, float *lu, int perturb, int threads,
+                  size_t fact_lds, hipStream_t st);
 void batch_group(const MatDesc *descs, const void *ios, const uint32_t *order, int count, double tol,
                  int maxit, int perturb, int threads, int mode, size_t fact_lds, size_t ref_lds, hipStream_t st);
 void tri_df(const TriDev &T, const float *lu, float *x, int l0, int l1, uint32_t k0, uint32_t k1, uint32_t n,
@@ -1675,6 +1677,19 @@ extern "C" glu_status glu_batch_solve(glu_plan *p, int batch, const double *d_ax
                        static_cast<hipStream_t>(stream));
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(GLU_ERR_DEVICE, std::string("batch launch: ") + hipGetErrorString(e));
+    return GLU_OK;
+}
+
+extern "C" glu_status glu_batch_factor(glu_plan *p, int batch, const double *d_ax, float *d_lu, unsigned flags,
+                                       void *stream)
+{
+    if (!p || batch <= 0 || !d_ax || !d_lu) return fail(GLU_ERR_ARGUMENT, "bad argument");
+    if (!p->has_refine) return fail(GLU_ERR_ARGUMENT, "plan has no analysis");
+    (void)launch::enable_lds();
+    launch::batch_factor(p->d_desc, batch, d_ax, d_lu, (int)(flags & 1u), p->fused_threads,
+                         sizeof(float) * p->win_nslot, static_cast<hipStream_t>(stream));
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(GLU_ERR_DEVICE, std::string("batch factor launch: ") + hipGetErrorString(e));
     return GLU_OK;
 }
 

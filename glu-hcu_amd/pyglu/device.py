@@ -169,6 +169,21 @@ class Plan:
                                               _stream()), "glu_batch_solve")
         return x, lu, iters, eta
 
+    def batch_factor(self, ax, lu=None, perturb=False):
+        """One launch: refresh + factor per matrix (no solve).
+
+        ax: (B, nnz_a) float64.  Returns lu (B, nnz_lu) float32, bit-identical
+        to refresh + factor of each matrix.
+        """
+        B = ax.shape[0]
+        _check_dev(ax, torch.float64, B * self.nnz_a, "ax")
+        if lu is None:
+            lu = torch.empty(B, self.nnz_lu, dtype=torch.float32, device=ax.device)
+        _check_dev(lu, torch.float32, B * self.nnz_lu, "lu")
+        capi.check(capi.lib().glu_batch_factor(self._h, B, _dptr(ax), _dptr(lu), int(bool(perturb)), _stream()),
+                   "glu_batch_factor")
+        return lu
+
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
             capi.lib().glu_plan_destroy(self._h)

@@ -21,7 +21,7 @@
  *       glu_plan_create_from_analysis + glu_factor + glu_solve
  *   No counterpart in the reference (the refactor semantics follow NICSLU's
  *   _I_NicsLU_MC64ScaleForRefact, src/nicslu/source/nicslu_mc64scale.c:58-85):
- *       glu_refresh, glu_refine, glu_adjoint_values, glu_batch_solve,
+ *       glu_refresh, glu_refine, glu_adjoint_values, glu_batch_solve, glu_batch_factor,
  *       glu_group_*.
  */
 #ifndef GLU_HIP_H
@@ -184,6 +184,16 @@ glu_status glu_adjoint_values(glu_plan *p, const double *d_lam, const double *d_
 glu_status glu_batch_solve(glu_plan *p, int batch, const double *d_ax, const double *d_b,
                            double *d_x, float *d_lu, double tol, int maxit, int32_t *d_iters,
                            double *d_eta, unsigned flags, void *stream);
+
+/* Same-pattern batched refactor, one launch: for every b < batch, refresh
+ * the values d_ax + b*nnz_a into d_lu + b*nnz_lu (frozen scaling, as
+ * glu_refresh) and factor them (as glu_factor: bit-identical factors), one
+ * workgroup per matrix.  This is LUonDevice (include/numeric.h:6,
+ * src/numeric.cu:347-639) over many value sets of one pattern, with the
+ * values staged from A instead of from Symbolic_Matrix::predictLU
+ * (src/symbolic.cc:82-103).  flags bit0: perturb. */
+glu_status glu_batch_factor(glu_plan *p, int batch, const double *d_ax, float *d_lu, unsigned flags,
+                            void *stream);
 
 /* ------------------------------------------------------------------------
  * Uneven batch: distinct patterns refactored + refined in one fused launch.

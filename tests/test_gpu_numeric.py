@@ -293,3 +293,25 @@ def test_plan_rejects_bad_krylov(torch_mod):
     for bad in (-2, 5):
         with pytest.raises(RuntimeError):
             Plan(an, krylov=bad)
+
+
+@pytest.mark.parametrize("perturb", [False, True])
+def test_batch_factor_matches_single(torch_mod, perturb):
+    """glu_batch_factor (refresh + factor, one launch) gives, per matrix, the
+    single path's factors bit for bit, and the same factors as the batched
+    solve."""
+    from pyglu import mna
+    torch = torch_mod
+    data, indices, indptr, n, an, plan, pat = _setup("mna2k")
+    B = 7
+    axd = torch.from_numpy(np.stack([mna.perturb_values(data, 3300 + b) for b in range(B)])).cuda()
+    bd = torch.from_numpy(np.stack([mna.rhs(n, 3700 + b) for b in range(B)])).cuda()
+    lu = plan.batch_factor(axd, perturb=perturb)
+    _, lu_s, _, _ = plan.batch_solve(axd, bd, tol=1e-10, maxit=20, perturb=perturb)
+    torch.cuda.synchronize()
+    assert torch.equal(lu.view(torch.int32), lu_s.view(torch.int32))
+    for b in range(B):
+        one = plan.refresh(axd[b]).reshape(-1)
+        plan.factor(one, perturb=perturb)
+        torch.cuda.synchronize()
+        assert torch.equal(one.view(torch.int32), lu[b].view(torch.int32)), b
