@@ -1735,6 +1735,73 @@ __global__ void __launch_bounds__(1024) k_batch_group_refine(const MatDesc *__re
 }
 
 // ---------------------------------------------------------------------------
+// ABFT column checksums (src/symbolic.cc:163-197), a batch of value sets
+// ---------------------------------------------------------------------------
+//
+// One thread per column and matrix (blockIdx.y), each sum sequential in the
+// reference's order, FP32, products rounded before they are added.
+
+// CCA[i] = sum of column i of the values (before the factorization).
+__global__ void __launch_bounds__(256) k_abft_cca(const uint32_t *__restrict__ col_ptr,
+                                                  const float *__restrict__ val, float *__restrict__ cca,
+                                                  uint32_t n, uint32_t nnz)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float *v = val + (size_t)blockIdx.y * nnz;
+    float s = 0.f;
+    for (uint32_t j = col_ptr[i]; j < col_ptr[i + 1]; ++j) s += v[j];
+    cca[(size_t)blockIdx.y * n + i] = s;
+}
+
+// CCL[r] = 1 + sum of L(:, r); first_bad / max_diff armed for the check.
+__global__ void __launch_bounds__(256) k_abft_ccl(const uint32_t *__restrict__ col_ptr,
+                                                  const uint32_t *__restrict__ diag, const float *__restrict__ lu,
+                                                  float *__restrict__ ccl, int32_t *__restrict__ first_bad,
+                                                  unsigned int *__restrict__ max_bits, uint32_t n, uint32_t nnz)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        first_bad[blockIdx.y] = INT32_MAX;
+        max_bits[blockIdx.y] = 0u;
+    }
+    if (i >= n) return;
+    const float *v = lu + (size_t)blockIdx.y * nnz;
+    float s = 1.f;
+    for (uint32_t j = diag[i] + 1; j < col_ptr[i + 1]; ++j) s += v[j];
+    ccl[(size_t)blockIdx.y * n + i] = s;
+}
+
+// CCA_ABFT[i] = sum over U(:, i) (diagonal included) of CCL[row] * U; a
+// column fails when |CCA_ABFT - CCA| > tol (the reference's test: a NaN
+// difference does not fail it, but max_diff becomes NaN).
+__global__ void __launch_bounds__(256) k_abft_check(const uint32_t *__restrict__ col_ptr,
+                                                    const uint32_t *__restrict__ row_idx,
+                                                    const uint32_t *__restrict__ diag, const float *__restrict__ lu,
+                                                    const float *__restrict__ ccl, const float *__restrict__ cca,
+                                                    float tol, int32_t *__restrict__ first_bad,
+                                                    unsigned int *__restrict__ max_bits, uint32_t n, uint32_t nnz)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float *v = lu + (size_t)blockIdx.y * nnz;
+    const float *cl = ccl + (size_t)blockIdx.y * n;
+    float s = 0.f;
+    for (uint32_t j = col_ptr[i]; j <= diag[i]; ++j) s += cl[row_idx[j]] * v[j];
+    const float d = fabsf(s - cca[(size_t)blockIdx.y * n + i]);
+    if (d > tol) atomicMin(first_bad + blockIdx.y, (int32_t)i);
+    // non-negative floats order like their bit patterns; NaN sorts above inf
+    atomicMax(max_bits + blockIdx.y, d == d ? __float_as_uint(d) : 0x7fc00000u);
+}
+
+// first_bad: INT32_MAX (no failing column) -> -1.
+__global__ void k_abft_finish(int32_t *__restrict__ first_bad, int batch)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < batch && first_bad[b] == INT32_MAX) first_bad[b] = -1;
+}
+
+// ---------------------------------------------------------------------------
 // host-side launch wrappers (C++ linkage, used by glu_plan.hip)
 // ---------------------------------------------------------------------------
 
@@ -1832,6 +1899,23 @@ void batch_same(const MatDesc *desc, int batch, const double *ax, const double *
     else
         k_batch_same_refine<kSweepLevels><<<batch, threads, ref_lds, st>>>(desc, ax, b, x, lu, lucsr, t32, r64,
                                                                             norms, kry, iters, eta, tol, maxit);
+}
+
+void abft_cca(const FactorDev &F, const float *val, float *cca, int batch, hipStream_t st)
+{
+    k_abft_cca<<<dim3(blocks_for(F.n, 256), batch), 256, 0, st>>>(F.col_ptr, val, cca, F.n, F.nnz);
+}
+
+// max_diff: `batch` floats, filled from the bit maxima in place.
+void abft_check(const FactorDev &F, const uint32_t *row_idx, const uint32_t *diag, const float *lu,
+                const float *cca, float *ccl, float tol, int32_t *first_bad, float *max_diff, int batch,
+                hipStream_t st)
+{
+    unsigned int *bits = reinterpret_cast<unsigned int *>(max_diff);
+    const dim3 grid(blocks_for(F.n, 256), batch);
+    k_abft_ccl<<<grid, 256, 0, st>>>(F.col_ptr, diag, lu, ccl, first_bad, bits, F.n, F.nnz);
+    k_abft_check<<<grid, 256, 0, st>>>(F.col_ptr, row_idx, diag, lu, ccl, cca, tol, first_bad, bits, F.n, F.nnz);
+    k_abft_finish<<<blocks_for(batch, 256), 256, 0, st>>>(first_bad, batch);
 }
 
 void batch_factor(const MatDesc *desc, int batch, const double *ax, float *lu, int perturb, int threads,

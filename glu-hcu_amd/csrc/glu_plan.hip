@@ -48,6 +48,10 @@ void batch_same(const MatDesc *desc, int batch, const double *ax, const double *
                 hipStream_t st);
 void batch_factor(const MatDesc *desc, int batch, const double *ax, float *lu, int perturb, int threads,
                   size_t fact_lds, hipStream_t st);
+void abft_cca(const FactorDev &F, const float *val, float *cca, int batch, hipStream_t st);
+void abft_check(const FactorDev &F, const uint32_t *row_idx, const uint32_t *diag, const float *lu,
+                const float *cca, float *ccl, float tol, int32_t *first_bad, float *max_diff, int batch,
+                hipStream_t st);
 void batch_group(const MatDesc *descs, const void *ios, const uint32_t *order, int count, double tol,
                  int maxit, int perturb, int threads, int mode, size_t fact_lds, size_t ref_lds, hipStream_t st);
 void tri_df(const TriDev &T, const float *lu, float *x, int l0, int l1, uint32_t k0, uint32_t k1, uint32_t n,
@@ -287,6 +291,9 @@ struct glu_plan {
     float *d_pert = nullptr;
     unsigned long long *d_norms = nullptr;
     float *d_t32 = nullptr;      // n
+    const uint32_t *d_rowidx = nullptr, *d_diag = nullptr;   // sym_r_idx, l_col_ptr (arena)
+    float *d_abft = nullptr;     // ABFT column sums of L, n per matrix (grown on demand)
+    int abft_cap = 0;
     double *d_r64 = nullptr;     // n
     float *d_val = nullptr;      // nnz (glu_factor_host)
 
@@ -306,6 +313,7 @@ struct glu_plan {
         if (d_br64) (void)hipFree(d_br64);
         if (d_blucsr) (void)hipFree(d_blucsr);
         if (d_bkry) (void)hipFree(d_bkry);
+        if (d_abft) (void)hipFree(d_abft);
         if (d_bnorms) (void)hipFree(d_bnorms);
         for (GraphSlot *g : {&g_factor, &g_solve[0], &g_solve[1]})
             if (g->exec) (void)hipGraphExecDestroy(g->exec);
@@ -1296,6 +1304,8 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
         return t;
     };
     const uint32_t *colptr = u32p(o_colptr), *rowidx = u32p(o_rowidx), *diag = u32p(o_diag);
+    p->d_rowidx = rowidx;
+    p->d_diag = diag;
     const uint32_t *crptr = u32p(o_crptr), *ccidx = u32p(o_ccidx), *cdiag = u32p(o_cdiag);
     SolveDev &S = p->S;
     S.lo = tri(0, crptr, 0, cdiag, ccidx, upos_d, nullptr, 0);
@@ -1690,6 +1700,36 @@ extern "C" glu_status glu_batch_factor(glu_plan *p, int batch, const double *d_a
                          sizeof(float) * p->win_nslot, static_cast<hipStream_t>(stream));
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(GLU_ERR_DEVICE, std::string("batch factor launch: ") + hipGetErrorString(e));
+    return GLU_OK;
+}
+
+extern "C" glu_status glu_abft_sums(glu_plan *p, int batch, const float *d_val, float *d_cca, void *stream)
+{
+    if (!p || batch <= 0 || !d_val || !d_cca) return fail(GLU_ERR_ARGUMENT, "bad argument");
+    if (batch > 65535) return fail(GLU_ERR_ARGUMENT, "batch above 65535");
+    launch::abft_cca(p->F, d_val, d_cca, batch, static_cast<hipStream_t>(stream));
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(GLU_ERR_DEVICE, std::string("abft launch: ") + hipGetErrorString(e));
+    return GLU_OK;
+}
+
+extern "C" glu_status glu_abft_check(glu_plan *p, int batch, const float *d_lu, const float *d_cca, float tol,
+                                     int32_t *d_first_bad, float *d_max_diff, void *stream)
+{
+    if (!p || batch <= 0 || !d_lu || !d_cca || !d_first_bad || !d_max_diff || !(tol >= 0.f))
+        return fail(GLU_ERR_ARGUMENT, "bad argument");
+    if (batch > 65535) return fail(GLU_ERR_ARGUMENT, "batch above 65535");
+    if (batch > p->abft_cap) {
+        if (p->d_abft) (void)hipFree(p->d_abft);
+        p->d_abft = nullptr;
+        p->abft_cap = 0;
+        HIP_TRY(hipMalloc(&p->d_abft, sizeof(float) * (size_t)p->n * batch));
+        p->abft_cap = batch;
+    }
+    launch::abft_check(p->F, p->d_rowidx, p->d_diag, d_lu, d_cca, p->d_abft, tol, d_first_bad, d_max_diff, batch,
+                       static_cast<hipStream_t>(stream));
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(GLU_ERR_DEVICE, std::string("abft launch: ") + hipGetErrorString(e));
     return GLU_OK;
 }
 

@@ -124,6 +124,74 @@ std::vector<REAL> Symbolic_Matrix::solve(const glu_analysis *an, const std::vect
     return x;
 }
 
+// Device ABFT of one value set: CCA of val (want_cca) or the check of the
+// factors in val against CCA (first failing column, or -1).  Returns false
+// on a device error (the message goes to err).
+static bool abft_on_device(Symbolic_Matrix &A, bool want_cca, std::vector<REAL> &cca, int32_t &first,
+                           std::ostream &err)
+{
+    glu_pattern pat = pattern_of(A);
+    glu_plan *plan = nullptr;
+    if (glu_plan_create(&pat, nullptr, &plan) != GLU_OK) {
+        err << "ABFT: " << glu_last_error() << std::endl;
+        return false;
+    }
+    float *d_val = nullptr, *d_cca = nullptr, *d_mx = nullptr;
+    int32_t *d_first = nullptr;
+    const size_t n = A.n, nnz = A.nnz;
+    bool ok = hipMalloc(&d_val, sizeof(float) * nnz) == hipSuccess &&
+              hipMalloc(&d_cca, sizeof(float) * n) == hipSuccess && hipMalloc(&d_mx, sizeof(float)) == hipSuccess &&
+              hipMalloc(&d_first, sizeof(int32_t)) == hipSuccess;
+    ok = ok && hipMemcpy(d_val, A.val.data(), sizeof(float) * nnz, hipMemcpyHostToDevice) == hipSuccess;
+    if (ok && want_cca) {
+        ok = glu_abft_sums(plan, 1, d_val, d_cca, nullptr) == GLU_OK &&
+             hipMemcpy(cca.data(), d_cca, sizeof(float) * n, hipMemcpyDeviceToHost) == hipSuccess;
+    } else if (ok) {
+        ok = hipMemcpy(d_cca, cca.data(), sizeof(float) * n, hipMemcpyHostToDevice) == hipSuccess &&
+             glu_abft_check(plan, 1, d_val, d_cca, 1e-5f, d_first, d_mx, nullptr) == GLU_OK &&
+             hipMemcpy(&first, d_first, sizeof(int32_t), hipMemcpyDeviceToHost) == hipSuccess;
+    }
+    if (!ok) err << "ABFT: " << glu_last_error() << std::endl;
+    (void)hipFree(d_val);
+    (void)hipFree(d_cca);
+    (void)hipFree(d_mx);
+    (void)hipFree(d_first);
+    glu_plan_destroy(plan);
+    return ok;
+}
+
+void Symbolic_Matrix::ABFTCalculateCCA()
+{
+    CCA.assign(n, 0.f);
+    int32_t first = -1;
+    if (n && nnz && val.size() == nnz) (void)abft_on_device(*this, true, CCA, first, m_err);
+}
+
+void Symbolic_Matrix::ABFTCheckResult()
+{
+    if (n == 0 || nnz == 0 || val.size() != nnz || CCA.size() != n) {
+        m_err << "ABFT: no checksums (call ABFTCalculateCCA before the factorization)" << std::endl;
+        return;
+    }
+    int32_t first = -1;
+    if (!abft_on_device(*this, false, CCA, first, m_err)) return;
+    if (first >= 0) {
+        // the failing column's rebuilt checksum, for the message
+        const unsigned i = (unsigned)first;
+        float s = 0.f;
+        for (unsigned j = sym_c_ptr[i]; j <= l_col_ptr[i]; ++j) {
+            const unsigned r = sym_r_idx[j];
+            float cl = 1.f;
+            for (unsigned q = l_col_ptr[r] + 1; q < sym_c_ptr[r + 1]; ++q) cl += val[q];
+            s += cl * val[j];
+        }
+        m_err << "Column " << i << ": CCA = " << CCA[i] << ", CCA_ABFT[i] = " << s << std::endl;
+        m_err << "More to come..." << std::endl;
+        return;
+    }
+    m_out << "Results passed ABFT check." << std::endl;
+}
+
 void LUonDevice(Symbolic_Matrix &A_sym, std::ostream &out, std::ostream &err, bool PERTURB)
 {
     if (A_sym.n == 0 || A_sym.nnz == 0) {

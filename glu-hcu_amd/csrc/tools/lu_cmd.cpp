@@ -1,8 +1,10 @@
 // lu_cmd: the command-line front end of the reference (src/lu_cmd.cpp:25-138)
-// on this build's C++ interface.  ./lu_cmd -i file.mtx [-p]
+// on this build's C++ interface.  ./lu_cmd -i file.mtx [-p] [-a]
 // Reads a column-sorted triplet file, preprocesses (MC64 + AMD), runs the
 // symbolic steps with their timings, factors on the GPU (LUonDevice),
-// solves A x = 1 and writes x to x.dat, one value per line.
+// solves A x = 1 and writes x to x.dat, one value per line.  -a runs the
+// ABFT checksums around the factorization, as the reference's GLU_DEBUG
+// build does (src/lu_cmd.cpp:110-120).
 #include <chrono>
 #include <cstdlib>
 #include <cstring>
@@ -21,7 +23,8 @@ static void help_message()
     std::cout << "\nGLU program (MI355X numeric phase)\n"
               << "Usage: ./lu_cmd -i inputfile\n"
               << "Additional usage: ./lu_cmd -i inputfile -p\n"
-              << "-p to enable perturbation" << std::endl;
+              << "-p to enable perturbation\n"
+              << "-a to check the factors with ABFT column checksums" << std::endl;
 }
 
 static double ms_since(Clock::time_point t)
@@ -32,7 +35,7 @@ static double ms_since(Clock::time_point t)
 int main(int argc, char **argv)
 {
     const char *matrixName = nullptr;
-    bool PERTURB = false;
+    bool PERTURB = false, ABFT = false;
     if (argc < 3) {
         help_message();
         return -1;
@@ -47,6 +50,9 @@ int main(int argc, char **argv)
             i += 2;
         } else if (std::strcmp(argv[i], "-p") == 0) {
             PERTURB = true;
+            i += 1;
+        } else if (std::strcmp(argv[i], "-a") == 0) {
+            ABFT = true;
             i += 1;
         } else {
             help_message();
@@ -84,7 +90,9 @@ int main(int argc, char **argv)
     A_sym.leveling();
     std::cout << "Leveling time: " << ms_since(t) << " ms" << std::endl;
 
+    if (ABFT) A_sym.ABFTCalculateCCA();
     LUonDevice(A_sym, std::cout, std::cerr, PERTURB);
+    if (ABFT) A_sym.ABFTCheckResult();
 
     std::vector<REAL> b(n, 1.f);
     std::vector<REAL> x = A_sym.solve(an, b);

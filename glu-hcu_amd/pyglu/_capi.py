@@ -30,7 +30,7 @@ EXPORTED = [
     "glu_last_error", "glu_analyze", "glu_analysis_destroy", "glu_analysis_info", "glu_analysis_get",
     "glu_plan_create", "glu_plan_create_from_analysis", "glu_plan_destroy", "glu_plan_info",
     "glu_factor", "glu_factor_host", "glu_solve", "glu_refresh", "glu_refine", "glu_adjoint_values",
-    "glu_batch_solve", "glu_batch_factor",
+    "glu_batch_solve", "glu_batch_factor", "glu_abft_sums", "glu_abft_check",
     "glu_group_create", "glu_group_destroy", "glu_group_solve",
 ]
 
@@ -88,6 +88,8 @@ def lib():
         "glu_adjoint_values": (st, [P, P, P, P, P]),
         "glu_batch_solve": (st, [P, I, P, P, P, P, D, I, P, P, C.c_uint, P]),
         "glu_batch_factor": (st, [P, I, P, P, C.c_uint, P]),
+        "glu_abft_sums": (st, [P, I, P, P, P]),
+        "glu_abft_check": (st, [P, I, P, P, C.c_float, P, P, P]),
         "glu_group_create": (st, [P, I, C.POINTER(P)]),
         "glu_group_destroy": (None, [P]),
         "glu_group_solve": (st, [P, P, P, P, P, D, I, P, P, C.c_uint, P]),

@@ -169,6 +169,29 @@ class Plan:
                                               _stream()), "glu_batch_solve")
         return x, lu, iters, eta
 
+    def abft_sums(self, val):
+        """ABFT checksums of values in the LU pattern before the factorization
+        (Symbolic_Matrix::ABFTCalculateCCA): (B, nnz_lu) or (nnz_lu,) float32
+        -> (B, n) or (n,) float32 column sums."""
+        B = val.numel() // self.nnz_lu
+        _check_dev(val, torch.float32, B * self.nnz_lu, "val")
+        cca = torch.empty(B, self.n, dtype=torch.float32, device=val.device)
+        capi.check(capi.lib().glu_abft_sums(self._h, B, _dptr(val), _dptr(cca), _stream()), "glu_abft_sums")
+        return cca if val.dim() > 1 else cca.reshape(-1)
+
+    def abft_check(self, lu, cca, tol=1e-5):
+        """Symbolic_Matrix::ABFTCheckResult on factors lu against cca from
+        abft_sums: (first failing column or -1, largest |difference|) per
+        matrix, as int32 / float32 tensors."""
+        B = lu.numel() // self.nnz_lu
+        _check_dev(lu, torch.float32, B * self.nnz_lu, "lu")
+        _check_dev(cca, torch.float32, B * self.n, "cca")
+        first = torch.empty(B, dtype=torch.int32, device=lu.device)
+        mx = torch.empty(B, dtype=torch.float32, device=lu.device)
+        capi.check(capi.lib().glu_abft_check(self._h, B, _dptr(lu), _dptr(cca), float(tol), _dptr(first), _dptr(mx),
+                                             _stream()), "glu_abft_check")
+        return first, mx
+
     def batch_factor(self, ax, lu=None, perturb=False):
         """One launch: refresh + factor per matrix (no solve).
 

@@ -16,6 +16,8 @@
  *       -> glu_plan_create + glu_factor_host (+ glu_plan_destroy)
  *   Symbolic_Matrix::solve(SNicsLU*, const vector<REAL>&)
  *       src/symbolic.cc:200-251                             -> glu_solve
+ *   Symbolic_Matrix::ABFTCalculateCCA / ABFTCheckResult
+ *       src/symbolic.cc:163-197 (GLU_DEBUG)  -> glu_abft_sums / glu_abft_check
  *   GLUFactorization(data, indices, indptr, n, perturb) / .solve(b)
  *       src/glu_binding.cpp:44-155 (pybind11 module _pyglu) -> glu_analyze +
  *       glu_plan_create_from_analysis + glu_factor + glu_solve
@@ -194,6 +196,22 @@ glu_status glu_batch_solve(glu_plan *p, int batch, const double *d_ax, const dou
  * (src/symbolic.cc:82-103).  flags bit0: perturb. */
 glu_status glu_batch_factor(glu_plan *p, int batch, const double *d_ax, float *d_lu, unsigned flags,
                             void *stream);
+
+/* ABFT column checksums, Symbolic_Matrix::ABFTCalculateCCA and
+ * ABFTCheckResult (src/symbolic.cc:163-197; called around LUonDevice by
+ * src/lu_cmd.cpp:110-120 under GLU_DEBUG), for `batch` value sets of the
+ * plan's pattern (stride nnz_lu; sums stride n), FP32 with sequential sums
+ * in the reference's order.
+ * glu_abft_sums: d_cca[i] = sum of column i of d_val (the values before the
+ * factorization).
+ * glu_abft_check: with CCL[r] = 1 + sum of L(:,r) and CCA_ABFT[i] =
+ * sum_{U(:,i), diagonal included} CCL[row] * U, d_first_bad[b] = the first
+ * column with |CCA_ABFT - CCA| > tol (the reference uses tol = 1e-5; as
+ * there, a NaN difference does not fail), or -1; d_max_diff[b] = the
+ * largest |CCA_ABFT - CCA| (NaN if any is NaN).  batch <= 65535. */
+glu_status glu_abft_sums(glu_plan *p, int batch, const float *d_val, float *d_cca, void *stream);
+glu_status glu_abft_check(glu_plan *p, int batch, const float *d_lu, const float *d_cca, float tol,
+                          int32_t *d_first_bad, float *d_max_diff, void *stream);
 
 /* ------------------------------------------------------------------------
  * Uneven batch: distinct patterns refactored + refined in one fused launch.

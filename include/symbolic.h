@@ -40,6 +40,15 @@ public:
     // (the message goes to the error stream).
     std::vector<REAL> solve(const glu_analysis *an, const std::vector<REAL> &rhs);
 
+    // ABFT (src/symbolic.cc:163-197; the reference compiles it under
+    // GLU_DEBUG, this build always): CCA = column sums of val before the
+    // factorization, then the checksum rebuilt from the factors in val,
+    // compared column by column with tolerance 1e-5.  Both run on the GPU
+    // (glu_abft_sums / glu_abft_check) and print the reference's lines.
+    std::vector<REAL> CCA;
+    void ABFTCalculateCCA();
+    void ABFTCheckResult();
+
     Symbolic_Matrix(unsigned n, std::ostream &out, std::ostream &err)
         : n(n), nnz(0), num_lev(0), m_out(out), m_err(err) {}
     Symbolic_Matrix(std::ostream &out, std::ostream &err)

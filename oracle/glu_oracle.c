@@ -22,6 +22,7 @@
  *   oracle_factor_f64  the same right-looking elimination in FP64 — the
  *                      FP64 LU of the same permuted matrix.
  *   oracle_solve_f64   FP64 permute/scale + triangular solves, A or A'.
+ *   oracle_abft_*      src/symbolic.cc:163-197 (ABFT column checksums).
  */
 #include "glu_oracle.h"
 
@@ -183,4 +184,41 @@ void oracle_solve_f64(const oracle_pattern *P, const double *lu, const uint32_t 
         for (uint32_t i = 0; i < n; ++i) x[i] = y[col_perm_inv[i]] * col_scale_perm[col_perm_inv[i]];
     }
     free(y);
+}
+
+/* src/symbolic.cc:163-170 */
+void oracle_abft_cca(const oracle_pattern *P, const float *val, float *cca)
+{
+    for (uint32_t i = 0; i < P->n; ++i) {
+        float s = 0.f;
+        for (uint32_t j = P->sym_c_ptr[i]; j < P->sym_c_ptr[i + 1]; ++j) s += val[j];
+        cca[i] = s;
+    }
+}
+
+/* src/symbolic.cc:172-197 */
+int32_t oracle_abft_check(const oracle_pattern *P, const float *lu, const float *cca, float tol, float *abft,
+                          float *max_diff)
+{
+    const uint32_t n = P->n;
+    float *ccl = (float *)malloc(sizeof(float) * (n ? n : 1));
+    for (uint32_t i = 0; i < n; ++i) {
+        float s = 1.f;
+        for (uint32_t j = P->l_col_ptr[i] + 1; j < P->sym_c_ptr[i + 1]; ++j) s += lu[j];
+        ccl[i] = s;
+    }
+    int32_t first = -1;
+    float mx = 0.f;
+    for (uint32_t i = 0; i < n; ++i) {
+        float s = 0.f;
+        for (uint32_t j = P->sym_c_ptr[i]; j <= P->l_col_ptr[i]; ++j) s += ccl[P->sym_r_idx[j]] * lu[j];
+        abft[i] = s;
+        const float d = fabsf(s - cca[i]);
+        if (d > tol && first < 0) first = (int32_t)i;
+        if (d != d || mx != mx) mx = NAN;
+        else if (d > mx) mx = d;
+    }
+    free(ccl);
+    *max_diff = mx;
+    return first;
 }

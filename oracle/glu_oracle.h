@@ -50,6 +50,18 @@ void oracle_solve_f64(const oracle_pattern *P, const double *lu, const uint32_t 
                       const uint32_t *col_perm_inv, const double *col_scale_perm,
                       const double *row_scale, const double *b, double *x, int transpose);
 
+/* ABFT of src/symbolic.cc:163-197 (Symbolic_Matrix::ABFTCalculateCCA and
+ * ABFTCheckResult, FP32, sequential sums in the reference's order).
+ * oracle_abft_cca: cca[i] = sum of column i of val (values before the
+ * factorization).  oracle_abft_check: ccl[r] = 1 + sum of L(:,r);
+ * abft[i] = sum_{j in U(:,i) incl. diagonal} ccl[row j] * lu[j]; returns the
+ * first column with |abft - cca| > tol (the reference's test, so a NaN
+ * difference passes), or -1; *max_diff = the largest |abft - cca| (NaN if
+ * any difference is NaN). */
+void oracle_abft_cca(const oracle_pattern *P, const float *val, float *cca);
+int32_t oracle_abft_check(const oracle_pattern *P, const float *lu, const float *cca, float tol, float *abft,
+                          float *max_diff);
+
 #ifdef __cplusplus
 }
 #endif

@@ -16,6 +16,11 @@ Pinning status of each function (see tests/test_oracle.py):
                 pivoting) on the dumped matrix.
   solve_f64     pinned: scipy.sparse.linalg.spsolve on the original A
                 (A and A').
+  abft_cca /    pinned: bit-exact CCA and the same first failing column as
+  abft_check    the reference's own ABFTCalculateCCA / ABFTCheckResult
+                (src/symbolic.cc:163-197, oracle/_ref built with GLU_DEBUG),
+                on passing factors and on planted errors; closed forms
+                (diagonal and unit-triangular matrices).
 """
 
 from __future__ import annotations
@@ -57,6 +62,9 @@ def _load():
         L.oracle_solve_f32.argtypes = [C.POINTER(Pattern), P, P, P, P, P, P, P]
         L.oracle_factor_f64.argtypes = [C.POINTER(Pattern), P]
         L.oracle_solve_f64.argtypes = [C.POINTER(Pattern)] + [P] * 9 + [C.c_int]
+        L.oracle_abft_cca.argtypes = [C.POINTER(Pattern), P, P]
+        L.oracle_abft_check.argtypes = [C.POINTER(Pattern), P, P, C.c_float, P, P]
+        L.oracle_abft_check.restype = C.c_int32
         _lib = L
     return _lib
 
@@ -164,3 +172,23 @@ def pattern_to_dense(pat: SymPattern, val):
         s, e = pat.sym_c_ptr[c], pat.sym_c_ptr[c + 1]
         D[pat.sym_r_idx[s:e], c] = val[s:e]
     return D
+
+
+def abft_cca(pat: SymPattern, val):
+    """src/symbolic.cc:163-170: FP32 column sums of the values before the
+    factorization."""
+    v = np.ascontiguousarray(val, dtype=np.float32)
+    cca = np.empty(pat.n, dtype=np.float32)
+    _load().oracle_abft_cca(pat.c, _p(v), _p(cca))
+    return cca
+
+
+def abft_check(pat: SymPattern, lu, cca, tol=1e-5):
+    """src/symbolic.cc:172-197: (first failing column or -1, the rebuilt
+    checksums CCA_ABFT, the largest |CCA_ABFT - CCA|)."""
+    lu = np.ascontiguousarray(lu, dtype=np.float32)
+    cca = np.ascontiguousarray(cca, dtype=np.float32)
+    abft = np.empty(pat.n, dtype=np.float32)
+    mx = np.zeros(1, dtype=np.float32)
+    first = _load().oracle_abft_check(pat.c, _p(lu), _p(cca), float(tol), _p(abft), _p(mx))
+    return int(first), abft, float(mx[0])

@@ -37,6 +37,8 @@ def _load():
         lib.ref_dump.argtypes = [P] * 4
         lib.ref_symbolic.argtypes = [P] * 10
         lib.ref_solve.argtypes = [P] * 4
+        lib.ref_abft.argtypes = [P] * 4
+        lib.ref_abft.restype = C.c_int
         lib.ref_nicslu_time.argtypes = [C.c_uint, C.c_uint, P, P, P, P, C.c_int, C.c_int, P, P]
         lib.ref_nicslu_time.restype = C.c_int
         _lib = lib
@@ -80,6 +82,16 @@ class RefAnalysis:
         x = np.empty(self.n, dtype=np.float32)
         _load().ref_solve(C.c_void_p(self._handle), _p(lu), _p(rhs), _p(x))
         return x
+
+    def abft(self, val0: np.ndarray, lu: np.ndarray):
+        """The reference's ABFT (src/symbolic.cc:163-197): CCA of val0, then
+        the check of the factors lu.  Returns (cca, first failing column or
+        -1 for a pass)."""
+        val0 = np.ascontiguousarray(val0, dtype=np.float32)
+        lu = np.ascontiguousarray(lu, dtype=np.float32)
+        cca = np.empty(self.n, dtype=np.float32)
+        first = _load().ref_abft(C.c_void_p(self._handle), _p(val0), _p(lu), _p(cca))
+        return cca, int(first)
 
     def close(self):
         if self._handle:

@@ -11,10 +11,12 @@
 // src/glu_binding.cpp:105-109 (fill_in, csr, predictLU, leveling), then
 // Symbolic_Matrix::solve (src/symbolic.cc:200-251).
 
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <ctime>
 #include <sstream>
+#include <string>
 #include <vector>
 
 extern "C" {
@@ -143,6 +145,30 @@ void ref_solve(void *hp, const float *lu, const float *rhs, float *x)
     std::vector<float> xv = s->solve(&h->nicslu, r);
     std::copy(xv.begin(), xv.end(), x);
     s->val.swap(saved);
+}
+
+// The reference's ABFT (src/symbolic.cc:163-197, compiled with GLU_DEBUG=1):
+// ABFTCalculateCCA on val0 (the values before the factorization), then
+// ABFTCheckResult on the factors lu.  cca[n] receives CCA.  Returns the
+// first failing column the check reports, -1 if it reports a pass, -2 if
+// its message is neither.
+int ref_abft(void *hp, const float *val0, const float *lu, float *cca)
+{
+    RefHandle *h = static_cast<RefHandle *>(hp);
+    Symbolic_Matrix *s = h->sym;
+    std::vector<float> saved(s->val);
+    std::copy(val0, val0 + s->nnz, s->val.begin());
+    s->ABFTCalculateCCA();
+    std::copy(s->CCA.begin(), s->CCA.end(), cca);
+    std::copy(lu, lu + s->nnz, s->val.begin());
+    h->out.str("");
+    h->err.str("");
+    s->ABFTCheckResult();
+    s->val.swap(saved);
+    const std::string e = h->err.str(), o = h->out.str();
+    unsigned col = 0;
+    if (sscanf(e.c_str(), "Column %u:", &col) == 1) return (int)col;
+    return o.find("passed ABFT check") != std::string::npos ? -1 : -2;
 }
 
 static double now_s()

@@ -147,5 +147,7 @@ def test_null_arguments():
     assert L.glu_solve(None, None, None, None, 0, None) == -2
     assert L.glu_batch_solve(None, 1, None, None, None, None, 1e-10, 5, None, None, 0, None) == -2
     assert L.glu_batch_factor(None, 1, None, None, 0, None) == -2
+    assert L.glu_abft_sums(None, 1, None, None, None) == -2
+    assert L.glu_abft_check(None, 1, None, None, 1e-5, None, None, None) == -2
     h = C.c_void_p()
     assert L.glu_analyze(0, 0, None, None, None, C.byref(h)) == -9

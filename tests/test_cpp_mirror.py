@@ -72,23 +72,27 @@ def test_triplet_reader(tmp_path):
 @pytest.mark.gpu
 def test_lu_cmd_on_add32(tmp_path):
     """The CLI flow of src/lu_cmd.cpp on add32: preprocess, symbolic,
-    LUonDevice, solve with b = 1, x.dat — x equals the oracle's FP32 solve
-    with the oracle's FP32 factors (to the 6 digits x.dat holds)."""
+    LUonDevice (with the ABFT check around it, -a), solve with b = 1, x.dat —
+    x equals the oracle's FP32 solve with the oracle's FP32 factors (to the 6
+    digits x.dat holds), and the factors pass the ABFT check, as the
+    oracle's restatement of src/symbolic.cc:172-197 says they do."""
     from oracle import oracle as O
     from pyglu.analysis import Analysis
     d, i, p, n = add32()
     f = tmp_path / "add32.mtx"
     write_mtx(f, d, i, p, n)
-    r = subprocess.run([os.path.join(PKG, "lu_cmd"), "-i", str(f)], capture_output=True, text=True,
+    r = subprocess.run([os.path.join(PKG, "lu_cmd"), "-i", str(f), "-a"], capture_output=True, text=True,
                        cwd=tmp_path, timeout=120)
     print(r.stdout)
     assert r.returncode == 0, r.stderr
-    for key in ("Symbolic nonzero:", "Number of levels:", "Device 0:", "Total GPU time:"):
+    for key in ("Symbolic nonzero:", "Number of levels:", "Device 0:", "Total GPU time:",
+                "Results passed ABFT check."):
         assert key in r.stdout
     x = np.loadtxt(tmp_path / "x.dat")
     an = Analysis(d, i, p, n)
     pat = O.SymPattern(an)
     lu, _ = O.factor_f32(pat, an.val)
+    assert O.abft_check(pat, lu, O.abft_cca(pat, an.val))[0] == -1
     want = O.solve_f32(pat, lu, an.col_perm, an.row_perm_inv, an.col_scale_perm, an.row_scale,
                        np.ones(n, np.float32))
     assert np.allclose(x, want, rtol=1e-5, atol=1e-6 * np.abs(want).max())

@@ -315,3 +315,33 @@ def test_batch_factor_matches_single(torch_mod, perturb):
         plan.factor(one, perturb=perturb)
         torch.cuda.synchronize()
         assert torch.equal(one.view(torch.int32), lu[b].view(torch.int32)), b
+
+
+def test_abft_device_matches_oracle(torch_mod):
+    """glu_abft_sums / glu_abft_check (src/symbolic.cc:163-197) on a batch of
+    three value sets: checksums bit-identical to the oracle's, the same
+    verdict and largest difference on correct factors, and the planted
+    error's column reported in the corrupted member only."""
+    from oracle import oracle as O
+    from pyglu import mna
+    torch = torch_mod
+    data, indices, indptr, n, an, plan, pat = _setup("mna2k")
+    B = 3
+    axd = torch.from_numpy(np.stack([mna.perturb_values(data, 3900 + b) for b in range(B)])).cuda()
+    val0 = torch.stack([plan.refresh(axd[b]).reshape(-1) for b in range(B)])
+    lu = val0.clone()
+    for b in range(B):
+        plan.factor(lu[b])
+    cca = plan.abft_sums(val0)
+    c = n // 2
+    lu[1, int(pat.l_col_ptr[c])] += 1.0
+    first, mx = plan.abft_check(lu, cca)
+    torch.cuda.synchronize()
+    v0h, luh, ccah = val0.cpu().numpy(), lu.cpu().numpy(), cca.cpu().numpy()
+    fh, mh = first.cpu().numpy(), mx.cpu().numpy()
+    for b in range(B):
+        want = O.abft_cca(pat, v0h[b])
+        assert np.array_equal(ccah[b].view(np.uint32), want.view(np.uint32))
+        f, _, m = O.abft_check(pat, luh[b], want)
+        assert int(fh[b]) == f and mh[b] == np.float32(m), (b, int(fh[b]), f, mh[b], m)
+    assert int(fh[0]) == -1 and int(fh[2]) == -1 and int(fh[1]) == c

@@ -263,3 +263,58 @@ def test_conflicts_brute_force():
         if c >= 2:
             want[pos[(i, j)]] = True
     assert np.array_equal(mark, want)
+
+
+def test_abft_closed_forms():
+    """ABFT on a diagonal matrix (CCL = 1, checksums = the pivots) and on a
+    2x2 matrix (CCL_0 = 1 + c/a: (1 + l) a = a + c, (1 + l) b + (d - l b) =
+    b + d) — src/symbolic.cc:163-197."""
+    M = np.diag(np.array([2.0, -3.0, 0.5, 7.0], np.float32))
+    P = DensePattern(M != 0)
+    pat = O.SymPattern(P)
+    v = P.values(M).astype(np.float32)
+    lu, _ = O.factor_f32(pat, v)
+    cca = O.abft_cca(pat, v)
+    assert np.array_equal(cca, np.array([2.0, -3.0, 0.5, 7.0], np.float32))
+    first, abft, mx = O.abft_check(pat, lu, cca)
+    assert first == -1 and mx == 0.0 and np.array_equal(abft, cca)
+    a, b, c, d = 4.0, 1.0, 2.0, 3.0
+    M2 = np.array([[a, b], [c, d]], np.float32)
+    P2 = DensePattern(M2 != 0)
+    pat2 = O.SymPattern(P2)
+    v2 = P2.values(M2).astype(np.float32)
+    lu2, _ = O.factor_f32(pat2, v2)
+    cca2 = O.abft_cca(pat2, v2)
+    first2, abft2, mx2 = O.abft_check(pat2, lu2, cca2)
+    assert np.array_equal(cca2, np.array([a + c, b + d], np.float32))
+    assert first2 == -1 and np.allclose(abft2, [a + c, b + d], rtol=1e-6)
+    # a wrong pivot shows up in its own column
+    bad = lu2.copy()
+    bad[pat2.l_col_ptr[1]] += 0.5
+    assert O.abft_check(pat2, bad, cca2)[0] == 1
+
+
+@pytest.mark.parametrize("name", ["mna600", "add32"])
+def test_abft_matches_reference(name):
+    """oracle_abft_* against the reference's own ABFTCalculateCCA /
+    ABFTCheckResult (oracle/_ref, GLU_DEBUG=1): CCA bit for bit, the same
+    verdict on the oracle's factors and the same first failing column after
+    planted errors in two columns."""
+    from oracle import ref
+    if not ref.available():
+        pytest.skip("oracle/_ref not built")
+    d, i, p, n = mna_case(name)
+    r = ref.analyze(d, i, p, n)
+    pat = O.SymPattern(r)
+    lu, _ = O.factor_f32(pat, r.val)
+    cca_ref, first_ref = r.abft(r.val, lu)
+    cca = O.abft_cca(pat, r.val)
+    assert np.array_equal(cca.view(np.uint32), cca_ref.view(np.uint32))
+    first, abft, mx = O.abft_check(pat, lu, cca)
+    assert first == first_ref == -1
+    bad = lu.copy()
+    c1, c2 = n // 3, (2 * n) // 3
+    bad[pat.l_col_ptr[c2]] += 1.0          # pivot of column c2
+    bad[pat.sym_c_ptr[c1]] += 1.0          # first (U or pivot) entry of column c1
+    first_b, _, _ = O.abft_check(pat, bad, cca)
+    assert first_b == r.abft(r.val, bad)[1] == c1
