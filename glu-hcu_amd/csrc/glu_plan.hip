@@ -17,6 +17,9 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <atomic>
+#include <system_error>
+#include <thread>
 #include <vector>
 
 #include "../../include/glu_hip.h"
@@ -147,6 +150,40 @@ extern "C" glu_status glu_analyze(uint32_t n, uint32_t nnz, const double *ax, co
     }
     if (rc != kOk) return fail((glu_status)rc, err);
     *out = a.release();
+    return GLU_OK;
+}
+
+// Independent patterns analysed on host threads (a work queue over the
+// matrices); each item as glu_analyze.
+extern "C" glu_status glu_analyze_many(int count, const uint32_t *n, const uint32_t *nnz,
+                                       const double *const *ax, const uint32_t *const *ai,
+                                       const uint32_t *const *ap, int threads, glu_analysis **out,
+                                       glu_status *status)
+{
+    if (count <= 0 || !n || !nnz || !ax || !ai || !ap || !out || !status)
+        return fail(GLU_ERR_ARGUMENT, "bad argument");
+    for (int i = 0; i < count; ++i) out[i] = nullptr;
+    int hw = (int)std::thread::hardware_concurrency();
+    int nt = threads > 0 ? threads : (hw > 0 ? hw : 1);
+    nt = std::max(1, std::min(nt, count));
+    std::vector<std::string> msg(count);
+    std::atomic<int> next(0);
+    auto work = [&]() {
+        for (int i = next.fetch_add(1); i < count; i = next.fetch_add(1)) {
+            status[i] = glu_analyze(n[i], nnz[i], ax[i], ai[i], ap[i], out + i);
+            if (status[i] != GLU_OK) msg[i] = glu_last_error();
+        }
+    };
+    try {
+        std::vector<std::thread> pool;
+        for (int t = 1; t < nt; ++t) pool.emplace_back(work);
+        work();
+        for (std::thread &t : pool) t.join();
+    } catch (const std::system_error &) {
+        work();   // no threads available: finish on this one
+    }
+    for (int i = 0; i < count; ++i)
+        if (status[i] != GLU_OK) return fail(status[i], "matrix " + std::to_string(i) + ": " + msg[i]);
     return GLU_OK;
 }
 

@@ -27,7 +27,7 @@ F_SYM_C_PTR, F_SYM_R_IDX, F_L_COL_PTR, F_CSR_R_PTR, F_CSR_C_IDX, F_CSR_DIAG_PTR,
     F_ROW_SCALE, F_COL_SCALE_PERM, F_DUMP_AX, F_DUMP_AI, F_DUMP_AP = range(18)
 
 EXPORTED = [
-    "glu_last_error", "glu_analyze", "glu_analysis_destroy", "glu_analysis_info", "glu_analysis_get",
+    "glu_last_error", "glu_analyze", "glu_analyze_many", "glu_analysis_destroy", "glu_analysis_info", "glu_analysis_get",
     "glu_plan_create", "glu_plan_create_from_analysis", "glu_plan_destroy", "glu_plan_info",
     "glu_factor", "glu_factor_host", "glu_solve", "glu_refresh", "glu_refine", "glu_adjoint_values",
     "glu_batch_solve", "glu_batch_factor", "glu_abft_sums", "glu_abft_check",
@@ -73,6 +73,7 @@ def lib():
     sig = {
         "glu_last_error": (C.c_char_p, []),
         "glu_analyze": (st, [U, U, P, P, P, C.POINTER(P)]),
+        "glu_analyze_many": (st, [I, P, P, P, P, P, I, P, P]),
         "glu_analysis_destroy": (None, [P]),
         "glu_analysis_info": (st, [P, P]),
         "glu_analysis_get": (st, [P, I, P, C.c_size_t]),

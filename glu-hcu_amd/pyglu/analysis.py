@@ -53,6 +53,16 @@ class Analysis:
         h = C.c_void_p()
         capi.check(L.glu_analyze(n, ax.size, capi.ptr(ax), capi.ptr(ai), capi.ptr(ap), C.byref(h)),
                    "glu_analyze")
+        self._adopt(h)
+
+    @classmethod
+    def _from_handle(cls, h):
+        obj = cls.__new__(cls)
+        obj._adopt(h)
+        return obj
+
+    def _adopt(self, h):
+        L = capi.lib()
         self._h = h
         info = np.zeros(8, dtype=np.int64)
         capi.check(L.glu_analysis_info(h, capi.ptr(info)), "glu_analysis_info")
@@ -92,3 +102,40 @@ class Analysis:
             self.close()
         except Exception:
             pass
+
+
+def analyze_many(mats, threads: int = 0):
+    """Analyse independent CSC matrices ``[(data, indices, indptr, n), ...]``
+    on host threads (``glu_analyze_many``; 0 = one per core).  Returns one
+    :class:`Analysis` per matrix, each identical to ``Analysis(*mat)``."""
+    L = capi.lib()
+    cnt = len(mats)
+    if cnt == 0:
+        return []
+    keep = []
+    ns = np.zeros(cnt, dtype=np.uint32)
+    nnzs = np.zeros(cnt, dtype=np.uint32)
+    pax = (C.c_void_p * cnt)()
+    pai = (C.c_void_p * cnt)()
+    pap = (C.c_void_p * cnt)()
+    for k, (data, indices, indptr, n) in enumerate(mats):
+        ax = np.ascontiguousarray(data, dtype=np.float64)
+        ai = np.ascontiguousarray(indices, dtype=np.uint32)
+        ap = np.ascontiguousarray(indptr, dtype=np.uint32)
+        if ap.size != n + 1:
+            raise ValueError(f"matrix {k}: indptr length {ap.size} inconsistent with n={n}")
+        if ai.size != ax.size:
+            raise ValueError(f"matrix {k}: data and indices length mismatch")
+        keep += [ax, ai, ap]
+        ns[k], nnzs[k] = n, ax.size
+        pax[k], pai[k], pap[k] = capi.ptr(ax), capi.ptr(ai), capi.ptr(ap)
+    out = (C.c_void_p * cnt)()
+    status = np.zeros(cnt, dtype=np.int32)
+    rc = L.glu_analyze_many(cnt, capi.ptr(ns), capi.ptr(nnzs), pax, pai, pap, int(threads), out,
+                            capi.ptr(status))
+    if rc != capi.GLU_OK:
+        for k in range(cnt):
+            if out[k]:
+                L.glu_analysis_destroy(C.c_void_p(out[k]))
+        capi.check(rc, "glu_analyze_many")
+    return [Analysis._from_handle(C.c_void_p(out[k])) for k in range(cnt)]

@@ -64,6 +64,14 @@ glu_status glu_analyze(uint32_t n, uint32_t nnz, const double *ax, const uint32_
                        const uint32_t *ap, glu_analysis **out);
 void glu_analysis_destroy(glu_analysis *a);
 
+/* Independent matrices analysed on `threads` host threads (<= 0: one per
+ * core), each exactly as glu_analyze (same results); out[i] / status[i] per
+ * matrix.  Returns GLU_OK, or the first failing matrix's status with its
+ * index in glu_last_error().  No counterpart in the reference: the
+ * patterns of an uneven batch are independent. */
+glu_status glu_analyze_many(int count, const uint32_t *n, const uint32_t *nnz, const double *const *ax,
+                            const uint32_t *const *ai, const uint32_t *const *ap, int threads,
+                            glu_analysis **out, glu_status *status);
 /* info[0]=n info[1]=nnz(A) info[2]=nnz(L+U) info[3]=num_lev
  * info[4]=scalar updates of one factorization  info[5]=L-solve levels
  * info[6]=U-solve levels  info[7]=reserved */

@@ -172,3 +172,31 @@ def test_python_frontend_validation():
         pyglu.GLUFactorization(np.ones(2), np.array([0]), np.array([0, 1, 2]), 2)
     with pytest.raises(RuntimeError):  # structurally singular: preprocessing fails
         pyglu.GLUFactorization(np.ones(2), np.array([0, 1], dtype=np.int32), np.array([0, 2, 2], dtype=np.int32), 2)
+
+
+def test_analyze_many_matches_single():
+    """glu_analyze_many on host threads: every array of every analysis equals
+    the one-at-a-time glu_analyze result (12 uneven MNA patterns, 4 threads),
+    and a failing member is reported with its index."""
+    from pyglu.analysis import analyze_many
+    mats = [mna.mna_csc(mna.MNASpec(n_nodes=min(spec.n_nodes, 4000), n_vsrc=min(spec.n_vsrc, 20),
+                                    extra_hi=spec.extra_hi, n_rails=spec.n_rails,
+                                    rail_deg=min(spec.rail_deg, 100)), seed)
+            for spec, seed in mna.uneven_specs(12, seed0=4200)]
+    many = analyze_many(mats, threads=4)
+    for m, a in zip(mats, many):
+        one = Analysis(*m)
+        assert (a.n, a.nnz_lu, a.num_lev) == (one.n, one.nnz_lu, one.num_lev)
+        for key in ("sym_c_ptr", "sym_r_idx", "l_col_ptr", "level_ptr", "level_idx", "row_perm", "col_perm"):
+            assert np.array_equal(getattr(a, key), getattr(one, key)), key
+        assert np.array_equal(a.val.view(np.uint32), one.val.view(np.uint32))
+        assert np.array_equal(a.row_scale, one.row_scale) and np.array_equal(a.col_scale_perm, one.col_scale_perm)
+    # a structurally singular member (an empty column) fails with its index
+    d, i, p, n = mats[1]
+    col = 3
+    keep = np.ones(d.size, bool)
+    keep[p[col]:p[col + 1]] = False
+    p2 = np.concatenate([[0], np.cumsum([(p[c + 1] - p[c]) if c != col else 0 for c in range(n)])])
+    bad = (d[keep], i[keep], p2, n)
+    with pytest.raises(GluError, match="matrix 1"):
+        analyze_many([mats[0], bad, mats[2]], threads=2)
