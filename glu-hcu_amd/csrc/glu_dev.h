@@ -60,7 +60,7 @@ struct WinDev {
     int32_t l0, l1;
     uint32_t nslot, pad;
     const uint32_t *pos;
-    const uint4 *cln;
+    const uint2 *cln;
     const uint32_t *ctgt;
     const uint32_t *cup;
     const uint32_t *ment;
@@ -73,11 +73,11 @@ struct FactorDev {
     // Column tasks, by level (tasks ctask_ptr[l] .. ctask_ptr[l+1]).  Lane
     // `lane` of task t owns one L entry of a pivot column k of the level and
     // up to four consecutive entries of U_k: cln[64 t + lane] = {L(i,k)
-    // position, pivot position, update count <= 4, 0}; its e-th update reads
-    // U(k,j) at cup[256 t + 64 e + lane] and targets ctgt[same index] (~0u:
-    // a merge entry does it).
+    // position, pivot position} ({0, 0} for an unused lane); its e-th update
+    // reads U(k,j) at cup[256 t + 64 e + lane] and targets ctgt[same index]
+    // (~0u: no update here — an unused slot, or a merge entry does it).
     const uint32_t *ctask_ptr; // [num_lev + 2]
-    const uint4 *cln;
+    const uint2 *cln;
     const uint32_t *ctgt;
     const uint32_t *cup;
     const uint32_t *ment;      // merge entries (target CSC positions), by level,

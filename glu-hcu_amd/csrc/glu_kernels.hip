@@ -101,19 +101,18 @@ __device__ __forceinline__ float gesp(float d, float pert, int perturb)
 // target's line is usually already in flight with the task's other values.
 //
 // CTab holds a task's table data for this lane (independent of the values,
-// so it can be loaded ahead): the L and pivot positions, the update count,
-// the U positions and the targets.
+// so it can be loaded ahead): the L and pivot positions, the U positions and
+// the targets (~0u: no update in that slot).
 struct CTab {
-    uint32_t lpos, dpos, cnt;
+    uint32_t lpos, dpos;
     uint32_t tt[4], up[4];
 };
 
 __device__ __forceinline__ void ctab_load(const FactorDev &F, uint32_t t, uint32_t lane, CTab &c)
 {
-    const uint4 ln = F.cln[(size_t)t * kWave + lane];
+    const uint2 ln = F.cln[(size_t)t * kWave + lane];
     c.lpos = ln.x;
     c.dpos = ln.y;
-    c.cnt = ln.z;
     const size_t q = (size_t)t * (4 * kWave) + lane;
 #pragma unroll
     for (uint32_t u = 0; u < 4; ++u) {
@@ -125,18 +124,19 @@ __device__ __forceinline__ void ctab_load(const FactorDev &F, uint32_t t, uint32
 __device__ __forceinline__ void ctab_run(const FactorDev &F, float *__restrict__ val, const CTab &c, float pert,
                                          int perturb)
 {
-    if (c.cnt == 0) return;
+    if ((c.tt[0] & c.tt[1] & c.tt[2] & c.tt[3]) == ~0u) return;   // nothing to update
     const float d = val[c.dpos], a = val[c.lpos];
     float uv[4], tv[4];
 #pragma unroll
-    for (uint32_t u = 0; u < 4; ++u) {
-        if (u < c.cnt) uv[u] = val[c.up[u]];
-        if (u < c.cnt && c.tt[u] != ~0u) tv[u] = val[c.tt[u]];
-    }
+    for (uint32_t u = 0; u < 4; ++u)
+        if (c.tt[u] != ~0u) {
+            uv[u] = val[c.up[u]];
+            tv[u] = val[c.tt[u]];
+        }
     const float l = a / gesp(d, pert, perturb);
 #pragma unroll
     for (uint32_t u = 0; u < 4; ++u)
-        if (u < c.cnt && c.tt[u] != ~0u) val[c.tt[u]] = tv[u] + (-(l * uv[u]));
+        if (c.tt[u] != ~0u) val[c.tt[u]] = tv[u] + (-(l * uv[u]));
 }
 
 // Column tasks [c0, c0 + nc) of one level for a group of cooperating

@@ -721,7 +721,7 @@ constexpr int kMinWindow = 6;
 struct Window {
     int l0 = 0, l1 = 0;            // steps [l0, l1)
     std::vector<uint32_t> pos;     // slot -> CSC position
-    std::vector<uint4> cln;
+    std::vector<uint2> cln;
     std::vector<uint32_t> ctgt, cup;
     std::vector<uint32_t> ment;
     std::vector<MergeSrc> msrc;
@@ -775,9 +775,8 @@ void make_window(const Tables &T, int a, int b, const std::vector<uint32_t> &slo
         w.cup.push_back(slot(T.cup[q]) != UINT32_MAX ? slot(T.cup[q]) : 0u);
     }
     for (size_t q = (size_t)w.c0 * 64; q < (size_t)c1 * 64; ++q) {
-        uint4 ln = T.cln[q];
-        if (ln.z) ln = make_uint4(slot(ln.x), slot(ln.y), ln.z, 0u);
-        w.cln.push_back(ln);
+        const uint4 &ln = T.cln[q];
+        w.cln.push_back(ln.z ? make_uint2(slot(ln.x), slot(ln.y)) : make_uint2(0u, 0u));
     }
     w.m0 = T.ment_ptr[a];
     const uint32_t m1 = T.ment_ptr[b];
@@ -1206,7 +1205,9 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
     const size_t o_cdiag = U.put(P.csr_diag_ptr, n);
     const size_t o_upos = U.put(T.upos);
     const size_t o_cptr = U.put(T.ctask_ptr);
-    const size_t o_cln = U.put(T.cln);
+    std::vector<uint2> cln2(T.cln.size());   // {L position, pivot position} per lane
+    for (size_t q = 0; q < T.cln.size(); ++q) cln2[q] = make_uint2(T.cln[q].x, T.cln[q].y);
+    const size_t o_cln = U.put(cln2);
     const size_t o_ctgt = U.put(T.ctgt);
     const size_t o_cup = U.put(T.cup);
     const size_t o_ment = U.put(T.ment);
@@ -1281,7 +1282,7 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
     FactorDev &F = p->F;
     F.col_ptr = u32p(o_colptr);
     F.ctask_ptr = u32p(o_cptr);
-    F.cln = reinterpret_cast<const uint4 *>(A + o_cln);
+    F.cln = reinterpret_cast<const uint2 *>(A + o_cln);
     F.ctgt = u32p(o_ctgt);
     F.cup = u32p(o_cup);
     F.ment = u32p(o_ment);
@@ -1308,7 +1309,7 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
             d.l1 = x.l1;
             d.nslot = (uint32_t)x.pos.size();
             d.pos = u32p(o_wpos[w]);
-            d.cln = reinterpret_cast<const uint4 *>(shifted(o_wcln[w], (size_t)x.c0 * 64, sizeof(uint4)));
+            d.cln = reinterpret_cast<const uint2 *>(shifted(o_wcln[w], (size_t)x.c0 * 64, sizeof(uint2)));
             d.ctgt = reinterpret_cast<const uint32_t *>(shifted(o_wctgt[w], x.g0, sizeof(uint32_t)));
             d.cup = reinterpret_cast<const uint32_t *>(shifted(o_wcup[w], x.g0, sizeof(uint32_t)));
             d.ment = reinterpret_cast<const uint32_t *>(shifted(o_wment[w], x.m0, sizeof(uint32_t)));
