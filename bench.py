@@ -111,9 +111,11 @@ def bench_uneven(dev, log, tol, maxit):
     t0 = time.perf_counter()
     ans = analyze_many(mats, threads=threads)   # host analysis, one pattern per thread
     t_an = time.perf_counter() - t0
-    plans, axs, bs = [], [], []
-    for m, ((d, i, p, n), an) in enumerate(zip(mats, ans)):
-        plans.append(Plan(an))
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(threads) as ex:   # plan tables are built on the host, one pattern per thread
+        plans = list(ex.map(Plan, ans))
+    axs, bs = [], []
+    for m, (d, i, p, n) in enumerate(mats):
         axs.append(torch.from_numpy(d).to(dev))
         bs.append(torch.from_numpy(mna.rhs(n, 4500 + m)).to(dev))
     g = Group(plans)
@@ -381,17 +383,21 @@ def main():
         from oracle import ref
         if ref.available():
             threads = min(16, os.cpu_count() or 1)
-            # one probe refactor, then a sample sized to ~cpu_seconds
-            probe = ref.nicslu_time(data, indices, indptr, n, ax_h[:2], threads=threads)
-            per = max(probe["refactor_s"] + probe["solve_s"], 1e-6)
-            m = int(max(4, min(B, args.cpu_seconds / per)))
+            # passes over the batch's value sets until about cpu_seconds of
+            # refactor + solve time have accumulated
             t0 = time.perf_counter()
-            res = ref.nicslu_time(data, indices, indptr, n, ax_h[:m], threads=threads)
-            rate = 1.0 / (res["refactor_s"] + res["solve_s"])
+            spent, done, passes = 0.0, 0, 0
+            while spent < args.cpu_seconds and passes < 1000:
+                res = ref.nicslu_time(data, indices, indptr, n, ax_h, threads=threads)
+                spent += B * (res["refactor_s"] + res["solve_s"])
+                done += B
+                passes += 1
+            rate = done / spent
             cpu = {"value": round(rate, 2), "unit": "refactors/s", "cores": threads, "kind": "reference",
-                   "sample": f"NICSLU (src/nicslu, -O2) ReFactorize_MT + Solve, FP64, {m} of the {B} "
-                             f"batch matrices, {threads} threads, {time.perf_counter() - t0:.1f}s"}
-            log(f"[bench] NICSLU CPU: {rate:.1f} refactors/s on {threads} threads (sample {m})")
+                   "sample": f"NICSLU (src/nicslu, -O2) ReFactorize_MT + Solve, FP64, the {B} batch matrices "
+                             f"x {passes} passes ({done} refactors, {spent:.1f} s of refactor + solve), "
+                             f"{threads} threads, {time.perf_counter() - t0:.1f} s wall"}
+            log(f"[bench] NICSLU CPU: {rate:.1f} refactors/s on {threads} threads ({done} refactors)")
         else:
             log("[bench] oracle/_ref not built: no CPU baseline")
 
