@@ -55,6 +55,25 @@ def alg_bytes_refine(nnz_a, nnz_lu, n, share):
     return 12 * nnz_a + 8 * (n + 1) / share + 8 * nnz_lu + 56 * n
 
 
+def fwd_error(data, indices, indptr, n, b, x):
+    """Forward error |x - x*|_inf / |x*|_inf.  x* is an FP64 sparse direct
+    solve of the original matrix (scipy SuperLU, partial pivoting) refined
+    twice with residuals in the host's extended precision, so x* itself is
+    accurate to far below the errors measured."""
+    import numpy as np
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as sla
+    A = sp.csc_matrix((data, indices, indptr), shape=(n, n))
+    lu = sla.splu(A)
+    b64 = np.asarray(b, dtype=np.float64)
+    xs = lu.solve(b64)
+    Al = sp.csr_matrix(A).astype(np.longdouble)
+    for _ in range(2):
+        r = b64.astype(np.longdouble) - Al @ xs.astype(np.longdouble)
+        xs = xs + lu.solve(r.astype(np.float64))
+    return float(np.abs(np.asarray(x, dtype=np.float64) - xs).max() / np.abs(xs).max())
+
+
 def _events(fn, reps):
     """Median time (ms) of fn() over reps, HIP events on the current stream."""
     import torch
@@ -91,10 +110,15 @@ def bench_small(dev, log):
         step()
     ms = _events(step, 11)
     fms = _events(lambda: (lu.copy_(base), plan.factor(lu)), 11)
+    b64 = mna.rhs(n, 101)
+    xr, _, eta = plan.refine(ax, lu, torch.from_numpy(b64).to(dev), tol=1e-10, maxit=20)
     out = {"n": n, "nnz": int(d.size), "nnz_lu": an.nnz_lu, "levels": an.num_lev,
            "factor_solve_ms": round(ms, 4), "factor_ms": round(fms, 4),
-           "us_per_level": round(fms * 1e3 / max(an.num_lev, 1), 2)}
-    log(f"[bench] n=4960: factor+solve {ms:.3f} ms (factor {fms:.3f} ms, {an.num_lev} levels)")
+           "us_per_level": round(fms * 1e3 / max(an.num_lev, 1), 2),
+           "fwd_err_fp32": fwd_error(d, i, p, n, b64, x32.cpu().numpy()),
+           "fwd_err_refined": fwd_error(d, i, p, n, b64, xr.cpu().numpy()), "refined_eta": eta}
+    log(f"[bench] n=4960: factor+solve {ms:.3f} ms (factor {fms:.3f} ms, {an.num_lev} levels); forward error "
+        f"vs FP64: FP32 solve {out['fwd_err_fp32']:.1e}, refined {out['fwd_err_refined']:.1e}")
     plan.close()
     return out
 
@@ -266,6 +290,8 @@ def main():
         r = b_h[b] - A @ xb
         e = np.abs(r).max() / (np.abs(A).sum(axis=1).max() * np.abs(xb).max() + np.abs(b_h[b]).max())
         ok = bool(ok and e <= 2 * args.tol)
+    fwd = max(fwd_error(ax_h[b], indices, indptr, n, b_h[b], x[b].cpu().numpy()) for b in (0, B // 2, B - 1))
+    log(f"[bench] batch forward error vs FP64 (matrices 0, {B // 2}, {B - 1}): {fwd:.2e}")
     value = B * K / wall
     log(f"[bench] batch: {wall / K * 1e3:.3f} ms/step (events {kernel_ms:.3f} ms) -> {value:.1f} refactors/s; "
         f"iters {iters_h.min()}..{iters_h.max()} max eta {eta_h.max():.2e} ok={ok}")
@@ -420,6 +446,7 @@ def main():
                    "parallelism": "one workgroup per matrix"},
         "correct": ok,
         "refine_iters_max": int(iters_h.max()),
+        "fwd_err_max_sample": fwd,
         "eta_max": float(eta_h.max()),
         "plain_fp32_loop": {"value": round(loop_rate, 2), "unit": "refactors/s", "sample": S,
                             "single_factor_ms": round(single_factor_ms, 4),

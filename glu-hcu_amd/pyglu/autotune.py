@@ -1,10 +1,17 @@
-"""Autotuner for the two schedule parameters of a plan.
+"""Autotuner for the schedule parameters of a plan.
 
 * ``fuse_units`` — level fusion: a factorization level needing at most this
   many lanes runs inside one workgroup (one barrier per level) instead of
   its own grid launch; solve levels fuse at 4 rows per thread.
 * ``block_threads`` — threads of the fused workgroup and of the batched
   per-matrix workgroup (the lanes that share one level's items).
+* ``lds_windows`` — how many runs of steps are factored with their values
+  staged in LDS (counted from the last step; -1 = none).
+* ``krylov`` (batched solve) — the FGMRES restart length of the refinement
+  (-1 = plain refinement).
+
+None of these changes a result: factors are bit-identical under every
+schedule (only the order of independent work changes).
 
 Each candidate plan is timed on the device with HIP events over the real
 workload (single-matrix factorization, or one batched launch) and the
@@ -24,12 +31,23 @@ FUSE_CANDIDATES = (256, 512, 1024, 2048, 4096, 8192, 16384)
 THREAD_CANDIDATES = (256, 512, 1024)
 
 
+WINDOW_CANDIDATES = (0, -1, 2)
+KRYLOV_CANDIDATES = (2, -1, 1, 4)
+
+
 @dataclass
 class TuneResult:
     fuse_units: int
     block_threads: int
     ms: float
     table: list = field(default_factory=list)
+    lds_windows: int = 0
+    krylov: int = 0
+
+    def plan(self, analysis):
+        """A plan with the chosen parameters."""
+        return Plan(analysis, fuse_units=self.fuse_units, block_threads=self.block_threads,
+                    lds_windows=self.lds_windows, krylov=self.krylov)
 
 
 def _time(fn, reps=5, warmup=2):
@@ -45,47 +63,75 @@ def _time(fn, reps=5, warmup=2):
     return s.elapsed_time(e) / reps
 
 
-def tune_factor(analysis, values=None, fuse=FUSE_CANDIDATES, threads=THREAD_CANDIDATES, reps=5):
-    """Best (fuse_units, block_threads) for the single-matrix factorization."""
+def tune_factor(analysis, values=None, fuse=FUSE_CANDIDATES, threads=THREAD_CANDIDATES,
+                windows=WINDOW_CANDIDATES, reps=5):
+    """Best (fuse_units, block_threads, lds_windows) for the single-matrix
+    factorization: every combination of fuse and threads at the default
+    window count, then the window counts at the best of those."""
     table = []
     base = torch.from_numpy(analysis.val.copy()).cuda() if values is None else values
     best = None
+
+    def measure(f, t, wnd):
+        plan = Plan(analysis, fuse_units=f, block_threads=t, lds_windows=wnd)
+        lu = base.clone()
+
+        def run():
+            lu.copy_(base)
+            plan.factor(lu)
+
+        ms = _time(run, reps)
+        table.append((f, t, plan.segments, ms, wnd))
+        plan.close()
+        return ms
+
     for t in threads:
         for f in fuse:
-            plan = Plan(analysis, fuse_units=f, block_threads=t)
-            lu = base.clone()
-
-            def run():
-                lu.copy_(base)
-                plan.factor(lu)
-
-            ms = _time(run, reps)
-            table.append((f, t, plan.segments, ms))
+            ms = measure(f, t, 0)
             if best is None or ms < best[2]:
-                best = (f, t, ms)
-            plan.close()
-    return TuneResult(best[0], best[1], best[2], table)
+                best = (f, t, ms, 0)
+    for wnd in windows:
+        if wnd == 0:
+            continue
+        ms = measure(best[0], best[1], wnd)
+        if ms < best[2]:
+            best = (best[0], best[1], ms, wnd)
+    return TuneResult(best[0], best[1], best[2], table, lds_windows=best[3])
 
 
-def tune_batch(analysis, ax, b, tol=1e-10, maxit=10, threads=THREAD_CANDIDATES, reps=3):
-    """Best block_threads for the batched launch (ax: (B, nnz_a), b: (B, n))."""
+def tune_batch(analysis, ax, b, tol=1e-10, maxit=10, threads=THREAD_CANDIDATES, krylov=KRYLOV_CANDIDATES,
+               reps=3):
+    """Best (block_threads, krylov) for the batched launch (ax: (B, nnz_a),
+    b: (B, n)): the thread counts at the default refinement, then the
+    restart lengths at the best thread count."""
     table = []
     best = None
-    for t in threads:
-        plan = Plan(analysis, block_threads=t)
+
+    def measure(t, m):
+        plan = Plan(analysis, block_threads=t, krylov=m)
         ms = _time(lambda: plan.batch_solve(ax, b, tol=tol, maxit=maxit), reps, warmup=1)
-        table.append((0, t, 0, ms))
-        if best is None or ms < best[2]:
-            best = (0, t, ms)
+        table.append((0, t, 0, ms, m))
         plan.close()
-    return TuneResult(best[0], best[1], best[2], table)
+        return ms
+
+    for t in threads:
+        ms = measure(t, 0)
+        if best is None or ms < best[2]:
+            best = (0, t, ms, 0)
+    for m in krylov:
+        ms = measure(best[1], m)
+        if ms < best[2]:
+            best = (0, best[1], ms, m)
+    return TuneResult(best[0], best[1], best[2], table, krylov=best[3])
 
 
 def report(res: TuneResult) -> str:
-    lines = [f"{'fuse_units':>10} {'threads':>8} {'segments':>9} {'ms':>10}"]
-    for f, t, s, ms in res.table:
-        mark = " *" if (f, t) == (res.fuse_units, res.block_threads) else ""
-        lines.append(f"{f:>10} {t:>8} {s:>9} {ms:>10.4f}{mark}")
+    lines = [f"{'fuse_units':>10} {'threads':>8} {'segments':>9} {'ms':>10} {'extra':>6}"]
+    best = min(res.table, key=lambda r: r[3]) if res.table else None
+    for row in res.table:
+        f, t, s, ms, x = row
+        mark = " *" if row is best else ""
+        lines.append(f"{f:>10} {t:>8} {s:>9} {ms:>10.4f} {x:>6}{mark}")
     return "\n".join(lines)
 
 

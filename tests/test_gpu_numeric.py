@@ -345,3 +345,33 @@ def test_abft_device_matches_oracle(torch_mod):
         f, _, m = O.abft_check(pat, luh[b], want)
         assert int(fh[b]) == f and mh[b] == np.float32(m), (b, int(fh[b]), f, mh[b], m)
     assert int(fh[0]) == -1 and int(fh[2]) == -1 and int(fh[1]) == c
+
+
+def test_autotune(torch_mod):
+    """The autotuner times every candidate, returns the fastest, and a plan
+    built from its choice gives the default plan's factors bit for bit (the
+    schedule parameters never change results)."""
+    from pyglu import mna
+    from pyglu.autotune import report, tune_batch, tune_factor
+    from pyglu.device import Plan
+    torch = torch_mod
+    data, indices, indptr, n, an, plan, pat = _setup("mna2k")
+    res = tune_factor(an, fuse=(256, 2048), threads=(512, 1024), windows=(0, -1), reps=2)
+    print(report(res), flush=True)
+    assert len(res.table) == 5 and res.ms == min(r[3] for r in res.table)
+    lu0 = torch.from_numpy(an.val.copy()).cuda()
+    plan.factor(lu0)
+    tuned = res.plan(an)
+    lu1 = torch.from_numpy(an.val.copy()).cuda()
+    tuned.factor(lu1)
+    torch.cuda.synchronize()
+    assert torch.equal(lu0.view(torch.int32), lu1.view(torch.int32))
+    B = 4
+    axd = torch.from_numpy(np.stack([mna.perturb_values(data, 3200 + b) for b in range(B)])).cuda()
+    bd = torch.from_numpy(np.stack([mna.rhs(n, 3600 + b) for b in range(B)])).cuda()
+    rb = tune_batch(an, axd, bd, threads=(512, 1024), krylov=(-1, 1), reps=1)
+    print(report(rb), flush=True)
+    assert len(rb.table) == 4 and rb.ms == min(r[3] for r in rb.table)
+    x, lu, iters, eta = rb.plan(an).batch_solve(axd, bd, tol=1e-10, maxit=20)
+    torch.cuda.synchronize()
+    assert np.all(eta.cpu().numpy() <= 1e-10)
