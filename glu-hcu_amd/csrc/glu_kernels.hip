@@ -25,6 +25,7 @@
 // host solve do; packed FP32 disabled.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "glu_dev.h"
@@ -1707,6 +1708,53 @@ __global__ void __launch_bounds__(1024) k_batch_same_refine(const MatDesc *__res
     wg_refine<kMode>(desc, io, tol, maxit, sh, hs, xs, flags);
 }
 
+// One matrix's factors into the sweeps' layout (grid): lucsr[df_dst[p]] =
+// lu[p]; padding slots keep the zeros they were allocated with.
+__global__ void __launch_bounds__(256) k_copy_df(const MatDesc *__restrict__ desc, const float *__restrict__ lu,
+                                                 float *__restrict__ lucsr)
+{
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t *dst = glb(desc->s.df_dst);
+    if (p < desc->nnz) {
+        const uint32_t q = dst[p];
+        if (q != ~0u) lucsr[q] = lu[p];
+    }
+    if (p < desc->s.df_nextra) {
+        const uint2 e = glb(desc->s.df_extra)[p];
+        lucsr[e.x] = lu[e.y];
+    }
+}
+
+// Refinement of one factored matrix (glu_refine): x = 0, r = b, |b|_inf,
+// then wg_refine in this one workgroup.  Dynamic LDS as k_batch_same_refine.
+template <int kMode>
+__global__ void __launch_bounds__(1024) k_refine_one(const MatDesc *__restrict__ desc,
+                                                     const double *__restrict__ ax, const double *__restrict__ b,
+                                                     double *__restrict__ x, const float *__restrict__ lu,
+                                                     float *__restrict__ lucsr, float *__restrict__ t32,
+                                                     double *__restrict__ r64, double *__restrict__ norms,
+                                                     double *__restrict__ kry, int32_t *__restrict__ iters,
+                                                     double *__restrict__ eta, double tol, int maxit)
+{
+    __shared__ double sh[1024 / kWave];
+    __shared__ double hs[kKrylovLds];
+    const MatIO io = batch_io(desc, ax, b, x, const_cast<float *>(lu), lucsr, t32, r64, norms, kry, iters, eta);
+    const uint32_t n = desc->n;
+    double bmax = 0.0;
+    for (uint32_t c = threadIdx.x; c < n; c += blockDim.x) {
+        bmax = nmax(bmax, fabs(io.b[c]));
+        io.x[c] = 0.0;
+        io.r64[c] = io.b[c];
+    }
+    const double bnorm = wg_max(bmax, sh);
+    if (threadIdx.x == 0) io.norms[1] = bnorm;
+    __syncthreads();
+    uint32_t *flags;
+    float *xs;
+    batch_lds<kMode>(n, io.t32, flags, xs);
+    wg_refine<kMode>(desc, io, tol, maxit, sh, hs, xs, flags);
+}
+
 // Distinct patterns: workgroup g runs matrix order[g] with its own tables.
 template <bool kCopy>
 __global__ void __launch_bounds__(1024) k_batch_group_factor(const MatDesc *__restrict__ descs,
@@ -1904,6 +1952,21 @@ void batch_same(const MatDesc *desc, int batch, const double *ax, const double *
                                                                             norms, kry, iters, eta, tol, maxit);
 }
 
+// glu_refine in one workgroup: the factors into the sweeps' layout, then
+// the refinement (mode: kSweepLds or kSweepFlags).
+void refine_one(const MatDesc *desc, uint32_t nnz, uint32_t nextra, const double *ax, const double *b, double *x,
+                const float *lu, float *lucsr, float *t32, double *r64, double *norms, double *kry, int32_t *iters,
+                double *eta, double tol, int maxit, int threads, int mode, size_t ref_lds, hipStream_t st)
+{
+    k_copy_df<<<blocks_for(std::max(nnz, nextra), 256), 256, 0, st>>>(desc, lu, lucsr);
+    if (mode == kSweepLds)
+        k_refine_one<kSweepLds><<<1, threads, ref_lds, st>>>(desc, ax, b, x, lu, lucsr, t32, r64, norms, kry, iters,
+                                                             eta, tol, maxit);
+    else
+        k_refine_one<kSweepFlags><<<1, threads, ref_lds, st>>>(desc, ax, b, x, lu, lucsr, t32, r64, norms, kry,
+                                                               iters, eta, tol, maxit);
+}
+
 void abft_cca(const FactorDev &F, const float *val, float *cca, int batch, hipStream_t st)
 {
     k_abft_cca<<<dim3(blocks_for(F.n, 256), batch), 256, 0, st>>>(F.col_ptr, val, cca, F.n, F.nnz);
@@ -1970,6 +2033,8 @@ size_t enable_lds()
         reinterpret_cast<const void *>(&k_batch_same_factor<true>),
         reinterpret_cast<const void *>(&k_batch_same_factor<false>),
         reinterpret_cast<const void *>(&k_batch_factor),
+        reinterpret_cast<const void *>(&k_refine_one<kSweepLds>),
+        reinterpret_cast<const void *>(&k_refine_one<kSweepFlags>),
         reinterpret_cast<const void *>(&k_batch_same_refine<kSweepLds>),
         reinterpret_cast<const void *>(&k_batch_same_refine<kSweepFlags>),
         reinterpret_cast<const void *>(&k_batch_same_refine<kSweepLevels>),

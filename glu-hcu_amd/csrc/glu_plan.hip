@@ -63,6 +63,9 @@ size_t enable_lds();
 void factor_window(const FactorDev &F, int w, uint32_t nslot, float *val, const float *d_pert, int perturb,
                    int threads, hipStream_t st);
 void zero_u64(unsigned long long *p, int count, hipStream_t st);
+void refine_one(const MatDesc *desc, uint32_t nnz, uint32_t nextra, const double *ax, const double *b, double *x,
+                const float *lu, float *lucsr, float *t32, double *r64, double *norms, double *kry, int32_t *iters,
+                double *eta, double tol, int maxit, int threads, int mode, size_t ref_lds, hipStream_t st);
 void adjoint_values(const RefineDev &R, const double *lam, const double *x, double *grad, uint32_t n,
                     hipStream_t st);
 size_t matio_size();
@@ -339,6 +342,7 @@ struct glu_plan {
     double *d_br64 = nullptr;
     float *d_blucsr = nullptr;   // sweep value copies, df_nval per matrix
     double *d_bnorms = nullptr;  // |A|_inf, |b|_inf per matrix
+    char *d_rout = nullptr;      // glu_refine's iteration count and backward error (device)
     double *d_bkry = nullptr;    // Arnoldi vectors V, Z = M V, W = A Z: (3 krylov + 1) n per matrix
     int batch_cap = 0;
 
@@ -350,6 +354,7 @@ struct glu_plan {
         if (d_br64) (void)hipFree(d_br64);
         if (d_blucsr) (void)hipFree(d_blucsr);
         if (d_bkry) (void)hipFree(d_bkry);
+        if (d_rout) (void)hipFree(d_rout);
         if (d_abft) (void)hipFree(d_abft);
         if (d_bnorms) (void)hipFree(d_bnorms);
         for (GraphSlot *g : {&g_factor, &g_solve[0], &g_solve[1]})
@@ -1662,6 +1667,32 @@ extern "C" glu_status glu_refine(glu_plan *p, const double *d_ax, const float *d
     hipStream_t st = static_cast<hipStream_t>(stream);
     const SolveDev &S = p->S;
     const uint32_t n = p->n;
+    if (!transpose) {
+        // one workgroup: the batch pipeline's dataflow sweeps and FGMRES
+        // (plan.krylov), when the sweeps' ready bits fit in LDS
+        const size_t cap = launch::enable_lds();
+        const int mode = sweep_mode(n, cap);
+        if (mode != 0) {
+            glu_status s = ensure_batch(p, 1);
+            if (s != GLU_OK) return s;
+            if (!p->d_rout) HIP_TRY(hipMalloc(&p->d_rout, 16));
+            int32_t *d_it = reinterpret_cast<int32_t *>(p->d_rout);
+            double *d_eta = reinterpret_cast<double *>(p->d_rout + 8);
+            launch::refine_one(p->d_desc, p->nnz, p->df_nextra, d_ax, d_b, d_x, d_lu, p->d_blucsr, p->d_bt32,
+                               p->d_br64, p->d_bnorms, p->d_bkry, d_it, d_eta, tol, maxit, p->fused_threads, mode,
+                               launch::batch_lds_bytes(mode, n), st);
+            HIP_TRY(hipGetLastError());
+            int32_t it = 0;
+            double eta = 0.0;
+            HIP_TRY(hipMemcpyAsync(&it, d_it, sizeof it, hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipMemcpyAsync(&eta, d_eta, sizeof eta, hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipStreamSynchronize(st));
+            if (h_iters) *h_iters = it;
+            if (h_eta) *h_eta = eta;
+            if (!(eta == eta)) return fail(GLU_ERR_NUMERIC, "refinement produced NaN");
+            return GLU_OK;
+        }
+    }
     HIP_TRY(hipMemsetAsync(d_x, 0, sizeof(double) * n, st));
     HIP_TRY(hipMemcpyAsync(p->d_r64, d_b, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
     double anorm = 0.0, bnorm = 0.0, eta = 0.0;
