@@ -153,6 +153,12 @@ struct SolveDev {
     const uint32_t *df_dst;    // value copy: slot of CSC position p in it (~0u: none) [nnz_lu]
     const uint2 *df_extra;     // (slot, position) of positions that take a second slot [df_nextra]
     uint32_t df_nval, df_nextra;   // slots per matrix (padding slots hold 0); extra pairs
+    // A' x = b over a second value copy (U' slots, U' pivots, L' slots),
+    // the same kind of tables
+    DfDev tup_df, tlo_df;
+    const uint32_t *tdf_dst;
+    const uint2 *tdf_extra;
+    uint32_t tdf_nval, tdf_nextra;
     const uint32_t *row_perm, *row_perm_inv, *col_perm, *col_perm_inv;
     const double *row_scale, *col_scale_perm;
 };

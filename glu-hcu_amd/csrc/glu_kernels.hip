@@ -1406,17 +1406,24 @@ __device__ double wg_sum(double v, double *sh)
 // col_scale_perm[j] rounded to FP32, then the FP32 L and U sweeps in place
 // (Symbolic_Matrix::solve's permutation and scaling, symbolic.cc:215-239).
 // The caller maps xs back with row_perm_inv / row_scale.
-template <int kMode>
+template <int kMode, bool kT = false>
 __device__ __forceinline__ void wg_precond(const MatDesc *Md, const MatIO &io, const double *v, float *xs,
                                            uint32_t *flags)
 {
     const uint32_t n = Md->n;
     const uint32_t tid = threadIdx.x, nt = blockDim.x;
     const uint32_t nwords = (n + 31) / 32;
-    {
+    if (!kT) {
         const uint32_t *cp = glb(Md->s.col_perm);
         const double *csp = glb(Md->s.col_scale_perm);
         for (uint32_t j = tid; j < n; j += nt) xs[j] = (float)(v[cp[j]] * csp[j]);
+    } else {   // A' x = b: rows by row_perm, scaled by row_scale
+        const uint32_t *rp = glb(Md->s.row_perm);
+        const double *rs = glb(Md->s.row_scale);
+        for (uint32_t j = tid; j < n; j += nt) {
+            const uint32_t pj = rp[j];
+            xs[j] = (float)(v[pj] * rs[pj]);
+        }
     }
     if (kMode == kSweepLevels) {
         __syncthreads();
@@ -1429,7 +1436,7 @@ __device__ __forceinline__ void wg_precond(const MatDesc *Md, const MatIO &io, c
         for (uint32_t i = tid; i < nwords; i += nt) flags[i] = 0u;
         __syncthreads();
         {
-            DfDev lo = Md->s.lo_df;
+            DfDev lo = kT ? Md->s.tup_df : Md->s.lo_df;
             globalize(lo);
             df2_sweep(lo, io.lucsr + lo.voff, xs, flags);
         }
@@ -1437,7 +1444,7 @@ __device__ __forceinline__ void wg_precond(const MatDesc *Md, const MatIO &io, c
         for (uint32_t i = tid; i < nwords; i += nt) flags[i] = 0u;
         __syncthreads();
         {
-            DfDev up = Md->s.up_df;
+            DfDev up = kT ? Md->s.tlo_df : Md->s.up_df;
             globalize(up);
             df2_sweep(up, io.lucsr + up.voff, xs, flags);
         }
@@ -1447,8 +1454,18 @@ __device__ __forceinline__ void wg_precond(const MatDesc *Md, const MatIO &io, c
 
 // r = b - A x in FP64 (into io.r64); returns the backward error
 // eta = |r|_inf / (|A|_inf |x|_inf + |b|_inf) to every thread.
+// Rows of A (kT: of A', the caller's CSC columns) for one workgroup.
+template <bool kT, typename F>
+__device__ __forceinline__ void wg_rows(const RefineDev &R, const double *__restrict__ ax,
+                                        const double *__restrict__ x, uint32_t n, F f)
+{
+    if (!kT) wg_rows_of_a(R, ax, x, n, f);
+    else for_rows_of_a(R, ax, x, n, 1, threadIdx.x, blockDim.x, f);
+}
+
 // anorm < 0: |A|_inf is taken from this product's row sums (and returned
 // in anorm).
+template <bool kT = false>
 __device__ __forceinline__ double wg_residual(const MatDesc *Md, const MatIO &io, double &anorm, double bnorm,
                                               double *sh)
 {
@@ -1457,7 +1474,7 @@ __device__ __forceinline__ double wg_residual(const MatDesc *Md, const MatIO &io
     double rmax = 0.0, xmax = 0.0, amax = 0.0;
     RefineDev R = Md->r;
     globalize(R);
-    wg_rows_of_a(R, io.ax, io.x, n, [&](uint32_t i, double dot, double asum) {
+    wg_rows<kT>(R, io.ax, io.x, n, [&](uint32_t i, double dot, double asum) {
         const double ri = io.b[i] - dot;
         io.r64[i] = ri;
         rmax = nmax(rmax, fabs(ri));
@@ -1496,7 +1513,7 @@ constexpr int kKrylovLds = kKrylovD + kMaxKrylov;
 //    checked after every solve, so a matrix stops as soon as it meets tol.
 // Every workgroup-wide sum has a fixed order: results do not depend on
 // timing.  xs is the FP32 sweep vector; flags the sweeps' ready bits.
-template <int kMode>
+template <int kMode, bool kT = false>
 __device__ __forceinline__ void wg_refine(const MatDesc *Md, const MatIO &io, double tol, int maxit, double *sh,
                                           double *hs, float *xs, uint32_t *flags)
 {
@@ -1505,17 +1522,21 @@ __device__ __forceinline__ void wg_refine(const MatDesc *Md, const MatIO &io, do
     const double bnorm = io.norms[1];
     double anorm = -1.0;   // |A|_inf: from the first product with A
     const int m = io.kry ? (int)Md->krylov : 0;
-    const uint32_t *rpi = glb(Md->s.row_perm_inv);
-    const double *rs = glb(Md->s.row_scale);
+    // back from the factors' frame: x[i] += xs[bp[i]] * bs[kT ? bp[i] : i]
+    const uint32_t *bp = glb(kT ? Md->s.col_perm_inv : Md->s.row_perm_inv);
+    const double *bs = glb(kT ? Md->s.col_scale_perm : Md->s.row_scale);
     int it = 0;
     double eta = 0.0;
     if (m == 0) {
         for (;;) {
-            wg_precond<kMode>(Md, io, io.r64, xs, flags);
+            wg_precond<kMode, kT>(Md, io, io.r64, xs, flags);
             ++it;
-            for (uint32_t i = tid; i < n; i += nt) io.x[i] = io.x[i] + (double)xs[rpi[i]] * rs[i];
+            for (uint32_t i = tid; i < n; i += nt) {
+                const uint32_t pi = bp[i];
+                io.x[i] = io.x[i] + (double)xs[pi] * bs[kT ? pi : i];
+            }
             __syncthreads();
-            eta = wg_residual(Md, io, anorm, bnorm, sh);
+            eta = wg_residual<kT>(Md, io, anorm, bnorm, sh);
             if (!(eta > tol) || it >= maxit) break;   // also stops on NaN
         }
     } else {
@@ -1541,15 +1562,18 @@ __device__ __forceinline__ void wg_refine(const MatDesc *Md, const MatIO &io, do
             __syncthreads();
             for (int j = 0; j < m; ++j) {
                 double *zj = Z + (size_t)j * n, *wj = W + (size_t)j * n, *vn = V + (size_t)(j + 1) * n;
-                wg_precond<kMode>(Md, io, V + (size_t)j * n, xs, flags);
+                wg_precond<kMode, kT>(Md, io, V + (size_t)j * n, xs, flags);
                 ++it;
-                for (uint32_t i = tid; i < n; i += nt) zj[i] = (double)xs[rpi[i]] * rs[i];
+                for (uint32_t i = tid; i < n; i += nt) {
+                    const uint32_t pi = bp[i];
+                    zj[i] = (double)xs[pi] * bs[kT ? pi : i];
+                }
                 __syncthreads();
                 {
                     RefineDev R = Md->r;
                     globalize(R);
                     double amax = 0.0;
-                    wg_rows_of_a(R, io.ax, zj, n, [&](uint32_t i, double dot, double asum) {
+                    wg_rows<kT>(R, io.ax, zj, n, [&](uint32_t i, double dot, double asum) {
                         wj[i] = dot;
                         vn[i] = dot;
                         amax = nmax(amax, asum);
@@ -1710,24 +1734,26 @@ __global__ void __launch_bounds__(1024) k_batch_same_refine(const MatDesc *__res
 
 // One matrix's factors into the sweeps' layout (grid): lucsr[df_dst[p]] =
 // lu[p]; padding slots keep the zeros they were allocated with.
+template <bool kT>
 __global__ void __launch_bounds__(256) k_copy_df(const MatDesc *__restrict__ desc, const float *__restrict__ lu,
                                                  float *__restrict__ lucsr)
 {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t *dst = glb(desc->s.df_dst);
+    const uint32_t *dst = glb(kT ? desc->s.tdf_dst : desc->s.df_dst);
     if (p < desc->nnz) {
         const uint32_t q = dst[p];
         if (q != ~0u) lucsr[q] = lu[p];
     }
-    if (p < desc->s.df_nextra) {
-        const uint2 e = glb(desc->s.df_extra)[p];
+    if (p < (kT ? desc->s.tdf_nextra : desc->s.df_nextra)) {
+        const uint2 e = glb(kT ? desc->s.tdf_extra : desc->s.df_extra)[p];
         lucsr[e.x] = lu[e.y];
     }
 }
 
-// Refinement of one factored matrix (glu_refine): x = 0, r = b, |b|_inf,
-// then wg_refine in this one workgroup.  Dynamic LDS as k_batch_same_refine.
-template <int kMode>
+// Refinement of one factored matrix (glu_refine; kT: A' x = b): x = 0,
+// r = b, |b|_inf, then wg_refine in this one workgroup.  Dynamic LDS as
+// k_batch_same_refine.
+template <int kMode, bool kT>
 __global__ void __launch_bounds__(1024) k_refine_one(const MatDesc *__restrict__ desc,
                                                      const double *__restrict__ ax, const double *__restrict__ b,
                                                      double *__restrict__ x, const float *__restrict__ lu,
@@ -1752,7 +1778,7 @@ __global__ void __launch_bounds__(1024) k_refine_one(const MatDesc *__restrict__
     uint32_t *flags;
     float *xs;
     batch_lds<kMode>(n, io.t32, flags, xs);
-    wg_refine<kMode>(desc, io, tol, maxit, sh, hs, xs, flags);
+    wg_refine<kMode, kT>(desc, io, tol, maxit, sh, hs, xs, flags);
 }
 
 // Distinct patterns: workgroup g runs matrix order[g] with its own tables.
@@ -1956,15 +1982,20 @@ void batch_same(const MatDesc *desc, int batch, const double *ax, const double *
 // the refinement (mode: kSweepLds or kSweepFlags).
 void refine_one(const MatDesc *desc, uint32_t nnz, uint32_t nextra, const double *ax, const double *b, double *x,
                 const float *lu, float *lucsr, float *t32, double *r64, double *norms, double *kry, int32_t *iters,
-                double *eta, double tol, int maxit, int threads, int mode, size_t ref_lds, hipStream_t st)
+                double *eta, double tol, int maxit, int threads, int mode, size_t ref_lds, int transpose,
+                hipStream_t st)
 {
-    k_copy_df<<<blocks_for(std::max(nnz, nextra), 256), 256, 0, st>>>(desc, lu, lucsr);
-    if (mode == kSweepLds)
-        k_refine_one<kSweepLds><<<1, threads, ref_lds, st>>>(desc, ax, b, x, lu, lucsr, t32, r64, norms, kry, iters,
-                                                             eta, tol, maxit);
-    else
-        k_refine_one<kSweepFlags><<<1, threads, ref_lds, st>>>(desc, ax, b, x, lu, lucsr, t32, r64, norms, kry,
-                                                               iters, eta, tol, maxit);
+    const unsigned grid = blocks_for(std::max(nnz, nextra), 256);
+    if (transpose) k_copy_df<true><<<grid, 256, 0, st>>>(desc, lu, lucsr);
+    else k_copy_df<false><<<grid, 256, 0, st>>>(desc, lu, lucsr);
+#define GLU_REFINE_ONE(M, T)                                                                                   \
+    k_refine_one<M, T><<<1, threads, ref_lds, st>>>(desc, ax, b, x, lu, lucsr, t32, r64, norms, kry, iters, eta, \
+                                                    tol, maxit)
+    if (mode == kSweepLds && !transpose) GLU_REFINE_ONE(kSweepLds, false);
+    else if (mode == kSweepLds) GLU_REFINE_ONE(kSweepLds, true);
+    else if (!transpose) GLU_REFINE_ONE(kSweepFlags, false);
+    else GLU_REFINE_ONE(kSweepFlags, true);
+#undef GLU_REFINE_ONE
 }
 
 void abft_cca(const FactorDev &F, const float *val, float *cca, int batch, hipStream_t st)
@@ -2033,8 +2064,10 @@ size_t enable_lds()
         reinterpret_cast<const void *>(&k_batch_same_factor<true>),
         reinterpret_cast<const void *>(&k_batch_same_factor<false>),
         reinterpret_cast<const void *>(&k_batch_factor),
-        reinterpret_cast<const void *>(&k_refine_one<kSweepLds>),
-        reinterpret_cast<const void *>(&k_refine_one<kSweepFlags>),
+        reinterpret_cast<const void *>(&k_refine_one<kSweepLds, false>),
+        reinterpret_cast<const void *>(&k_refine_one<kSweepFlags, false>),
+        reinterpret_cast<const void *>(&k_refine_one<kSweepLds, true>),
+        reinterpret_cast<const void *>(&k_refine_one<kSweepFlags, true>),
         reinterpret_cast<const void *>(&k_batch_same_refine<kSweepLds>),
         reinterpret_cast<const void *>(&k_batch_same_refine<kSweepFlags>),
         reinterpret_cast<const void *>(&k_batch_same_refine<kSweepLevels>),

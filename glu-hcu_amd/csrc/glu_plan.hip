@@ -65,7 +65,8 @@ void factor_window(const FactorDev &F, int w, uint32_t nslot, float *val, const 
 void zero_u64(unsigned long long *p, int count, hipStream_t st);
 void refine_one(const MatDesc *desc, uint32_t nnz, uint32_t nextra, const double *ax, const double *b, double *x,
                 const float *lu, float *lucsr, float *t32, double *r64, double *norms, double *kry, int32_t *iters,
-                double *eta, double tol, int maxit, int threads, int mode, size_t ref_lds, hipStream_t st);
+                double *eta, double tol, int maxit, int threads, int mode, size_t ref_lds, int transpose,
+                hipStream_t st);
 void adjoint_values(const RefineDev &R, const double *lam, const double *x, double *grad, uint32_t n,
                     hipStream_t st);
 size_t matio_size();
@@ -301,6 +302,8 @@ struct glu_plan {
     size_t lds_cap = 0;          // dynamic LDS per workgroup (bytes)
     uint32_t df_nval = 0;        // values of the per-matrix sweep copy
     uint32_t df_nextra = 0;      // its positions with a second slot
+    uint32_t tdf_nval = 0, tdf_nextra = 0;   // the same for A' x = b
+    float *d_tlucsr = nullptr;   // glu_refine's A' x = b value copy
     int max_windows = 1;         // LDS windows allowed (from the last level)
     int krylov = kDefaultKrylov;  // batch refinement: FGMRES restart length (0: plain)
     int nwin = 0;                // LDS windows of the factorization
@@ -355,6 +358,7 @@ struct glu_plan {
         if (d_blucsr) (void)hipFree(d_blucsr);
         if (d_bkry) (void)hipFree(d_bkry);
         if (d_rout) (void)hipFree(d_rout);
+        if (d_tlucsr) (void)hipFree(d_tlucsr);
         if (d_abft) (void)hipFree(d_abft);
         if (d_bnorms) (void)hipFree(d_bnorms);
         for (GraphSlot *g : {&g_factor, &g_solve[0], &g_solve[1]})
@@ -1120,25 +1124,50 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
         for (uint32_t q = P.csr_r_ptr[r + 1]; q > P.csr_diag_ptr[r] + 1; --q)
             out.push_back(make_uint2(P.csr_c_idx[q - 1], T.upos[q - 1]));
     }, true, [&](uint32_t r) { return P.l_col_ptr[r]; }, dup);
+    // A' x = b: rows of U' (columns of U, pivot last) forward, rows of L'
+    // (columns of L) backward
+    DfHost dtu, dtl;
+    build_df(p->tri[2], [&](uint32_t r, std::vector<uint2> &out) {
+        for (uint32_t q = P.sym_c_ptr[r]; q < P.l_col_ptr[r]; ++q) out.push_back(make_uint2(P.sym_r_idx[q], q));
+    }, true, [&](uint32_t r) { return P.l_col_ptr[r]; }, dtu);
+    build_df(p->tri[3], [&](uint32_t r, std::vector<uint2> &out) {
+        for (uint32_t q = P.sym_c_ptr[r + 1]; q > P.l_col_ptr[r] + 1; --q)
+            out.push_back(make_uint2(P.sym_r_idx[q - 1], q - 1));
+    }, false, [&](uint32_t) { return 0u; }, dtl);
+    // the copy runs in CSC order (contiguous reads, scattered stores): the
+    // slot of each position, plus pairs for positions with a second slot;
+    // padding slots are zeroed once when the copies are allocated
+    auto copy_map = [&](const std::vector<uint32_t> &vpos, std::vector<uint32_t> &dst, std::vector<uint2> &extra,
+                        uint32_t &nextra) {
+        dst.assign(P.nnz, UINT32_MAX);
+        extra.clear();
+        for (uint32_t q = 0; q < (uint32_t)vpos.size(); ++q) {
+            const uint32_t pos = vpos[q];
+            if (pos == UINT32_MAX) continue;
+            if (dst[pos] == UINT32_MAX) dst[pos] = q;
+            else extra.push_back(make_uint2(q, pos));
+        }
+        nextra = (uint32_t)extra.size();
+        if (extra.empty()) extra.push_back(make_uint2(UINT32_MAX, 0u));   // keeps the table non-empty
+    };
+    std::vector<uint32_t> tdf_vpos(dtu.vpos);   // U' slots, U' pivots (+64 pad), L' slots
+    tdf_vpos.insert(tdf_vpos.end(), dtu.pvpos.begin(), dtu.pvpos.end());
+    tdf_vpos.resize(tdf_vpos.size() + 64, UINT32_MAX);
+    const uint32_t tlo_voff = (uint32_t)tdf_vpos.size();
+    tdf_vpos.insert(tdf_vpos.end(), dtl.vpos.begin(), dtl.vpos.end());
+    p->tdf_nval = (uint32_t)tdf_vpos.size();
+    std::vector<uint32_t> tdf_dst;
+    std::vector<uint2> tdf_extra;
+    copy_map(tdf_vpos, tdf_dst, tdf_extra, p->tdf_nextra);
     // per-matrix value copy: L slots, U slots, U pivots
     std::vector<uint32_t> df_vpos(dlo.vpos);
     df_vpos.insert(df_vpos.end(), dup.vpos.begin(), dup.vpos.end());
     df_vpos.insert(df_vpos.end(), dup.pvpos.begin(), dup.pvpos.end());
     df_vpos.resize(df_vpos.size() + 64, UINT32_MAX);   // pivot reads of the last positions' lanes
     p->df_nval = (uint32_t)df_vpos.size();
-    // the copy runs in CSC order (contiguous reads, scattered stores): the
-    // slot of each position, plus pairs for positions with a second slot;
-    // padding slots are zeroed once when the copies are allocated
-    std::vector<uint32_t> df_dst(P.nnz, UINT32_MAX);
+    std::vector<uint32_t> df_dst;
     std::vector<uint2> df_extra;
-    for (uint32_t q = 0; q < p->df_nval; ++q) {
-        const uint32_t pos = df_vpos[q];
-        if (pos == UINT32_MAX) continue;
-        if (df_dst[pos] == UINT32_MAX) df_dst[pos] = q;
-        else df_extra.push_back(make_uint2(q, pos));
-    }
-    if (df_extra.empty()) df_extra.push_back(make_uint2(UINT32_MAX, 0u));   // keeps the table non-empty
-    p->df_nextra = (uint32_t)(df_extra.size() - (df_extra[0].x == UINT32_MAX));
+    copy_map(df_vpos, df_dst, df_extra, p->df_nextra);
 
     // Refinement / refresh tables from the analysis.
     std::vector<uint32_t> a_rptr, a_cidx, a_src, lu_src, lu_col, a_rows, a_cols;
@@ -1236,6 +1265,10 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
     const size_t o_wdev = U.reserve(sizeof(WinDev) * std::max<size_t>(wins.size(), 1));
     const size_t o_dlo_task = U.put(dlo.task), o_dlo_rl = U.put(dlo.rl), o_dlo_idx = U.put(dlo.idx);
     const size_t o_dup_task = U.put(dup.task), o_dup_rl = U.put(dup.rl), o_dup_idx = U.put(dup.idx);
+    const size_t o_dtu_task = U.put(dtu.task), o_dtu_rl = U.put(dtu.rl), o_dtu_idx = U.put(dtu.idx);
+    const size_t o_dtl_task = U.put(dtl.task), o_dtl_rl = U.put(dtl.rl), o_dtl_idx = U.put(dtl.idx);
+    const size_t o_tdfdst = U.put(tdf_dst);
+    const size_t o_tdfextra = U.put(tdf_extra);
     const size_t o_dfdst = U.put(df_dst);
     const size_t o_dfextra = U.put(df_extra);
     size_t o_trows[4], o_tptr[4], o_tlong[4], o_ttask[4];
@@ -1369,6 +1402,12 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
     };
     S.lo_df = df(dlo, o_dlo_task, o_dlo_rl, o_dlo_idx, 0, 0);
     S.up_df = df(dup, o_dup_task, o_dup_rl, o_dup_idx, 1, (uint32_t)dlo.idx.size());
+    S.tup_df = df(dtu, o_dtu_task, o_dtu_rl, o_dtu_idx, 1, 0);
+    S.tlo_df = df(dtl, o_dtl_task, o_dtl_rl, o_dtl_idx, 0, tlo_voff);
+    S.tdf_dst = u32p(o_tdfdst);
+    S.tdf_extra = reinterpret_cast<const uint2 *>(A + o_tdfextra);
+    S.tdf_nval = p->tdf_nval;
+    S.tdf_nextra = p->tdf_nextra;
     S.df_dst = u32p(o_dfdst);
     S.df_extra = reinterpret_cast<const uint2 *>(A + o_dfextra);
     S.df_nval = p->df_nval;
@@ -1667,7 +1706,7 @@ extern "C" glu_status glu_refine(glu_plan *p, const double *d_ax, const float *d
     hipStream_t st = static_cast<hipStream_t>(stream);
     const SolveDev &S = p->S;
     const uint32_t n = p->n;
-    if (!transpose) {
+    {
         // one workgroup: the batch pipeline's dataflow sweeps and FGMRES
         // (plan.krylov), when the sweeps' ready bits fit in LDS
         const size_t cap = launch::enable_lds();
@@ -1676,11 +1715,19 @@ extern "C" glu_status glu_refine(glu_plan *p, const double *d_ax, const float *d
             glu_status s = ensure_batch(p, 1);
             if (s != GLU_OK) return s;
             if (!p->d_rout) HIP_TRY(hipMalloc(&p->d_rout, 16));
+            float *lucsr = p->d_blucsr;
+            if (transpose) {
+                if (!p->d_tlucsr) {
+                    HIP_TRY(hipMalloc(&p->d_tlucsr, sizeof(float) * p->tdf_nval));
+                    HIP_TRY(hipMemset(p->d_tlucsr, 0, sizeof(float) * p->tdf_nval));   // padding slots
+                }
+                lucsr = p->d_tlucsr;
+            }
             int32_t *d_it = reinterpret_cast<int32_t *>(p->d_rout);
             double *d_eta = reinterpret_cast<double *>(p->d_rout + 8);
-            launch::refine_one(p->d_desc, p->nnz, p->df_nextra, d_ax, d_b, d_x, d_lu, p->d_blucsr, p->d_bt32,
-                               p->d_br64, p->d_bnorms, p->d_bkry, d_it, d_eta, tol, maxit, p->fused_threads, mode,
-                               launch::batch_lds_bytes(mode, n), st);
+            launch::refine_one(p->d_desc, p->nnz, transpose ? p->tdf_nextra : p->df_nextra, d_ax, d_b, d_x, d_lu,
+                               lucsr, p->d_bt32, p->d_br64, p->d_bnorms, p->d_bkry, d_it, d_eta, tol, maxit,
+                               p->fused_threads, mode, launch::batch_lds_bytes(mode, n), transpose, st);
             HIP_TRY(hipGetLastError());
             int32_t it = 0;
             double eta = 0.0;
