@@ -1173,8 +1173,12 @@ __device__ __forceinline__ void df2_issue(const DfDev &D, const float *__restric
 // Waits spin on the LDS ready bits without s_sleep: measured on the n=20k
 // batch, the wake-up latency of a sleeping wave costs more (7.08 vs
 // 6.92 ms per batch) than the LDS reads of the spinning ones.
+// kExact: every row's products subtracted one at a time in the row's order
+// (group rows through the wave's LDS slice `lds`, by the group's first
+// lane) — the arithmetic of Symbolic_Matrix::solve, for glu_solve.
+template <bool kExact = false>
 __device__ __forceinline__ void df2_run(const DfDev &D, const float *__restrict__ sv, float *x, uint32_t *flags,
-                                        const DfTask &t, uint32_t lane)
+                                        const DfTask &t, uint32_t lane, float *lds = nullptr)
 {
     if (!t.wave) {
         bool pending = lane < t.cnt;
@@ -1221,6 +1225,39 @@ __device__ __forceinline__ void df2_run(const DfDev &D, const float *__restrict_
     // task); only a 64-lane row may need more than one pass, and then loads
     // four passes at a time
     const uint32_t gw = 1u << t.lg, j = lane & (gw - 1u);
+    if (kExact) {
+        const bool lead = j == 0 && (lane >> t.lg) < t.cnt;
+        float acc = lead ? x[t.row] : 0.f;
+        for (uint32_t c0 = 0; c0 < t.len || c0 == 0; c0 += kWave) {   // a pass of gw (64: of 64) entries
+            uint32_t col = t.idx[0];
+            float v = t.val[0];
+            if (c0 > 0) {   // gw == 64 here
+                col = c0 + j < t.len ? D.idx[t.base + c0 + j] : 0u;
+                v = c0 + j < t.len ? sv[t.base + c0 + j] : 0.f;
+            }
+            float pr = 0.f;
+            if (c0 + j < t.len) {
+                while (!df_ready(flags, col)) {}
+                pr = v * x[col];
+            }
+            lds[lane] = pr;
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            if (lead) {
+                const uint32_t cnt = min(gw, t.len - c0);
+                acc = serial_sub(acc, lds + (lane & ~(gw - 1u)), cnt, false);
+            }
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            if (t.len == 0) break;
+        }
+        if (lead) {
+            if (D.div) acc = acc / t.dval;
+            x[t.row] = acc;
+            df_set(flags, t.row);
+        }
+        return;
+    }
     float sum = 0.f;
     if (j < t.len) {
         while (!df_ready(flags, t.idx[0])) {}
@@ -1257,7 +1294,9 @@ __device__ __forceinline__ void df2_run(const DfDev &D, const float *__restrict_
 // scalar/LDS wait counter, so waiting for one never waits for the vector
 // loads in flight), fetched two tasks ahead; a task's rows, columns and
 // values are vector loads issued one task ahead.
-__device__ __forceinline__ void df2_sweep(const DfDev &D, const float *__restrict__ sv, float *x, uint32_t *flags)
+template <bool kExact = false>
+__device__ __forceinline__ void df2_sweep(const DfDev &D, const float *__restrict__ sv, float *x, uint32_t *flags,
+                                          float *lds = nullptr)
 {
     const uint32_t lane = threadIdx.x & (kWave - 1), nw = blockDim.x / kWave;
     const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);   // uniform: scalar loads below
@@ -1281,7 +1320,7 @@ __device__ __forceinline__ void df2_sweep(const DfDev &D, const float *__restric
         dn = dn2;
         dn2 = desc(j + 2);
         df2_issue(D, sv, dn, lane, nxt);
-        df2_run(D, sv, x, flags, cur, lane);
+        df2_run<kExact>(D, sv, x, flags, cur, lane, lds);
     }
 }
 
@@ -1781,6 +1820,40 @@ __global__ void __launch_bounds__(1024) k_refine_one(const MatDesc *__restrict__
     wg_refine<kMode, kT>(desc, io, tol, maxit, sh, hs, xs, flags);
 }
 
+// glu_solve in one workgroup (A x = b): xs = float(b[col_perm] *
+// col_scale_perm), the exact dataflow L and U sweeps over the sweep-layout
+// copy of the factors, x = float(xs[row_perm_inv] * row_scale) — the
+// arithmetic of Symbolic_Matrix::solve (src/symbolic.cc:215-247).  Dynamic
+// LDS as k_batch_same_refine.
+template <int kMode>
+__global__ void __launch_bounds__(1024) k_solve_one(const MatDesc *__restrict__ desc,
+                                                    const float *__restrict__ lucsr, const float *__restrict__ b,
+                                                    float *__restrict__ x, float *__restrict__ t32)
+{
+    __shared__ __attribute__((aligned(16))) float wl[1024 / kWave][kWave];
+    const uint32_t n = desc->n, tid = threadIdx.x, nt = blockDim.x;
+    uint32_t *flags;
+    float *xs;
+    batch_lds<kMode>(n, t32, flags, xs);
+    {
+        const uint32_t *cp = glb(desc->s.col_perm);
+        const double *csp = glb(desc->s.col_scale_perm);
+        for (uint32_t j = tid; j < n; j += nt) xs[j] = (float)((double)b[cp[j]] * csp[j]);
+    }
+    const uint32_t nwords = (n + 31) / 32;
+    for (int sweep = 0; sweep < 2; ++sweep) {
+        for (uint32_t i = tid; i < nwords; i += nt) flags[i] = 0u;
+        __syncthreads();
+        DfDev d = sweep == 0 ? desc->s.lo_df : desc->s.up_df;
+        globalize(d);
+        df2_sweep<true>(d, glb(lucsr) + d.voff, xs, flags, wl[tid / kWave]);
+        __syncthreads();
+    }
+    const uint32_t *rpi = glb(desc->s.row_perm_inv);
+    const double *rs = glb(desc->s.row_scale);
+    for (uint32_t j = tid; j < n; j += nt) x[j] = (float)((double)xs[rpi[j]] * rs[j]);
+}
+
 // Distinct patterns: workgroup g runs matrix order[g] with its own tables.
 template <bool kCopy>
 __global__ void __launch_bounds__(1024) k_batch_group_factor(const MatDesc *__restrict__ descs,
@@ -1998,6 +2071,16 @@ void refine_one(const MatDesc *desc, uint32_t nnz, uint32_t nextra, const double
 #undef GLU_REFINE_ONE
 }
 
+// glu_solve in one workgroup: the factors into the sweep layout, then the
+// exact sweeps (mode: kSweepLds or kSweepFlags).
+void solve_one(const MatDesc *desc, uint32_t nnz, uint32_t nextra, const float *lu, float *lucsr, const float *b,
+               float *x, float *t32, int threads, int mode, size_t lds, hipStream_t st)
+{
+    k_copy_df<false><<<blocks_for(std::max(nnz, nextra), 256), 256, 0, st>>>(desc, lu, lucsr);
+    if (mode == kSweepLds) k_solve_one<kSweepLds><<<1, threads, lds, st>>>(desc, lucsr, b, x, t32);
+    else k_solve_one<kSweepFlags><<<1, threads, lds, st>>>(desc, lucsr, b, x, t32);
+}
+
 void abft_cca(const FactorDev &F, const float *val, float *cca, int batch, hipStream_t st)
 {
     k_abft_cca<<<dim3(blocks_for(F.n, 256), batch), 256, 0, st>>>(F.col_ptr, val, cca, F.n, F.nnz);
@@ -2064,6 +2147,8 @@ size_t enable_lds()
         reinterpret_cast<const void *>(&k_batch_same_factor<true>),
         reinterpret_cast<const void *>(&k_batch_same_factor<false>),
         reinterpret_cast<const void *>(&k_batch_factor),
+        reinterpret_cast<const void *>(&k_solve_one<kSweepLds>),
+        reinterpret_cast<const void *>(&k_solve_one<kSweepFlags>),
         reinterpret_cast<const void *>(&k_refine_one<kSweepLds, false>),
         reinterpret_cast<const void *>(&k_refine_one<kSweepFlags, false>),
         reinterpret_cast<const void *>(&k_refine_one<kSweepLds, true>),

@@ -63,6 +63,8 @@ size_t enable_lds();
 void factor_window(const FactorDev &F, int w, uint32_t nslot, float *val, const float *d_pert, int perturb,
                    int threads, hipStream_t st);
 void zero_u64(unsigned long long *p, int count, hipStream_t st);
+void solve_one(const MatDesc *desc, uint32_t nnz, uint32_t nextra, const float *lu, float *lucsr, const float *b,
+               float *x, float *t32, int threads, int mode, size_t lds, hipStream_t st);
 void refine_one(const MatDesc *desc, uint32_t nnz, uint32_t nextra, const double *ax, const double *b, double *x,
                 const float *lu, float *lucsr, float *t32, double *r64, double *norms, double *kry, int32_t *iters,
                 double *eta, double tol, int maxit, int threads, int mode, size_t ref_lds, int transpose,
@@ -1661,9 +1663,19 @@ extern "C" glu_status glu_solve(glu_plan *p, const float *d_lu, const float *d_b
     if (!p || !d_lu || !d_b || !d_x) return fail(GLU_ERR_ARGUMENT, "null argument");
     if (!p->has_scaling) return fail(GLU_ERR_ARGUMENT, "plan has no permutation/scaling (create it from an analysis)");
     hipStream_t st = static_cast<hipStream_t>(stream);
+    // A x = b in one workgroup (exact dataflow sweeps) when the ready bits
+    // fit in LDS and the dataflow solve is on
+    const int mode = !transpose && p->tri_dataflow ? sweep_mode(p->n, launch::enable_lds()) : 0;
+    if (mode != 0) {
+        glu_status s = ensure_batch(p, 1);   // the sweep-layout copy (before any graph capture)
+        if (s != GLU_OK) return s;
+    }
     auto issue = [&](hipStream_t s) {
         const SolveDev &S = p->S;
-        if (!transpose) {
+        if (mode != 0) {
+            launch::solve_one(p->d_desc, p->nnz, p->df_nextra, d_lu, p->d_blucsr, d_b, d_x, p->d_bt32,
+                              p->fused_threads, mode, launch::batch_lds_bytes(mode, p->n), s);
+        } else if (!transpose) {
             launch::perm_scale_f32(d_b, p->d_t32, S.col_perm, S.col_scale_perm, 0, p->n, s);
             run_sweeps(p, d_lu, p->d_t32, 0, s);
             launch::perm_scale_f32(p->d_t32, d_x, S.row_perm_inv, S.row_scale, 0, p->n, s);
