@@ -124,10 +124,10 @@ struct TriDev {
 
 // A pipelined dataflow sweep over a per-matrix copy of the factor values
 // laid out task by task (the batch refinement).  Task k = {base, p0, z, m}:
-//  - z = rows (< 2^24), no kLongTask bit: rows of up to kDfK entries, one
-//    per lane: lane l takes position p0 + l (l < rows), its entry e at slot
-//    base + 64 e + l (e < m, m = the task's longest row; shorter rows leave
-//    padding slots);
+//  - z = rows (< 2^24), no kLongTask bit: rows one per lane: lane l takes
+//    position p0 + l (l < rows), its entry e at slot base + 64 e + l (e < m,
+//    m = the task's longest row; shorter rows leave padding slots); a row
+//    longer than kDfK is walked kDfK entries at a time;
 //  - z = kLongTask | log2(gw) << 24 | rows: rows of one level (so no row of
 //    the task depends on another) taken gw = 16, 32 or 64 lanes each: row
 //    g at position p0 + g, its entry j at slot base + gw g + j; a 64-lane
@@ -136,6 +136,10 @@ struct TriDev {
 // the row consumes them.  div: U sweeps, the pivot of position p is value
 // nslots + p.
 constexpr uint32_t kDfK = 6;
+// Rows of kDfK < len <= kDfLaneMax entries are lane rows (walked kDfK
+// entries at a time) in levels that hold at least kDfWideLevel of them.
+constexpr uint32_t kDfLaneMax = 64;
+constexpr uint32_t kDfWideLevel = 32;
 
 struct DfDev {
     const uint4 *task;         // [ntask]

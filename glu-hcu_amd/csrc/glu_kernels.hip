@@ -567,6 +567,10 @@ __device__ __forceinline__ bool df_ready(const uint32_t *flags, uint32_t c)
     return (__hip_atomic_load(flags + (c >> 5), __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) >> (c & 31u)) & 1u;
 }
 
+// Bound on the polls of a long-lane-row task (far above any real wait: a
+// sweep takes milliseconds, a poll tens of nanoseconds).
+constexpr uint32_t kSpinCap = 1u << 24;
+
 __device__ __forceinline__ void df_set(uint32_t *flags, uint32_t r)
 {
     __hip_atomic_fetch_or(flags + (r >> 5), 1u << (r & 31u), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -1170,6 +1174,73 @@ __device__ __forceinline__ void df2_issue(const DfDev &D, const float *__restric
     }
 }
 
+// A task of lane rows some longer than kDfK entries: each lane walks its
+// row kDfK entries at a time (the slots of entry e at base + 64 e + lane),
+// consuming ready entries in the row's order as the short-row path does.
+// Polls are bounded (kSpinCap) so that every wave exits even if a table
+// were inconsistent.
+__device__ __forceinline__ void df2_lane_long(const DfDev &D, const float *__restrict__ sv, float *x,
+                                              uint32_t *flags, const DfTask &t, uint32_t lane)
+{
+    bool pending = lane < t.cnt;
+    float acc = pending ? x[t.row] : 0.f;
+    // the row's entries kDfK at a time: [c0, c0 + kDfK) in idx / val (the
+    // first chunk read again from the slots the task's prefetch brought in:
+    // taking it from t would push the short-row path's arrays to scratch)
+    uint32_t idx[kDfK];
+    float val[kDfK];
+#pragma unroll
+    for (uint32_t e = 0; e < kDfK; ++e) {
+        idx[e] = D.idx[t.base + e * kWave + lane];
+        val[e] = sv[t.base + e * kWave + lane];
+    }
+    uint32_t ci = 0, c0 = 0;   // entries consumed, first entry of the chunk
+    for (uint32_t spins = 0; __ballot(pending) && spins < kSpinCap; ++spins) {
+        if (pending) {
+            if (ci == c0 + kDfK) {
+                // chunk consumed, row not done: its next kDfK entries
+                // (slots past the row, up to kDfK - 1 passes beyond the
+                // task's longest row, exist: the next task's or the
+                // padding after the last task)
+                c0 += kDfK;
+#pragma unroll
+                for (uint32_t e = 0; e < kDfK; ++e) {
+                    const uint32_t sl = t.base + (c0 + e) * kWave + lane;
+                    idx[e] = D.idx[sl];
+                    val[e] = sv[sl];
+                }
+            }
+            // every entry's ready bit and x value read without branches
+            // (independent LDS reads, one wait each), then the ready
+            // prefix consumed in order
+            uint32_t fw[kDfK];
+#pragma unroll
+            for (uint32_t e = 0; e < kDfK; ++e)
+                fw[e] = __hip_atomic_load(flags + (idx[e] >> 5), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            uint32_t mask = 0;
+#pragma unroll
+            for (uint32_t e = 0; e < kDfK; ++e)
+                mask |= (((fw[e] >> (idx[e] & 31u)) & 1u) & (uint32_t)(c0 + e >= ci && c0 + e < t.len)) << e;
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            float xv[kDfK];
+#pragma unroll
+            for (uint32_t e = 0; e < kDfK; ++e) xv[e] = x[idx[e]];
+#pragma unroll
+            for (uint32_t e = 0; e < kDfK; ++e)
+                if (c0 + e == ci && ((mask >> e) & 1u)) {
+                    acc = acc - val[e] * xv[e];
+                    ++ci;
+                }
+            if (ci == t.len) {
+                if (D.div) acc = acc / t.dval;
+                x[t.row] = acc;
+                df_set(flags, t.row);
+                pending = false;
+            }
+        }
+    }
+}
+
 // Waits spin on the LDS ready bits without s_sleep: measured on the n=20k
 // batch, the wake-up latency of a sleeping wave costs more (7.08 vs
 // 6.92 ms per batch) than the LDS reads of the spinning ones.
@@ -1181,6 +1252,10 @@ __device__ __forceinline__ void df2_run(const DfDev &D, const float *__restrict_
                                         const DfTask &t, uint32_t lane, float *lds = nullptr)
 {
     if (!t.wave) {
+        if (t.m > kDfK) {   // uniform: a task of long lane rows
+            df2_lane_long(D, sv, x, flags, t, lane);
+            return;
+        }
         bool pending = lane < t.cnt;
         float acc = pending ? x[t.row] : 0.f;
         uint32_t ci = 0;

@@ -947,7 +947,7 @@ struct DfHost {
 inline uint32_t df_group(uint32_t len) { return len <= 16 ? 16u : len <= 32 ? 32u : 64u; }
 
 template <typename Entries, typename Pivot>
-void build_df(const TriHost &H, Entries entries, bool div, Pivot pivot, DfHost &D)
+void build_df(const TriHost &H, Entries entries, bool div, Pivot pivot, bool long_lanes, DfHost &D)
 {
     const uint32_t n = (uint32_t)H.rows.size();
     const int nlev = (int)H.lvl_ptr.size() - 1;
@@ -957,19 +957,55 @@ void build_df(const TriHost &H, Entries entries, bool div, Pivot pivot, DfHost &
         entries(r, tmp);
         len[r] = (uint32_t)tmp.size();
     }
-    // Within each level (whose rows are independent) the rows taken a lane
-    // each are kept together, after the wave rows on even levels and before
-    // them on odd ones, so that lane runs of adjacent levels meet and pack
-    // into full tasks; wave rows are sorted by length so that rows sharing
+    // Rows are taken a lane each (lane rows) or by groups of lanes (wave
+    // rows).  Short rows (<= kDfK entries) are always lane rows.  With
+    // long_lanes (orders whose sweep vector does not fit in LDS, where a
+    // sweep is bound by the one workgroup's throughput), longer rows of up
+    // to kDfLaneMax entries are lane rows too in a level with at least
+    // kDfWideLevel of them: a lane walks its row kDfK entries at a time, 64
+    // rows per task, where a wave would take 1 to 4 of them; in narrower
+    // levels, which make up the sweep's critical path, they stay wave rows
+    // (one memory round trip per row).  Measured: n=500k solve 55.7 ->
+    // 34.6 ms, n=50k 2.57 -> 1.81 ms; the n=20k batch (vector in LDS) loses
+    // 0.3 ms with them, so smaller orders keep wave rows.
+    // Within each level (whose rows are independent) the short lane rows
+    // are kept together, after the wave rows on even levels and before them
+    // on odd ones, so that lane runs of adjacent levels meet and pack into
+    // full tasks; long lane rows sit between the two, sorted by length so
+    // that a task's rows have similar lengths (a task's slots are sized by
+    // its longest row); wave rows are sorted by length so that rows sharing
     // a wave (16 or 32 lanes each) have the same lane count.
     std::vector<uint32_t> order(n);
+    std::vector<uint8_t> lane_row(n);   // by position
     for (int l = 0; l < nlev; ++l) {
-        std::vector<uint32_t> a, b;
-        for (uint32_t q = H.lvl_ptr[l]; q < H.lvl_ptr[l + 1]; ++q) (len[H.rows[q]] <= kDfK ? a : b).push_back(H.rows[q]);
-        std::stable_sort(b.begin(), b.end(), [&](uint32_t x, uint32_t y) { return len[x] < len[y]; });
-        if (!(l & 1)) std::swap(a, b);
-        std::copy(a.begin(), a.end(), order.begin() + H.lvl_ptr[l]);
-        std::copy(b.begin(), b.end(), order.begin() + H.lvl_ptr[l] + a.size());
+        std::vector<uint32_t> a, c, b;   // short lane rows, long lane rows, wave rows
+        for (uint32_t q = H.lvl_ptr[l]; q < H.lvl_ptr[l + 1]; ++q) {
+            const uint32_t r = H.rows[q];
+            (len[r] <= kDfK ? a : len[r] <= kDfLaneMax ? c : b).push_back(r);
+        }
+        if (!long_lanes || c.size() < kDfWideLevel) {
+            b.insert(b.end(), c.begin(), c.end());
+            c.clear();
+        }
+        auto by_len = [&](uint32_t x, uint32_t y) { return len[x] < len[y]; };
+        std::stable_sort(b.begin(), b.end(), by_len);
+        std::stable_sort(c.begin(), c.end(), by_len);
+        uint32_t q = H.lvl_ptr[l];
+        auto put_rows = [&](auto first, auto last, uint8_t lane) {
+            for (; first != last; ++first, ++q) {
+                order[q] = *first;
+                lane_row[q] = lane;
+            }
+        };
+        if (l & 1) {
+            put_rows(a.begin(), a.end(), 1);
+            put_rows(c.begin(), c.end(), 1);
+            put_rows(b.begin(), b.end(), 0);
+        } else {
+            put_rows(b.begin(), b.end(), 0);
+            put_rows(c.rbegin(), c.rend(), 1);
+            put_rows(a.begin(), a.end(), 1);
+        }
     }
     std::vector<int> lev_of(n);
     for (int l = 0; l < nlev; ++l)
@@ -989,11 +1025,11 @@ void build_df(const TriHost &H, Entries entries, bool div, Pivot pivot, DfHost &
     while (p < n) {
         const uint32_t base = (uint32_t)D.idx.size();
         const uint32_t p0 = p;
-        if (ent[p].size() > kDfK) {
+        if (!lane_row[p]) {
             // rows of one lane count and one level (no dependencies inside)
             const uint32_t gw = df_group((uint32_t)ent[p].size()), per = 64 / gw;
             uint32_t m = 0;
-            while (p < n && p - p0 < per && lev_of[p] == lev_of[p0] && ent[p].size() > kDfK &&
+            while (p < n && p - p0 < per && lev_of[p] == lev_of[p0] && !lane_row[p] &&
                    df_group((uint32_t)ent[p].size()) == gw)
                 m = std::max(m, (uint32_t)ent[p++].size());
             if (gw == 64) {   // padded to whole passes of 64 (every lane loads its slot)
@@ -1008,8 +1044,11 @@ void build_df(const TriHost &H, Entries entries, bool div, Pivot pivot, DfHost &
             D.task.push_back(make_uint4(base, p0, kLongTask | (lg << 24) | (p - p0), m));
             continue;
         }
+        // lane rows, short and long ones in separate tasks
+        const bool lng = ent[p].size() > kDfK;
         uint32_t m = 0;
-        while (p < n && p - p0 < 64 && ent[p].size() <= kDfK) m = std::max(m, (uint32_t)ent[p++].size());
+        while (p < n && p - p0 < 64 && lane_row[p] && (ent[p].size() > kDfK) == lng)
+            m = std::max(m, (uint32_t)ent[p++].size());
         for (uint32_t e = 0; e < m; ++e)
             for (uint32_t l = 0; l < 64; ++l) put(p0 + l < p && e < ent[p0 + l].size() ? ent[p0 + l][e] : pad);
         D.task.push_back(make_uint4(base, p0, p - p0, m));
@@ -1118,24 +1157,27 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
         tri_segments(*p, h);
         tri_tasks(h);
     }
+    // sweep vector beyond LDS (the richest sweep mode does not fit): long
+    // lane rows in wide levels
+    const bool long_lanes = launch::batch_lds_bytes(2, n) > p->lds_cap;
     DfHost dlo, dup;
     build_df(p->tri[0], [&](uint32_t r, std::vector<uint2> &out) {
         for (uint32_t q = P.csr_r_ptr[r]; q < P.csr_diag_ptr[r]; ++q) out.push_back(make_uint2(P.csr_c_idx[q], T.upos[q]));
-    }, false, [&](uint32_t) { return 0u; }, dlo);
+    }, false, [&](uint32_t) { return 0u; }, long_lanes, dlo);
     build_df(p->tri[1], [&](uint32_t r, std::vector<uint2> &out) {
         for (uint32_t q = P.csr_r_ptr[r + 1]; q > P.csr_diag_ptr[r] + 1; --q)
             out.push_back(make_uint2(P.csr_c_idx[q - 1], T.upos[q - 1]));
-    }, true, [&](uint32_t r) { return P.l_col_ptr[r]; }, dup);
+    }, true, [&](uint32_t r) { return P.l_col_ptr[r]; }, long_lanes, dup);
     // A' x = b: rows of U' (columns of U, pivot last) forward, rows of L'
     // (columns of L) backward
     DfHost dtu, dtl;
     build_df(p->tri[2], [&](uint32_t r, std::vector<uint2> &out) {
         for (uint32_t q = P.sym_c_ptr[r]; q < P.l_col_ptr[r]; ++q) out.push_back(make_uint2(P.sym_r_idx[q], q));
-    }, true, [&](uint32_t r) { return P.l_col_ptr[r]; }, dtu);
+    }, true, [&](uint32_t r) { return P.l_col_ptr[r]; }, long_lanes, dtu);
     build_df(p->tri[3], [&](uint32_t r, std::vector<uint2> &out) {
         for (uint32_t q = P.sym_c_ptr[r + 1]; q > P.l_col_ptr[r] + 1; --q)
             out.push_back(make_uint2(P.sym_r_idx[q - 1], q - 1));
-    }, false, [&](uint32_t) { return 0u; }, dtl);
+    }, false, [&](uint32_t) { return 0u; }, long_lanes, dtl);
     // the copy runs in CSC order (contiguous reads, scattered stores): the
     // slot of each position, plus pairs for positions with a second slot;
     // padding slots are zeroed once when the copies are allocated

@@ -64,10 +64,11 @@ def test_factor_host_matches_device(torch_mod, name):
         assert np.array_equal(host.view(np.uint32), lu.cpu().numpy().view(np.uint32))
 
 
-@pytest.mark.parametrize("name", CASES)
+@pytest.mark.parametrize("name", CASES + ["grad"])
 def test_solve_bit_exact(torch_mod, name):
     """With the same factors, the device solve equals the reference's FP32
-    host solve (restated in the oracle) bit for bit."""
+    host solve (restated in the oracle) bit for bit (n=50k "grad": ready
+    bits in LDS, the vector in global memory)."""
     from oracle import oracle as O
     from pyglu import mna
     torch = torch_mod
