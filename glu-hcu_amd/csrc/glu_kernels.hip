@@ -1187,28 +1187,40 @@ __device__ __forceinline__ void df2_lane_long(const DfDev &D, const float *__res
     // the row's entries kDfK at a time: [c0, c0 + kDfK) in idx / val (the
     // first chunk read again from the slots the task's prefetch brought in:
     // taking it from t would push the short-row path's arrays to scratch)
-    uint32_t idx[kDfK];
-    float val[kDfK];
+    // The next chunk is loaded while this one is consumed (nidx / nval),
+    // when the lane's row has entries there (so within the task's slots).
+    uint32_t idx[kDfK], nidx[kDfK];
+    float val[kDfK], nval[kDfK];
 #pragma unroll
     for (uint32_t e = 0; e < kDfK; ++e) {
         idx[e] = D.idx[t.base + e * kWave + lane];
         val[e] = sv[t.base + e * kWave + lane];
     }
+    if (pending && kDfK < t.len)
+#pragma unroll
+        for (uint32_t e = 0; e < kDfK; ++e) {
+            nidx[e] = D.idx[t.base + (kDfK + e) * kWave + lane];
+            nval[e] = sv[t.base + (kDfK + e) * kWave + lane];
+        }
     uint32_t ci = 0, c0 = 0;   // entries consumed, first entry of the chunk
     for (uint32_t spins = 0; __ballot(pending) && spins < kSpinCap; ++spins) {
         if (pending) {
             if (ci == c0 + kDfK) {
-                // chunk consumed, row not done: its next kDfK entries
-                // (slots past the row, up to kDfK - 1 passes beyond the
-                // task's longest row, exist: the next task's or the
-                // padding after the last task)
+                // chunk consumed, row not done: the next kDfK entries
+                // (loaded ahead), and the chunk after them requested
                 c0 += kDfK;
 #pragma unroll
                 for (uint32_t e = 0; e < kDfK; ++e) {
-                    const uint32_t sl = t.base + (c0 + e) * kWave + lane;
-                    idx[e] = D.idx[sl];
-                    val[e] = sv[sl];
+                    idx[e] = nidx[e];
+                    val[e] = nval[e];
                 }
+                if (c0 + kDfK < t.len)
+#pragma unroll
+                    for (uint32_t e = 0; e < kDfK; ++e) {
+                        const uint32_t sl = t.base + (c0 + kDfK + e) * kWave + lane;
+                        nidx[e] = D.idx[sl];
+                        nval[e] = sv[sl];
+                    }
             }
             // every entry's ready bit and x value read without branches
             // (independent LDS reads, one wait each), then the ready
