@@ -305,6 +305,9 @@ __device__ __forceinline__ void factor_level(const FactorDev &F, float *__restri
 // (the next one of the level, or its first one of the next level, issued
 // before the barrier) while it runs the current one, so a task waits only
 // for its values.  Same items and order as factor_level.
+// kDeferDiv: no division items here (the caller divides after the last
+// step: no update reads a divided L value, each divides on the fly).
+template <bool kDeferDiv = false>
 __device__ __forceinline__ void factor_levels_wg(const FactorDev &F, float *__restrict__ val, int l0, int l1,
                                                  float pert, int perturb, float *lds)
 {
@@ -315,7 +318,7 @@ __device__ __forceinline__ void factor_levels_wg(const FactorDev &F, float *__re
     if (l0 < l1) {
         if (F.ctask_ptr[l0] + w < F.ctask_ptr[l0 + 1]) ctab_load(F, F.ctask_ptr[l0] + w, lane, cur);
         const LevelSpan s = level_span(F, l0);
-        if (tid < s.nd) dv = F.dvd[s.d0 + tid];
+        if (!kDeferDiv && tid < s.nd) dv = F.dvd[s.d0 + tid];
     }
     for (int lvl = l0; lvl < l1; ++lvl) {
         const LevelSpan s = level_span(F, lvl);
@@ -328,12 +331,14 @@ __device__ __forceinline__ void factor_levels_wg(const FactorDev &F, float *__re
             if (more) cur = nxt;
         }
         for (uint32_t t = s.ml + tid; t < s.nm; t += nt) factor_merge(F, val, s.m0 + t, pert, perturb);
-        if (tid < s.nd) factor_divide(val, dv, pert, perturb);
-        for (uint32_t t = tid + nt; t < s.nd; t += nt) factor_divide(val, F.dvd[s.d0 + t], pert, perturb);
+        if (!kDeferDiv) {
+            if (tid < s.nd) factor_divide(val, dv, pert, perturb);
+            for (uint32_t t = tid + nt; t < s.nd; t += nt) factor_divide(val, F.dvd[s.d0 + t], pert, perturb);
+        }
         if (lvl + 1 < l1) {   // the next step's table data, loaded across the barrier
             const LevelSpan q = level_span(F, lvl + 1);
             if (q.c0 + w < q.c0 + q.nc) ctab_load(F, q.c0 + w, lane, cur);
-            if (tid < q.nd) dv = F.dvd[q.d0 + tid];
+            if (!kDeferDiv && tid < q.nd) dv = F.dvd[q.d0 + tid];
         }
         __syncthreads();
     }
@@ -412,6 +417,7 @@ __device__ __forceinline__ WinDev load_window(const FactorDev &F, int w)
 
 // Steps [W.l0, W.l1) inside one workgroup with the values they touch staged
 // in LDS (slots): load, one barrier per step, store back.
+template <bool kDeferDiv = false>
 __device__ __forceinline__ void factor_window(const FactorDev &F, const WinDev &W, float *__restrict__ val,
                                               float *slots, float pert, int perturb, float *wlds)
 {
@@ -419,24 +425,31 @@ __device__ __forceinline__ void factor_window(const FactorDev &F, const WinDev &
     for (uint32_t s = tid; s < W.nslot; s += nt) slots[s] = val[W.pos[s]];
     __syncthreads();
     const FactorDev T = window_view(F, W);
-    factor_levels_wg(T, slots, W.l0, W.l1, pert, perturb, wlds);
+    factor_levels_wg<kDeferDiv>(T, slots, W.l0, W.l1, pert, perturb, wlds);
     for (uint32_t s = tid; s < W.nslot; s += nt) val[W.pos[s]] = slots[s];
     __syncthreads();
 }
 
 // All steps of one factorization by one workgroup: windows in LDS, the
-// steps between them in global memory.
+// steps between them in global memory.  The divisions L(i,k) /= U(k,k) of
+// every column run after the last step, all at once: no step reads a
+// divided L value (updates divide on the fly) and no step writes L(:,k) or
+// U(k,k) after column k's step, so the values are those of the per-step
+// order, one barrier-bound pass per step less.
 __device__ __forceinline__ void factor_all_wg(const FactorDev &F, float *__restrict__ val, float *slots, float pert,
                                               int perturb, float *wlds)
 {
     int l = 0;
     for (int w = 0; w < F.nwin; ++w) {
         const WinDev W = load_window(F, w);
-        factor_levels_wg(F, val, l, W.l0, pert, perturb, wlds);
-        factor_window(F, W, val, slots, pert, perturb, wlds);
+        factor_levels_wg<true>(F, val, l, W.l0, pert, perturb, wlds);
+        factor_window<true>(F, W, val, slots, pert, perturb, wlds);
         l = W.l1;
     }
-    factor_levels_wg(F, val, l, F.num_lev + 1, pert, perturb, wlds);
+    factor_levels_wg<true>(F, val, l, F.num_lev + 1, pert, perturb, wlds);
+    const uint32_t nd = F.dvd_ptr[F.num_lev];
+    for (uint32_t t = threadIdx.x; t < nd; t += blockDim.x) factor_divide(val, F.dvd[t], pert, perturb);
+    __syncthreads();
 }
 
 extern __shared__ float g_lds_vec[];   // dynamic LDS of the kernels below
