@@ -290,6 +290,8 @@ __device__ __forceinline__ LevelSpan level_span(const FactorDev &F, int lvl)
 // workgroup or the whole grid), `tid` = this lane's index in the group:
 // long merges and column tasks one wave each, short merges and divisions
 // one lane each.  lds: this wave's 64-float slice.
+// (Division items: none here; the factorization ends with one pass over all
+// of them, k_factor_divide.)
 __device__ __forceinline__ void factor_level(const FactorDev &F, float *__restrict__ val, const LevelSpan &s,
                                              uint32_t tid, uint32_t nthreads, float pert, int perturb, float *lds)
 {
@@ -297,7 +299,6 @@ __device__ __forceinline__ void factor_level(const FactorDev &F, float *__restri
     for (uint32_t e = w; e < s.ml; e += nw) factor_merge_wave(F, val, s.m0 + e, pert, perturb, lane, lds);
     factor_ctasks(F, val, s.c0, s.nc, tid, nthreads, pert, perturb);
     for (uint32_t t = s.ml + tid; t < s.nm; t += nthreads) factor_merge(F, val, s.m0 + t, pert, perturb);
-    for (uint32_t t = tid; t < s.nd; t += nthreads) factor_divide(val, F.dvd[s.d0 + t], pert, perturb);
 }
 
 // Steps [l0, l1) inside one workgroup, a barrier after each, with the column
@@ -385,7 +386,19 @@ __global__ void __launch_bounds__(1024) k_factor_fused(FactorDev F, float *__res
 {
     __shared__ __attribute__((aligned(16))) float lds[1024 / kWave][kWave];
     const float pert = perturb ? *d_pert : 0.f;
-    factor_levels_wg(F, val, l0, l1, pert, perturb, lds[threadIdx.x / kWave]);
+    factor_levels_wg<true>(F, val, l0, l1, pert, perturb, lds[threadIdx.x / kWave]);
+}
+
+// The divisions L(i,k) /= U(k,k) of every column, after the last step (no
+// step reads a divided L value: updates divide on the fly; no step writes
+// L(:,k) or U(k,k) after column k's step).
+__global__ void __launch_bounds__(256) k_factor_divide(FactorDev F, float *__restrict__ val,
+                                                       const float *__restrict__ d_pert, int perturb)
+{
+    const float pert = perturb ? *d_pert : 0.f;
+    const uint32_t nd = F.dvd_ptr[F.num_lev];
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nd; t += gridDim.x * blockDim.x)
+        factor_divide(val, F.dvd[t], pert, perturb);
 }
 
 // A window's view of the tables: the same level spans, items that address
@@ -461,7 +474,7 @@ __global__ void __launch_bounds__(1024) k_factor_window(FactorDev F, float *__re
     __shared__ __attribute__((aligned(16))) float lds[1024 / kWave][kWave];
     const float pert = perturb ? *d_pert : 0.f;
     const WinDev W = load_window(F, w);
-    factor_window(F, W, val, g_lds_vec, pert, perturb, lds[threadIdx.x / kWave]);
+    factor_window<true>(F, W, val, g_lds_vec, pert, perturb, lds[threadIdx.x / kWave]);
 }
 
 // ---------------------------------------------------------------------------
@@ -2077,6 +2090,12 @@ void factor_fused(const FactorDev &F, float *val, const float *d_pert, int l0, i
                   int threads, hipStream_t st)
 {
     k_factor_fused<<<1, threads, 0, st>>>(F, val, d_pert, l0, l1, perturb);
+}
+
+// ndiv: the division items (all steps).
+void factor_divide(const FactorDev &F, float *val, const float *d_pert, uint32_t ndiv, int perturb, hipStream_t st)
+{
+    k_factor_divide<<<std::max(1u, std::min(blocks_for(ndiv, 256), 2048u)), 256, 0, st>>>(F, val, d_pert, perturb);
 }
 
 // lanes: long rows x 64 + short rows of the level

@@ -33,6 +33,7 @@ void factor_grid(const FactorDev &F, float *val, const float *d_pert, int lvl, u
                  int threads, hipStream_t st);
 void factor_fused(const FactorDev &F, float *val, const float *d_pert, int l0, int l1, int perturb,
                   int threads, hipStream_t st);
+void factor_divide(const FactorDev &F, float *val, const float *d_pert, uint32_t ndiv, int perturb, hipStream_t st);
 void tri_grid(const TriDev &T, const float *lu, float *x, int lvl, uint32_t rows, hipStream_t st);
 void tri_fused(const TriDev &T, const float *lu, float *x, int l0, int l1, int threads, hipStream_t st);
 void perm_scale_f32(const float *in, float *out, const uint32_t *P, const double *S, int by_perm, uint32_t n,
@@ -290,6 +291,7 @@ struct glu_plan {
     int device = 0;
     uint32_t n = 0, nnz = 0, nnz_a = 0;
     int num_lev = 0;
+    uint32_t ndiv = 0;           // division items of the factorization
     int64_t updates = 0;
     int64_t units = 0;
     int fused_threads = kDefaultFusedThreads;
@@ -1295,6 +1297,7 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
     const size_t o_msrc = U.put(T.msrc);
     const size_t o_dvd = U.put(T.dvd);
     const size_t o_dptr = U.put(T.dvd_ptr);
+    p->ndiv = (uint32_t)T.dvd.size();
     std::vector<size_t> o_wpos(wins.size()), o_wcln(wins.size()), o_wctgt(wins.size()), o_wcup(wins.size()),
         o_wment(wins.size()), o_wmsrc(wins.size()), o_wdvd(wins.size());
     for (size_t w = 0; w < wins.size(); ++w) {
@@ -1530,6 +1533,7 @@ glu_status run_factor(glu_plan *p, float *d_val, int perturb, hipStream_t st)
         else
             launch::factor_grid(p->F, d_val, p->d_pert, s.l0, s.items, perturb, kGridThreads, st);
     }
+    launch::factor_divide(p->F, d_val, p->d_pert, p->ndiv, perturb, st);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(GLU_ERR_DEVICE, std::string("factor launch: ") + hipGetErrorString(e));
     return GLU_OK;
