@@ -720,8 +720,7 @@ uint32_t level_lanes(const Tables &T, int l)
     const uint32_t nc = T.ctask_ptr[l + 1] - T.ctask_ptr[l];
     const uint32_t nm = T.ment_ptr[l + 1] - T.ment_ptr[l];
     const uint32_t ml = T.ment_long[l];
-    const uint32_t nd = l > 0 ? T.dvd_ptr[l] - T.dvd_ptr[l - 1] : 0;
-    return std::max({nc * 64, nm - ml, nd, ml * 64});
+    return std::max({nc * 64, nm - ml, ml * 64});   // (divisions run after the last step)
 }
 
 // LDS windows: runs of consecutive steps whose items touch at most `budget`
@@ -743,7 +742,8 @@ struct Window {
 };
 
 // The CSC positions the items of step l read or write, each once per call
-// of f (duplicates possible).
+// of f (duplicates possible).  Division items are not step items: they run
+// after the last step, on the values in global memory.
 template <typename F>
 void step_positions(const Tables &T, int l, F f)
 {
@@ -765,11 +765,6 @@ void step_positions(const Tables &T, int l, F f)
             f(m.lpos), f(m.upos), f(m.dpos);
         }
     }
-    if (l > 0)
-        for (uint32_t q = T.dvd_ptr[l - 1]; q < T.dvd_ptr[l]; ++q) {
-            f(T.dvd[q].x);
-            f(T.dvd[q].y & ~kFirstBit);
-        }
 }
 
 void make_window(const Tables &T, int a, int b, const std::vector<uint32_t> &slot_of_pos,
@@ -799,12 +794,7 @@ void make_window(const Tables &T, int a, int b, const std::vector<uint32_t> &slo
         const MergeSrc &m = T.msrc[k];
         w.msrc.push_back(MergeSrc{slot(m.lpos), slot(m.upos), slot(m.dpos), 0});
     }
-    w.d0 = a > 0 ? T.dvd_ptr[a - 1] : 0;
-    const uint32_t d1 = b > 0 ? T.dvd_ptr[b - 1] : 0;
-    for (uint32_t q = w.d0; q < d1; ++q) {
-        const uint2 &d = T.dvd[q];
-        w.dvd.push_back(make_uint2(slot(d.x), slot(d.y & ~kFirstBit) | (d.y & kFirstBit)));
-    }
+    w.d0 = 0;   // no division items inside a window (they run after the last step)
 }
 
 // Windows chosen backward from the last step, so that the final run (the
