@@ -1339,8 +1339,15 @@ __device__ __forceinline__ void df2_run(const DfDev &D, const float *__restrict_
     // four passes at a time
     const uint32_t gw = 1u << t.lg, j = lane & (gw - 1u);
     if (kExact) {
-        const bool lead = j == 0 && (lane >> t.lg) < t.cnt;
-        float acc = lead ? x[t.row] : 0.f;
+        // The products of a pass in parallel, then subtracted in the row's
+        // order along the group's lanes: lane j takes lane j - 1's running
+        // value (a DPP shift within each row of 16 lanes; across rows, the
+        // previous row's lane 15) and subtracts its own product.  Lanes past
+        // the row's end subtract +0, which changes no value.  The group's
+        // last lane ends with acc - p[0] - p[1] - ... - p[len - 1].
+        const bool on = (lane >> t.lg) < t.cnt;
+        float carry = on ? x[t.row] : 0.f;   // the running value entering a pass, in lane 0 of the group
+        float a = 0.f;
         for (uint32_t c0 = 0; c0 < t.len || c0 == 0; c0 += kWave) {   // a pass of gw (64: of 64) entries
             uint32_t col = t.idx[0];
             float v = t.val[0];
@@ -1353,20 +1360,23 @@ __device__ __forceinline__ void df2_run(const DfDev &D, const float *__restrict_
                 while (!df_ready(flags, col)) {}
                 pr = v * x[col];
             }
-            lds[lane] = pr;
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            if (lead) {
-                const uint32_t cnt = min(gw, t.len - c0);
-                acc = serial_sub(acc, lds + (lane & ~(gw - 1u)), cnt, false);
+            a = j == 0 ? carry - pr : pr;
+            for (uint32_t r = 0; r < gw / 16u; ++r) {
+                if (r > 0) {   // row_bcast:15
+                    const float b = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(a), 0x142, 0xf, 0xf, false));
+                    if (j == 16u * r) a = b - pr;
+                }
+#pragma unroll
+                for (uint32_t k = 1; k < 16u; ++k) {   // row_shr:1
+                    const float prev = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(a), 0x111, 0xf, 0xf, false));
+                    if (j == 16u * r + k) a = prev - pr;
+                }
             }
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            if (t.len == 0) break;
+            if (gw == kWave) carry = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(a), kWave - 1));
         }
-        if (lead) {
-            if (D.div) acc = acc / t.dval;
-            x[t.row] = acc;
+        if (on && j == gw - 1u) {
+            if (D.div) a = a / t.dval;
+            x[t.row] = a;
             df_set(flags, t.row);
         }
         return;
