@@ -209,6 +209,46 @@ def bench_autograd(dev, log):
     return {"n": n, "nnz": int(d.size), "forward_backward_ms": round(ms, 3)}
 
 
+def bench_large(dev, log, tol):
+    """n=500k MNA matrix: single refactor (median of 5, HIP events) and the
+    FP64 refinement to tol (wall clock of one glu_refine call)."""
+    import torch
+    from pyglu import mna
+    from pyglu.analysis import Analysis
+    from pyglu.device import Plan
+    t0 = time.perf_counter()
+    dL, iL, pL, nL = mna.mna_csc(mna.SPEC_LARGE, mna.SEED_LARGE)
+    anL = Analysis(dL, iL, pL, nL)
+    planL = Plan(anL)
+    t_an = time.perf_counter() - t0
+    axL = torch.from_numpy(dL).to(dev)
+    luL = planL.refresh(axL).reshape(-1)
+    base = luL.clone()
+    planL.factor(luL)
+    torch.cuda.synchronize()
+    reps = 5
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+    for s_, e_ in evs:
+        luL.copy_(base)
+        s_.record()
+        planL.factor(luL)
+        e_.record()
+    torch.cuda.synchronize()
+    ms = sorted(s_.elapsed_time(e_) for s_, e_ in evs)[reps // 2]
+    bL = torch.from_numpy(mna.rhs(nL, 102)).to(dev)
+    planL.refine(axL, luL, bL, tol=tol, maxit=20)   # warm (scratch allocation)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, itL, etaL = planL.refine(axL, luL, bL, tol=tol, maxit=20)
+    refine_ms = (time.perf_counter() - t0) * 1e3
+    log(f"[bench] n=500k: analysis+plan {t_an:.1f}s refactor {ms:.3f} ms, refine {itL} iters {refine_ms:.1f} ms "
+        f"eta {etaL:.2e}")
+    planL.close()
+    return {"n": nL, "nnz": int(dL.size), "nnz_lu": anL.nnz_lu, "levels": anL.num_lev,
+            "refactor_ms": round(ms, 3), "refine_iters": itL, "eta": etaL, "refine_ms": round(refine_ms, 3),
+            "host_analysis_s": round(t_an, 2)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -227,6 +267,9 @@ def main():
                     help="skip the n=4960, uneven-group and autograd side measurements")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run one 256 MiB device copy (counter calibration)")
+    ap.add_argument("--single", choices=("small", "large"),
+                    help="only the n=4960 factor + solve or the n=500k refactor + refinement measurement "
+                         "(its JSON on stdout; for per-kernel profiles)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -247,6 +290,10 @@ def main():
     torch.cuda.set_device(0)
     dev = torch.device("cuda:0")
     log = lambda *a: print(*a, file=sys.stderr, flush=True)
+    if args.single:
+        res = bench_small(dev, log) if args.single == "small" else bench_large(dev, log, args.tol)
+        print(json.dumps({"single": args.single, **res}))
+        return
 
     # ---------------- workload ----------------------------------------
     B = args.batch
@@ -374,39 +421,7 @@ def main():
         grad = bench_autograd(dev, log)
 
     # ---------------- n = 500k single refactor --------------------------
-    large = None
-    if not args.no_large:
-        t0 = time.perf_counter()
-        dL, iL, pL, nL = mna.mna_csc(mna.SPEC_LARGE, mna.SEED_LARGE)
-        anL = Analysis(dL, iL, pL, nL)
-        planL = Plan(anL)
-        t_an = time.perf_counter() - t0
-        axL = torch.from_numpy(dL).to(dev)
-        luL = planL.refresh(axL).reshape(-1)
-        base = luL.clone()
-        planL.factor(luL)
-        torch.cuda.synchronize()
-        reps = 5
-        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
-        for s_, e_ in evs:
-            luL.copy_(base)
-            s_.record()
-            planL.factor(luL)
-            e_.record()
-        torch.cuda.synchronize()
-        ms = sorted(s_.elapsed_time(e_) for s_, e_ in evs)[reps // 2]
-        bL = torch.from_numpy(mna.rhs(nL, 102)).to(dev)
-        planL.refine(axL, luL, bL, tol=args.tol, maxit=20)   # warm (scratch allocation)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        _, itL, etaL = planL.refine(axL, luL, bL, tol=args.tol, maxit=20)
-        refine_ms = (time.perf_counter() - t0) * 1e3
-        large = {"n": nL, "nnz": int(dL.size), "nnz_lu": anL.nnz_lu, "levels": anL.num_lev,
-                 "refactor_ms": round(ms, 3), "refine_iters": itL, "eta": etaL, "refine_ms": round(refine_ms, 3),
-                 "host_analysis_s": round(t_an, 2)}
-        log(f"[bench] n=500k: analysis+plan {t_an:.1f}s refactor {ms:.3f} ms, refine {itL} iters {refine_ms:.1f} ms "
-            f"eta {etaL:.2e}")
-        del planL, luL, base, axL
+    large = None if args.no_large else bench_large(dev, log, args.tol)
 
     # ---------------- CPU baseline (NICSLU, reference code) ------------
     cpu = None
