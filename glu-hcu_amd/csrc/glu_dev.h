@@ -65,6 +65,7 @@ struct WinDev {
     const uint32_t *cup;
     const uint32_t *ment;
     const MergeSrc *msrc;
+    const uint4 *mrec;
     const uint2 *dvd;
 };
 
@@ -86,6 +87,11 @@ struct FactorDev {
     const uint32_t *ment_long; // long merge entries at the head of each level [num_lev + 1]
     const uint32_t *msrc_ptr;  // per merge entry [entries + 1]
     const MergeSrc *msrc;      // sources in increasing column order
+    // per merge entry e, its first two sources in one place (every merge
+    // entry has at least two): mrec[2e] = {target, L, U, pivot position of
+    // source 0}, mrec[2e + 1] = {L, U, pivot position of source 1, source
+    // count}; sources from the third on are read from msrc
+    const uint4 *mrec;
     const uint2 *dvd;          // division items, by level
     const uint32_t *dvd_ptr;   // [num_lev + 1]
     int32_t num_lev;

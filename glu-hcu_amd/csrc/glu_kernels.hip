@@ -153,17 +153,13 @@ __device__ __forceinline__ void factor_ctasks(const FactorDev &F, float *__restr
     }
 }
 
-// Merge entry e: every update of this level in increasing column order,
-// t += -(l * u) — the order a sequential pass over the level applies them.
-// Sources are fetched four at a time so the loads overlap; the additions
-// stay sequential.
-__device__ __forceinline__ void factor_merge(const FactorDev &F, float *__restrict__ val, uint32_t e,
-                                             float pert, int perturb)
+// Sources 2.. of merge entry e subtracted from v in order (four loaded at a
+// time so that the loads overlap; the additions stay sequential).
+__device__ __forceinline__ void factor_merge_rest(const FactorDev &F, float *__restrict__ val, uint32_t e, float &v,
+                                                  float pert, int perturb)
 {
-    const uint32_t t = F.ment[e];
-    float v = val[t];
     const uint32_t s1 = F.msrc_ptr[e + 1];
-    uint32_t s = F.msrc_ptr[e];
+    uint32_t s = F.msrc_ptr[e] + 2;
     for (; s + 4 <= s1; s += 4) {
         float p[4];
 #pragma unroll
@@ -178,6 +174,25 @@ __device__ __forceinline__ void factor_merge(const FactorDev &F, float *__restri
         const MergeSrc m = F.msrc[s];
         v = v + (-((val[m.lpos] / gesp(val[m.dpos], pert, perturb)) * val[m.upos]));
     }
+}
+
+// Merge entry e: every update of this level in increasing column order,
+// t += -(l * u) — the order a sequential pass over the level applies them.
+// Sources are fetched four at a time so the loads overlap; the additions
+// stay sequential.
+__device__ __forceinline__ void factor_merge(const FactorDev &F, float *__restrict__ val, uint32_t e,
+                                             float pert, int perturb)
+{
+    // the target and the first two sources from the entry's record (one
+    // dependent table load instead of three)
+    const uint4 r0 = F.mrec[2 * e], r1 = F.mrec[2 * e + 1];
+    const uint32_t t = r0.x;
+    float v = val[t];
+    const float p0 = (val[r0.y] / gesp(val[r0.w], pert, perturb)) * val[r0.z];
+    const float p1 = (val[r1.x] / gesp(val[r1.z], pert, perturb)) * val[r1.y];
+    v = v + (-p0);
+    v = v + (-p1);
+    if (r1.w > 2) factor_merge_rest(F, val, e, v, pert, perturb);
     val[t] = v;
 }
 
@@ -411,6 +426,7 @@ __device__ __forceinline__ FactorDev window_view(const FactorDev &F, const WinDe
     T.cup = W.cup;
     T.ment = W.ment;
     T.msrc = W.msrc;
+    T.mrec = W.mrec;
     T.dvd = W.dvd;
     return T;
 }
@@ -424,6 +440,7 @@ __device__ __forceinline__ WinDev load_window(const FactorDev &F, int w)
     W.cup = glb(W.cup);
     W.ment = glb(W.ment);
     W.msrc = glb(W.msrc);
+    W.mrec = glb(W.mrec);
     W.dvd = glb(W.dvd);
     return W;
 }
@@ -1064,6 +1081,7 @@ __device__ __forceinline__ void globalize(FactorDev &F)
     F.ment_long = glb(F.ment_long);
     F.msrc_ptr = glb(F.msrc_ptr);
     F.msrc = glb(F.msrc);
+    F.mrec = glb(F.mrec);
     F.dvd = glb(F.dvd);
     F.dvd_ptr = glb(F.dvd_ptr);
     F.win = glb(F.win);

@@ -737,9 +737,24 @@ struct Window {
     std::vector<uint32_t> ctgt, cup;
     std::vector<uint32_t> ment;
     std::vector<MergeSrc> msrc;
+    std::vector<uint4> mrec;
     std::vector<uint2> dvd;
     uint32_t c0 = 0, g0 = 0, m0 = 0, s0 = 0, d0 = 0;   // first task / target / item of each kind
 };
+
+// The records {target, L, U, pivot of source 0}, {L, U, pivot of source 1,
+// source count} of merge entries [e0, e1), positions through map.
+template <typename Map>
+void merge_records(const std::vector<uint32_t> &ment, const std::vector<uint32_t> &msrc_ptr,
+                   const std::vector<MergeSrc> &msrc, uint32_t e0, uint32_t e1, Map map, std::vector<uint4> &out)
+{
+    for (uint32_t e = e0; e < e1; ++e) {
+        const uint32_t s = msrc_ptr[e], cnt = msrc_ptr[e + 1] - s;
+        const MergeSrc &a = msrc[s], &b = msrc[s + 1];
+        out.push_back(make_uint4(map(ment[e]), map(a.lpos), map(a.upos), map(a.dpos)));
+        out.push_back(make_uint4(map(b.lpos), map(b.upos), map(b.dpos), cnt));
+    }
+}
 
 // The CSC positions the items of step l read or write, each once per call
 // of f (duplicates possible).  Division items are not step items: they run
@@ -790,6 +805,7 @@ void make_window(const Tables &T, int a, int b, const std::vector<uint32_t> &slo
     const uint32_t m1 = T.ment_ptr[b];
     w.s0 = T.msrc_ptr[w.m0];
     for (uint32_t e = w.m0; e < m1; ++e) w.ment.push_back(slot(T.ment[e]));
+    merge_records(T.ment, T.msrc_ptr, T.msrc, w.m0, m1, slot, w.mrec);
     for (uint32_t k = w.s0; k < T.msrc_ptr[m1]; ++k) {
         const MergeSrc &m = T.msrc[k];
         w.msrc.push_back(MergeSrc{slot(m.lpos), slot(m.upos), slot(m.dpos), 0});
@@ -1285,11 +1301,14 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
     const size_t o_mlong = U.put(T.ment_long);
     const size_t o_msptr = U.put(T.msrc_ptr);
     const size_t o_msrc = U.put(T.msrc);
+    std::vector<uint4> mrec;
+    merge_records(T.ment, T.msrc_ptr, T.msrc, 0, (uint32_t)T.ment.size(), [](uint32_t q) { return q; }, mrec);
+    const size_t o_mrec = U.put(mrec);
     const size_t o_dvd = U.put(T.dvd);
     const size_t o_dptr = U.put(T.dvd_ptr);
     p->ndiv = (uint32_t)T.dvd.size();
     std::vector<size_t> o_wpos(wins.size()), o_wcln(wins.size()), o_wctgt(wins.size()), o_wcup(wins.size()),
-        o_wment(wins.size()), o_wmsrc(wins.size()), o_wdvd(wins.size());
+        o_wment(wins.size()), o_wmsrc(wins.size()), o_wmrec(wins.size()), o_wdvd(wins.size());
     for (size_t w = 0; w < wins.size(); ++w) {
         o_wpos[w] = U.put(wins[w].pos);
         o_wcln[w] = U.put(wins[w].cln);
@@ -1297,6 +1316,7 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
         o_wcup[w] = U.put(wins[w].cup);
         o_wment[w] = U.put(wins[w].ment);
         o_wmsrc[w] = U.put(wins[w].msrc);
+        o_wmrec[w] = U.put(wins[w].mrec);
         o_wdvd[w] = U.put(wins[w].dvd);
     }
     const size_t o_wdev = U.reserve(sizeof(WinDev) * std::max<size_t>(wins.size(), 1));
@@ -1365,6 +1385,7 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
     F.ment_long = u32p(o_mlong);
     F.msrc_ptr = u32p(o_msptr);
     F.msrc = reinterpret_cast<const MergeSrc *>(A + o_msrc);
+    F.mrec = reinterpret_cast<const uint4 *>(A + o_mrec);
     F.dvd = reinterpret_cast<const uint2 *>(A + o_dvd);
     F.dvd_ptr = u32p(o_dptr);
     const uint32_t *upos_d = u32p(o_upos);
@@ -1389,6 +1410,7 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
             d.cup = reinterpret_cast<const uint32_t *>(shifted(o_wcup[w], x.g0, sizeof(uint32_t)));
             d.ment = reinterpret_cast<const uint32_t *>(shifted(o_wment[w], x.m0, sizeof(uint32_t)));
             d.msrc = reinterpret_cast<const MergeSrc *>(shifted(o_wmsrc[w], x.s0, sizeof(MergeSrc)));
+            d.mrec = reinterpret_cast<const uint4 *>(shifted(o_wmrec[w], 2 * (size_t)x.m0, sizeof(uint4)));
             d.dvd = reinterpret_cast<const uint2 *>(shifted(o_wdvd[w], x.d0, sizeof(uint2)));
         }
         if (!wd.empty())
