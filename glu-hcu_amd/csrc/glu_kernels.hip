@@ -257,10 +257,9 @@ __device__ __forceinline__ void factor_merge_wave(const FactorDev &F, float *__r
         lds[lane] = p;
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        if (lane == 0) {
-            const uint32_t cnt = min((uint32_t)kWave, s1 - base);
-            v = serial_sub(v, lds, cnt, false);   // v + (-p) is v - p exactly
-        }
+        // all 64 slots: lanes past the entry's last source hold +0, and
+        // v - 0 is v exactly, so the full-slice path serves partial passes
+        if (lane == 0) v = serial_sub(v, lds, kWave, false);   // v + (-p) is v - p exactly
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     }
@@ -538,7 +537,7 @@ __device__ __forceinline__ void tri_row_wave(const TriDev &T, const float *__res
         lds[lane] = p;
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        if (lane == 0) acc = serial_sub(acc, lds, hi - lo, T.desc != 0);
+        if (lane == 0) acc = serial_sub(acc, lds, kWave, T.desc != 0);   // lanes past hi - lo hold +0
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     }
@@ -727,7 +726,7 @@ __device__ __forceinline__ void df_long_task(const TriDev &T, const float *__res
         lds[lane] = p;
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        if (lane == 0) acc = serial_sub(acc, lds, hi - lo, T.desc != 0);
+        if (lane == 0) acc = serial_sub(acc, lds, kWave, T.desc != 0);   // lanes past hi - lo hold +0
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     }
