@@ -241,6 +241,11 @@ __device__ __forceinline__ float serial_sub(float acc, const float *p, uint32_t 
 // The same for an entry with many sources, by one wave: the products of 64
 // sources at a time in parallel, then lane 0 adds them in source order from
 // the wave's LDS slice (64 floats) — the same arithmetic, sequence included.
+// kFull: every pass subtracts all 64 slots (lanes past the entry's last
+// source hold +0, and v - 0 is v exactly), so partial passes take the
+// full-slice path too; it holds more registers, which the batch kernels
+// (one matrix per workgroup, near their register limit) cannot spare.
+template <bool kFull = true>
 __device__ __forceinline__ void factor_merge_wave(const FactorDev &F, float *__restrict__ val, uint32_t e,
                                                   float pert, int perturb, uint32_t lane, float *lds)
 {
@@ -257,9 +262,10 @@ __device__ __forceinline__ void factor_merge_wave(const FactorDev &F, float *__r
         lds[lane] = p;
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        // all 64 slots: lanes past the entry's last source hold +0, and
-        // v - 0 is v exactly, so the full-slice path serves partial passes
-        if (lane == 0) v = serial_sub(v, lds, kWave, false);   // v + (-p) is v - p exactly
+        if (lane == 0) {
+            const uint32_t cnt = kFull ? kWave : min((uint32_t)kWave, s1 - base);
+            v = serial_sub(v, lds, cnt, false);   // v + (-p) is v - p exactly
+        }
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     }
@@ -322,7 +328,8 @@ __device__ __forceinline__ void factor_level(const FactorDev &F, float *__restri
 // for its values.  Same items and order as factor_level.
 // kDeferDiv: no division items here (the caller divides after the last
 // step: no update reads a divided L value, each divides on the fly).
-template <bool kDeferDiv = false>
+// kFullMerge: see factor_merge_wave.
+template <bool kDeferDiv = false, bool kFullMerge = true>
 __device__ __forceinline__ void factor_levels_wg(const FactorDev &F, float *__restrict__ val, int l0, int l1,
                                                  float pert, int perturb, float *lds)
 {
@@ -337,7 +344,8 @@ __device__ __forceinline__ void factor_levels_wg(const FactorDev &F, float *__re
     }
     for (int lvl = l0; lvl < l1; ++lvl) {
         const LevelSpan s = level_span(F, lvl);
-        for (uint32_t e = w; e < s.ml; e += nw) factor_merge_wave(F, val, s.m0 + e, pert, perturb, lane, lds);
+        for (uint32_t e = w; e < s.ml; e += nw)
+            factor_merge_wave<kFullMerge>(F, val, s.m0 + e, pert, perturb, lane, lds);
         for (uint32_t t = s.c0 + w; t < s.c0 + s.nc; t += nw) {
             const bool more = t + nw < s.c0 + s.nc;
             CTab nxt;
@@ -446,7 +454,7 @@ __device__ __forceinline__ WinDev load_window(const FactorDev &F, int w)
 
 // Steps [W.l0, W.l1) inside one workgroup with the values they touch staged
 // in LDS (slots): load, one barrier per step, store back.
-template <bool kDeferDiv = false>
+template <bool kDeferDiv = false, bool kFullMerge = true>
 __device__ __forceinline__ void factor_window(const FactorDev &F, const WinDev &W, float *__restrict__ val,
                                               float *slots, float pert, int perturb, float *wlds)
 {
@@ -454,7 +462,7 @@ __device__ __forceinline__ void factor_window(const FactorDev &F, const WinDev &
     for (uint32_t s = tid; s < W.nslot; s += nt) slots[s] = val[W.pos[s]];
     __syncthreads();
     const FactorDev T = window_view(F, W);
-    factor_levels_wg<kDeferDiv>(T, slots, W.l0, W.l1, pert, perturb, wlds);
+    factor_levels_wg<kDeferDiv, kFullMerge>(T, slots, W.l0, W.l1, pert, perturb, wlds);
     for (uint32_t s = tid; s < W.nslot; s += nt) val[W.pos[s]] = slots[s];
     __syncthreads();
 }
@@ -471,11 +479,11 @@ __device__ __forceinline__ void factor_all_wg(const FactorDev &F, float *__restr
     int l = 0;
     for (int w = 0; w < F.nwin; ++w) {
         const WinDev W = load_window(F, w);
-        factor_levels_wg<true>(F, val, l, W.l0, pert, perturb, wlds);
-        factor_window<true>(F, W, val, slots, pert, perturb, wlds);
+        factor_levels_wg<true, false>(F, val, l, W.l0, pert, perturb, wlds);
+        factor_window<true, false>(F, W, val, slots, pert, perturb, wlds);
         l = W.l1;
     }
-    factor_levels_wg<true>(F, val, l, F.num_lev + 1, pert, perturb, wlds);
+    factor_levels_wg<true, false>(F, val, l, F.num_lev + 1, pert, perturb, wlds);
     const uint32_t nd = F.dvd_ptr[F.num_lev];
     for (uint32_t t = threadIdx.x; t < nd; t += blockDim.x) factor_divide(val, F.dvd[t], pert, perturb);
     __syncthreads();
