@@ -2032,6 +2032,31 @@ __global__ void __launch_bounds__(1024) k_batch_group_refine(const MatDesc *__re
     wg_refine<kMode>(descs + m, io, tol, maxit, sh, hs, xs, flags);
 }
 
+// Distinct patterns whose largest order leaves the vector out of LDS: each
+// workgroup keeps its own matrix's vector in LDS when it fits the launch's
+// dynamic LDS (lds_bytes), else only the ready bits.
+__global__ void __launch_bounds__(1024) k_batch_group_refine_mixed(const MatDesc *__restrict__ descs,
+                                                                   const MatIO *__restrict__ ios,
+                                                                   const uint32_t *__restrict__ order, double tol,
+                                                                   int maxit, uint32_t lds_bytes)
+{
+    __shared__ double sh[1024 / kWave];
+    __shared__ double hs[kKrylovLds];
+    const uint32_t m = order[blockIdx.x];
+    MatIO io = ios[m];
+    globalize(io);
+    const uint32_t n = descs[m].n;
+    uint32_t *flags;
+    float *xs;
+    if (4u * ((n + 31) / 32) + 4u * n <= lds_bytes) {
+        batch_lds<kSweepLds>(n, io.t32, flags, xs);
+        wg_refine<kSweepLds>(descs + m, io, tol, maxit, sh, hs, xs, flags);
+    } else {
+        batch_lds<kSweepFlags>(n, io.t32, flags, xs);
+        wg_refine<kSweepFlags>(descs + m, io, tol, maxit, sh, hs, xs, flags);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // ABFT column checksums (src/symbolic.cc:163-197), a batch of value sets
 // ---------------------------------------------------------------------------
@@ -2268,8 +2293,9 @@ void batch_group(const MatDesc *descs, const void *ios, const uint32_t *order, i
         k_batch_group_factor<true><<<count, threads, fact_lds, st>>>(descs, io, order, perturb);
     if (mode == kSweepLds)
         k_batch_group_refine<kSweepLds><<<count, threads, ref_lds, st>>>(descs, io, order, tol, maxit);
-    else if (mode == kSweepFlags)
-        k_batch_group_refine<kSweepFlags><<<count, threads, ref_lds, st>>>(descs, io, order, tol, maxit);
+    else if (mode == kSweepFlags)   // per matrix: vector in LDS where it fits ref_lds
+        k_batch_group_refine_mixed<<<count, threads, ref_lds, st>>>(descs, io, order, tol, maxit,
+                                                                   (uint32_t)ref_lds);
     else
         k_batch_group_refine<kSweepLevels><<<count, threads, ref_lds, st>>>(descs, io, order, tol, maxit);
 }
@@ -2315,6 +2341,7 @@ size_t enable_lds()
         reinterpret_cast<const void *>(&k_batch_group_refine<kSweepLds>),
         reinterpret_cast<const void *>(&k_batch_group_refine<kSweepFlags>),
         reinterpret_cast<const void *>(&k_batch_group_refine<kSweepLevels>),
+        reinterpret_cast<const void *>(&k_batch_group_refine_mixed),
         reinterpret_cast<const void *>(&k_tri_df),
         reinterpret_cast<const void *>(&k_factor_window),
     };

@@ -2037,8 +2037,16 @@ extern "C" glu_status glu_group_solve(glu_group *g, const double *const *d_ax, c
     }
     const size_t cap = launch::enable_lds();
     const int mode = sweep_mode((uint32_t)nmax, cap);
+    // ready bits only for the largest order: each matrix whose vector fits
+    // keeps it in LDS too (the launch's LDS: the largest such need)
+    size_t ref_lds = launch::batch_lds_bytes(mode, (uint32_t)nmax);
+    if (mode == 1)
+        for (glu_plan *p : g->plans) {
+            const size_t full = launch::batch_lds_bytes(2, p->n);
+            ref_lds = std::max(ref_lds, full <= cap ? full : launch::batch_lds_bytes(1, p->n));
+        }
     launch::batch_group(g->d_descs, g->d_ios, g->d_order, count, tol, maxit, (int)(flags & 1u), g->threads, mode, tmax,
-                        launch::batch_lds_bytes(mode, (uint32_t)nmax), st);
+                        ref_lds, st);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(GLU_ERR_DEVICE, std::string("group launch: ") + hipGetErrorString(e));
     // ios is read by the kernel from device memory; the host copy may go.
