@@ -25,6 +25,8 @@
 // host solve do; packed FP32 disabled.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include <algorithm>
 #include <cstdint>
 
@@ -2321,8 +2323,14 @@ size_t enable_lds()
 {
     int dev = 0, lim = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 0;
+    // once per device (the attributes stay set for the process); plans may
+    // be created from several host threads
+    static std::mutex mu;
+    static int done_dev = -1;
+    static size_t done_avail = 0;
+    std::lock_guard<std::mutex> lock(mu);
+    if (dev == done_dev) return done_avail;
     if (hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return 0;
-    const size_t avail = (size_t)lim > 8192 ? (size_t)lim - 8192 : 0;  // static LDS of the kernels (< 8 KiB)
     const void *fns[] = {
         reinterpret_cast<const void *>(&k_batch_same_factor<true>),
         reinterpret_cast<const void *>(&k_batch_same_factor<false>),
@@ -2345,7 +2353,21 @@ size_t enable_lds()
         reinterpret_cast<const void *>(&k_tri_df),
         reinterpret_cast<const void *>(&k_factor_window),
     };
+    // the limit less the largest static LDS among them (their wave slices
+    // and reduction arrays, ~4 KiB); 8 KiB if the attributes are unavailable
+    size_t stat = 0;
+    for (const void *f : fns) {
+        hipFuncAttributes a;
+        if (hipFuncGetAttributes(&a, f) != hipSuccess) {
+            stat = 8192;
+            break;
+        }
+        stat = std::max(stat, (size_t)a.sharedSizeBytes);
+    }
+    const size_t avail = (size_t)lim > stat ? (size_t)lim - stat : 0;
     for (const void *f : fns) (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)avail);
+    done_dev = dev;
+    done_avail = avail;
     return avail;
 }
 
