@@ -308,7 +308,10 @@ struct glu_plan {
     uint32_t df_nextra = 0;      // its positions with a second slot
     uint32_t tdf_nval = 0, tdf_nextra = 0;   // the same for A' x = b
     float *d_tlucsr = nullptr;   // glu_refine's A' x = b value copy
-    int max_windows = 1;         // LDS windows allowed (from the last level)
+    // LDS windows allowed (from the last level); windows hold narrow steps
+    // only, so small orders get one whatever the cap, and n=500k MNA gains
+    // up to ~16 (refactor 14.3 -> 12.4 ms; no change beyond)
+    int max_windows = 64;
     int krylov = kDefaultKrylov;  // batch refinement: FGMRES restart length (0: plain)
     int nwin = 0;                // LDS windows of the factorization
     int win_steps = 0;           // steps they cover

@@ -31,7 +31,7 @@ FUSE_CANDIDATES = (256, 512, 1024, 2048, 4096, 8192, 16384)
 THREAD_CANDIDATES = (256, 512, 1024)
 
 
-WINDOW_CANDIDATES = (0, -1, 2)
+WINDOW_CANDIDATES = (0, -1, 1)
 KRYLOV_CANDIDATES = (2, -1, 1, 4)
 
 

@@ -124,7 +124,7 @@ typedef struct glu_plan_opts {
     int32_t use_graph;      /* 1 = replay the level schedule from a hipGraph */
     int32_t lds_windows;    /* runs of narrow levels factored with their values
                                staged in LDS, counted from the last level:
-                               0 = default (1), -1 = none, k > 0 = at most k */
+                               0 = default (64), -1 = none, k > 0 = at most k */
     int32_t solve_dataflow; /* narrow levels of the single-matrix solves run
                                barrier-free in one workgroup: 0 = default (on
                                when LDS holds the ready bits), 1 = on, -1 = off */
