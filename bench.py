@@ -55,22 +55,75 @@ def alg_bytes_refine(nnz_a, nnz_lu, n, share):
     return 12 * nnz_a + 8 * (n + 1) / share + 8 * nnz_lu + 56 * n
 
 
-def fwd_error(data, indices, indptr, n, b, x):
-    """Forward error |x - x*|_inf / |x*|_inf.  x* is an FP64 sparse direct
-    solve of the original matrix (scipy SuperLU, partial pivoting) refined
-    twice with residuals in the host's extended precision, so x* itself is
-    accurate to far below the errors measured."""
+def fp64_solution(an, data, indices, indptr, b):
+    """x* for A x = b on the host: an FP64 sparse LU (SuperLU) of A with its
+    rows and columns in the analysis's MC64 + AMD order (natural order, no
+    pivoting: the permuted matrix carries the matching on its diagonal),
+    then two refinement steps with the residual in the host's extended
+    precision, so x* is accurate far below the errors it measures.
+    Independent of the GPU path: SuperLU factors A's FP64 values itself."""
     import numpy as np
     import scipy.sparse as sp
     import scipy.sparse.linalg as sla
+    n = an.n
     A = sp.csc_matrix((data, indices, indptr), shape=(n, n))
-    lu = sla.splu(A)
+    rp = np.asarray(an.col_perm, dtype=np.int64)   # GLU row r = caller row col_perm[r]
+    cq = np.asarray(an.row_perm, dtype=np.int64)   # GLU column c = caller column row_perm[c]
+    lu = sla.splu(sp.csc_matrix(A[rp, :][:, cq]), permc_spec="NATURAL", diag_pivot_thresh=0.0,
+                  options=dict(Equil=False))
+
+    def solve(r):
+        x = np.empty(n)
+        x[cq] = lu.solve(r[rp])
+        return x
+
     b64 = np.asarray(b, dtype=np.float64)
-    xs = lu.solve(b64)
+    xs = solve(b64)
     Al = sp.csr_matrix(A).astype(np.longdouble)
     for _ in range(2):
         r = b64.astype(np.longdouble) - Al @ xs.astype(np.longdouble)
-        xs = xs + lu.solve(r.astype(np.float64))
+        xs = xs + solve(r.astype(np.float64))
+    return xs
+
+
+def accuracy(an, data, indices, indptr, b, x):
+    """Forward error |x - x*|_inf / |x*|_inf against fp64_solution, the
+    normwise backward error eta = |r|_inf / (|A|_inf |x|_inf + |b|_inf) and
+    the componentwise backward error omega = max_i |r_i| / (|A||x| + |b|)_i,
+    with r = b - A x formed in extended precision."""
+    import numpy as np
+    import scipy.sparse as sp
+    n = an.n
+    x = np.asarray(x, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    xs = fp64_solution(an, data, indices, indptr, b)
+    A = sp.csr_matrix(sp.csc_matrix((data, indices, indptr), shape=(n, n)))
+    r = np.abs((b.astype(np.longdouble) - A.astype(np.longdouble) @ x.astype(np.longdouble)).astype(np.float64))
+    absA = abs(A)
+    den = absA @ np.abs(x) + np.abs(b)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        comp = np.where(den > 0, r / den, np.where(r > 0, np.inf, 0.0))
+    return {"fwd": float(np.abs(x - xs).max() / np.abs(xs).max()),
+            "eta": float(r.max() / (absA.sum(axis=1).max() * np.abs(x).max() + np.abs(b).max())),
+            "omega": float(comp.max())}
+
+
+def summarize(accs):
+    """max / median of the accuracy figures over a set of results."""
+    import numpy as np
+    out = {}
+    for k in ("fwd", "eta", "omega"):
+        v = np.array([a[k] for a in accs])
+        out[f"{k}_max"] = float(v.max())
+        out[f"{k}_median"] = float(np.median(v))
+    out["count"] = len(accs)
+    return out
+
+
+def fwd_error(an, data, indices, indptr, b, x):
+    """Forward error |x - x*|_inf / |x*|_inf against fp64_solution."""
+    import numpy as np
+    xs = fp64_solution(an, data, indices, indptr, b)
     return float(np.abs(np.asarray(x, dtype=np.float64) - xs).max() / np.abs(xs).max())
 
 
@@ -111,14 +164,21 @@ def bench_small(dev, log):
     ms = _events(step, 11)
     fms = _events(lambda: (lu.copy_(base), plan.factor(lu)), 11)
     b64 = mna.rhs(n, 101)
-    xr, _, eta = plan.refine(ax, lu, torch.from_numpy(b64).to(dev), tol=1e-10, maxit=20)
+    bd = torch.from_numpy(b64).to(dev)
+    xr, it, eta = plan.refine(ax, lu, bd, tol=1e-10, maxit=20)
+    xt, itt, etat = plan.refine(ax, lu, bd, tol=1e-13, maxit=30)
+    acc32 = accuracy(an, d, i, p, b64, x32.cpu().numpy())
+    accr = accuracy(an, d, i, p, b64, xr.cpu().numpy())
+    acct = accuracy(an, d, i, p, b64, xt.cpu().numpy())
     out = {"n": n, "nnz": int(d.size), "nnz_lu": an.nnz_lu, "levels": an.num_lev,
            "factor_solve_ms": round(ms, 4), "factor_ms": round(fms, 4),
            "us_per_level": round(fms * 1e3 / max(an.num_lev, 1), 2),
-           "fwd_err_fp32": fwd_error(d, i, p, n, b64, x32.cpu().numpy()),
-           "fwd_err_refined": fwd_error(d, i, p, n, b64, xr.cpu().numpy()), "refined_eta": eta}
+           "accuracy_fp32_solve": acc32,
+           "accuracy_refined_1e-10": {**accr, "solves": it},
+           "accuracy_refined_1e-13": {**acct, "solves": itt}}
     log(f"[bench] n=4960: factor+solve {ms:.3f} ms (factor {fms:.3f} ms, {an.num_lev} levels); forward error "
-        f"vs FP64: FP32 solve {out['fwd_err_fp32']:.1e}, refined {out['fwd_err_refined']:.1e}")
+        f"vs FP64: FP32 solve {acc32['fwd']:.1e}, refined to 1e-10 {accr['fwd']:.1e} ({it} solves), "
+        f"to 1e-13 {acct['fwd']:.1e} ({itt} solves)")
     plan.close()
     return out
 
@@ -144,15 +204,31 @@ def bench_uneven(dev, log, tol, maxit):
         bs.append(torch.from_numpy(mna.rhs(n, 4500 + m)).to(dev))
     g = Group(plans)
     t_plan = time.perf_counter() - t0 - t_an
-    g.solve(axs, bs, tol=tol, maxit=maxit)
-    torch.cuda.synchronize()
-    reps = 5
+
+    def launch_ms(grp, reps=5):
+        grp.solve(axs, bs, tol=tol, maxit=maxit)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            res = grp.solve(axs, bs, tol=tol, maxit=maxit)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps * 1e3, res
+
+    default_ms, _ = launch_ms(g)
+    # group-wide tiles (workgroup size), refinement restart and LDS windows,
+    # autotuned on this group; the tuned group is the one measured
+    from pyglu import autotune
     t0 = time.perf_counter()
-    for _ in range(reps):
-        _, _, iters, eta = g.solve(axs, bs, tol=tol, maxit=maxit)
-    torch.cuda.synchronize()
-    ms = (time.perf_counter() - t0) / reps * 1e3
+    with ThreadPoolExecutor(threads) as ex:
+        tune = autotune.tune_group(ans, axs, bs, tol=tol, maxit=maxit, plan_pool=ex)
+        tuned_plans = list(ex.map(lambda a: Plan(a, block_threads=tune.block_threads, krylov=tune.krylov,
+                                                lds_windows=tune.lds_windows), ans))
+    t_tune = time.perf_counter() - t0
+    gt = Group(tuned_plans)
+    ms, (xs, _, iters, eta) = launch_ms(gt)
     it_h, eta_h = iters.cpu().numpy(), eta.cpu().numpy()
+    accs = [accuracy(ans[m], d, i, p, mna.rhs(n, 4500 + m), xs[m].cpu().numpy())
+            for m, (d, i, p, n) in enumerate(mats)]
     # the plain loop over the same matrices
     t0 = time.perf_counter()
     for p, ax, b in zip(plans, axs, bs):
@@ -164,13 +240,20 @@ def bench_uneven(dev, log, tol, maxit):
     out = {"count": len(plans), "sum_n": int(sum(p.n for p in plans)), "max_n": int(max(p.n for p in plans)),
            "max_levels": int(max(p.num_lev for p in plans)), "launch_ms": round(ms, 3),
            "refactors_per_s": round(len(plans) / ms * 1e3, 1), "plain_loop_ms": round(loop_ms, 3),
+           "default_params_launch_ms": round(default_ms, 3),
+           "autotuned": {"block_threads": tune.block_threads, "krylov": tune.krylov,
+                         "lds_windows": tune.lds_windows, "candidates": len(tune.table),
+                         "gain_vs_default": round(default_ms / ms, 3), "tune_s": round(t_tune, 2)},
            "refine_iters_max": int(it_h.max()), "eta_max": float(eta_h.max()),
+           "accuracy": summarize(accs),
            "correct": bool(np.all(eta_h <= tol)), "host_analysis_s": round(t_an, 2),
            "host_analysis_threads": threads, "plans_s": round(t_plan, 2)}
-    log(f"[bench] uneven group: {ms:.3f} ms per launch (plain loop {loop_ms:.1f} ms), "
-        f"iters max {it_h.max()}, eta max {eta_h.max():.2e}")
+    log(f"[bench] uneven group: {ms:.3f} ms per launch autotuned (threads {tune.block_threads}, krylov "
+        f"{tune.krylov}, windows {tune.lds_windows}; defaults {default_ms:.3f} ms; plain loop {loop_ms:.1f} ms), "
+        f"iters max {it_h.max()}, eta max {eta_h.max():.2e}, forward error max {out['accuracy']['fwd_max']:.1e}")
     g.close()
-    for p in plans:
+    gt.close()
+    for p in plans + tuned_plans:
         p.close()
     return out
 
@@ -239,14 +322,98 @@ def bench_large(dev, log, tol):
     planL.refine(axL, luL, bL, tol=tol, maxit=20)   # warm (scratch allocation)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    _, itL, etaL = planL.refine(axL, luL, bL, tol=tol, maxit=20)
+    xL, itL, etaL = planL.refine(axL, luL, bL, tol=tol, maxit=20)
     refine_ms = (time.perf_counter() - t0) * 1e3
+    acc = accuracy(anL, dL, iL, pL, mna.rhs(nL, 102), xL.cpu().numpy())
     log(f"[bench] n=500k: analysis+plan {t_an:.1f}s refactor {ms:.3f} ms, refine {itL} iters {refine_ms:.1f} ms "
-        f"eta {etaL:.2e}")
+        f"eta {etaL:.2e}, forward error {acc['fwd']:.1e}, componentwise backward error {acc['omega']:.1e}")
+    out = {"n": nL, "nnz": int(dL.size), "nnz_lu": anL.nnz_lu, "levels": anL.num_lev,
+           "lds_windows": planL.lds_window_count, "refactor_ms": round(ms, 3), "refine_iters": itL, "eta": etaL,
+           "refine_ms": round(refine_ms, 3), "accuracy": acc, "host_analysis_s": round(t_an, 2)}
     planL.close()
-    return {"n": nL, "nnz": int(dL.size), "nnz_lu": anL.nnz_lu, "levels": anL.num_lev,
-            "refactor_ms": round(ms, 3), "refine_iters": itL, "eta": etaL, "refine_ms": round(refine_ms, 3),
-            "host_analysis_s": round(t_an, 2)}
+    return out
+
+
+def cpu_cores():
+    """(threads to use, affinity cores, cgroup CPU quota or None, online
+    cores): the cores this process may run on (its affinity mask), capped by
+    a cgroup CPU quota when one is set, so that NICSLU's threads are not
+    oversubscribed."""
+    aff = len(os.sched_getaffinity(0))
+    quota = None
+    for path, parse in (("/sys/fs/cgroup/cpu.max", lambda s: s.split()),
+                        ("/sys/fs/cgroup/cpu/cpu.cfs_quota_us", lambda s: [s.strip(), None])):
+        try:
+            with open(path) as f:
+                q, per = parse(f.read())
+            if per is None:
+                with open("/sys/fs/cgroup/cpu/cpu.cfs_period_us") as f:
+                    per = f.read().strip()
+            if q not in ("max", "-1"):
+                quota = max(1, int(int(q) // int(per)))
+            break
+        except (OSError, ValueError):
+            continue
+    return (min(aff, quota) if quota else aff), aff, quota, os.cpu_count()
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(ref, data, indices, indptr, n, ax_h, args, log):
+    """NICSLU (the reference's vendored CPU solver, compiled from its sources
+    into oracle/_ref) on the host cores, per src/nicslu/demo/demop_time.c:
+    Analyze, Factorize (one thread, partial pivoting), then with T threads
+    ReFactorize_MT per value set and one Solve; wall clock (CLOCK_MONOTONIC).
+    The reported value is the batch configuration's refactor + solve rate
+    (passes over the 256 value sets until about cpu_seconds of refactor +
+    solve time); the n=4,960 matrix, the 64-pattern uneven group and, with
+    --cpu-large, the n=500k matrix are timed beside it."""
+    import numpy as np
+    from pyglu import mna
+    T, aff, quota, online = cpu_cores()
+    B = ax_h.shape[0]
+    t0 = time.perf_counter()
+    spent, done, passes, first = 0.0, 0, 0, None
+    while spent < args.cpu_seconds and passes < 1000:
+        res = ref.nicslu_time(data, indices, indptr, n, ax_h, threads=T)
+        first = first or res
+        spent += B * (res["refactor_s"] + res["solve_s"])
+        done += B
+        passes += 1
+    rate = done / spent
+    keep = lambda r: {k: round(float(v) * 1e3, 4) for k, v in r.items() if k != "x"}
+    configs = {"batch_n20k": {**{f"{k[:-2]}_ms": v for k, v in keep(first).items()},
+                              "refactor_solve_per_s": round(rate, 2)}}
+    d, i, p, ns = mna.mna_csc(mna.SPEC_SMALL, mna.SEED_SMALL)
+    r = ref.nicslu_time(d, i, p, ns, np.stack([d] * 20), threads=T)
+    configs["small_n4960"] = {f"{k[:-2]}_ms": v for k, v in keep(r).items()}
+    tot = {"analyze_s": 0.0, "factorize_s": 0.0, "refactor_s": 0.0, "solve_s": 0.0}
+    for m, (spec, seed) in enumerate(mna.uneven_specs()):
+        d, i, p, nm = mna.mna_csc(spec, seed)
+        r = ref.nicslu_time(d, i, p, nm, d[None, :], threads=T)
+        for k in tot:
+            tot[k] += float(r[k])
+    configs["uneven_64"] = {f"{k[:-2]}_ms_sum": round(v * 1e3, 3) for k, v in tot.items()}
+    if args.cpu_large:
+        d, i, p, nl = mna.mna_csc(mna.SPEC_LARGE, mna.SEED_LARGE)
+        r = ref.nicslu_time(d, i, p, nl, d[None, :], threads=T)
+        configs["large_n500k"] = {f"{k[:-2]}_ms": v for k, v in keep(r).items()}
+    log(f"[bench] NICSLU CPU: {rate:.1f} refactors/s on {T} threads ({done} refactors); per config {configs}")
+    return {"value": round(rate, 2), "unit": "refactors/s", "cores": T, "kind": "reference",
+            "sample": f"NICSLU (src/nicslu, -O2 -DSSE2 -DNO_ATOMIC) ReFactorize_MT + Solve, FP64, the {B} batch "
+                      f"matrices x {passes} passes ({done} refactors, {spent:.1f} s of refactor + solve), "
+                      f"{T} threads, {time.perf_counter() - t0:.1f} s wall incl. the other configurations",
+            "cpu_model": cpu_model(), "affinity_cores": aff, "cgroup_quota_cpus": quota, "online_cpus": online,
+            "configs": configs}
 
 
 def main():
@@ -262,6 +429,8 @@ def main():
     ap.add_argument("--no-large", action="store_true", help="skip the n=500k refactor timing")
     ap.add_argument("--no-cpu", action="store_true", help="skip the NICSLU CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--cpu-large", action="store_true",
+                    help="also time NICSLU on the n=500k matrix (its Analyze alone takes minutes)")
     ap.add_argument("--block-threads", type=int, default=0)
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the n=4960, uneven-group and autograd side measurements")
@@ -328,17 +497,18 @@ def main():
     kernel_ms = float(np.mean([s.elapsed_time(e) for s, e in zip(starts, ends)]))
     iters_h = iters.cpu().numpy()
     eta_h = eta.cpu().numpy()
-    ok = bool(np.all(eta_h <= args.tol) and np.all(np.isfinite(x.cpu().numpy())))
-    # host check of a few backward errors on the original matrices
-    import scipy.sparse as sp
-    for b in (0, B // 2, B - 1):
-        A = sp.csc_matrix((ax_h[b], indices, indptr), shape=(n, n))
-        xb = x[b].cpu().numpy()
-        r = b_h[b] - A @ xb
-        e = np.abs(r).max() / (np.abs(A).sum(axis=1).max() * np.abs(xb).max() + np.abs(b_h[b]).max())
-        ok = bool(ok and e <= 2 * args.tol)
-    fwd = max(fwd_error(ax_h[b], indices, indptr, n, b_h[b], x[b].cpu().numpy()) for b in (0, B // 2, B - 1))
-    log(f"[bench] batch forward error vs FP64 (matrices 0, {B // 2}, {B - 1}): {fwd:.2e}")
+    x_h = x.cpu().numpy()
+    ok = bool(np.all(eta_h <= args.tol) and np.all(np.isfinite(x_h)))
+    # every matrix's result on the host: forward error against an FP64
+    # sparse LU, normwise and componentwise backward errors
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(min(16, os.cpu_count() or 1)) as ex:
+        accs = list(ex.map(lambda b: accuracy(an, ax_h[b], indices, indptr, b_h[b], x_h[b]), range(B)))
+    batch_acc = summarize(accs)
+    ok = bool(ok and batch_acc["eta_max"] <= 2 * args.tol)
+    log(f"[bench] batch accuracy over all {B} matrices: forward error max {batch_acc['fwd_max']:.2e} "
+        f"(median {batch_acc['fwd_median']:.2e}), eta max {batch_acc['eta_max']:.2e}, componentwise max "
+        f"{batch_acc['omega_max']:.2e}")
     value = B * K / wall
     log(f"[bench] batch: {wall / K * 1e3:.3f} ms/step (events {kernel_ms:.3f} ms) -> {value:.1f} refactors/s; "
         f"iters {iters_h.min()}..{iters_h.max()} max eta {eta_h.max():.2e} ok={ok}")
@@ -428,22 +598,7 @@ def main():
     if not args.no_cpu:
         from oracle import ref
         if ref.available():
-            threads = min(16, os.cpu_count() or 1)
-            # passes over the batch's value sets until about cpu_seconds of
-            # refactor + solve time have accumulated
-            t0 = time.perf_counter()
-            spent, done, passes = 0.0, 0, 0
-            while spent < args.cpu_seconds and passes < 1000:
-                res = ref.nicslu_time(data, indices, indptr, n, ax_h, threads=threads)
-                spent += B * (res["refactor_s"] + res["solve_s"])
-                done += B
-                passes += 1
-            rate = done / spent
-            cpu = {"value": round(rate, 2), "unit": "refactors/s", "cores": threads, "kind": "reference",
-                   "sample": f"NICSLU (src/nicslu, -O2) ReFactorize_MT + Solve, FP64, the {B} batch matrices "
-                             f"x {passes} passes ({done} refactors, {spent:.1f} s of refactor + solve), "
-                             f"{threads} threads, {time.perf_counter() - t0:.1f} s wall"}
-            log(f"[bench] NICSLU CPU: {rate:.1f} refactors/s on {threads} threads ({done} refactors)")
+            cpu = cpu_baseline(ref, data, indices, indptr, n, ax_h, args, log)
         else:
             log("[bench] oracle/_ref not built: no CPU baseline")
 
@@ -466,8 +621,8 @@ def main():
                    "parallelism": "one workgroup per matrix"},
         "correct": ok,
         "refine_iters_max": int(iters_h.max()),
-        "fwd_err_max_sample": fwd,
         "eta_max": float(eta_h.max()),
+        "accuracy": batch_acc,
         "plain_fp32_loop": {"value": round(loop_rate, 2), "unit": "refactors/s", "sample": S,
                             "single_factor_ms": round(single_factor_ms, 4),
                             "speedup": round(value / loop_rate, 2)},

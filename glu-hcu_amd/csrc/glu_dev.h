@@ -201,7 +201,7 @@ struct MatDesc {
     RefineDev r;
     uint32_t n, nnz;
     uint32_t krylov;   // Arnoldi vectors per refinement cycle (0: plain refinement)
-    uint32_t pad;
+    uint32_t a_kmax;   // most entries in a row or a column of A (rounding bound of the refinement)
 };
 
 }  // namespace glu

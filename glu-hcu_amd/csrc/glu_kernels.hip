@@ -619,13 +619,43 @@ __device__ __forceinline__ bool df_ready(const uint32_t *flags, uint32_t c)
     return (__hip_atomic_load(flags + (c >> 5), __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) >> (c & 31u)) & 1u;
 }
 
-// Bound on the polls of a long-lane-row task (far above any real wait: a
-// sweep takes milliseconds, a poll tens of nanoseconds).
+// Bound on the polls of one wait for a ready bit (far above any real wait:
+// a whole sweep takes at most tens of milliseconds, a poll tens of
+// nanoseconds).
 constexpr uint32_t kSpinCap = 1u << 24;
 
 __device__ __forceinline__ void df_set(uint32_t *flags, uint32_t r)
 {
     __hip_atomic_fetch_or(flags + (r >> 5), 1u << (r & 31u), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Waits for ready bits.  Every dependency of a row is finished in an
+// earlier task or earlier in the row's own task, and a task's lanes poll
+// together, so every wait ends; the plan builder checks exactly this on the
+// host for every sweep table before any launch (an inconsistent table is
+// refused with GLU_ERR_INVALID).  On top of that, the waits of the
+// level-synchronous path's dataflow runs and of long lane rows are bounded
+// (kSpinCap polls): a row whose wait gives up completes with NaN, so its
+// dependents, and every wave, still finish, and the NaN result is reported
+// (glu_refine: GLU_ERR_NUMERIC; batched solves: eta = NaN).  The waits of
+// the short-row and wave-row paths, the hot loops of the batched
+// refinement, stay unbounded: a bound there costs ~4% of the batch step.
+__device__ __forceinline__ float df_nan() { return __builtin_nanf(""); }
+
+// x[c] once row c's ready bit is set (acquire), polling at most kSpinCap
+// times (with s_sleep between polls when kSleep); NaN if the wait gives up.
+// A ready bit that is already set costs one load and a branch.
+template <bool kSleep>
+__device__ __forceinline__ float df_get(const uint32_t *flags, const float *x, uint32_t c)
+{
+    if (!df_ready(flags, c)) {
+        uint32_t s = 0;
+        do {
+            if (kSleep) __builtin_amdgcn_s_sleep(1);
+            if (++s >= kSpinCap) return df_nan();
+        } while (!df_ready(flags, c));
+    }
+    return x[c];
 }
 
 // Up to 64 rows of at most kDfLaneRow entries, one per lane: each lane
@@ -650,7 +680,7 @@ __device__ __forceinline__ void df_short_task(const TriDev &T, const float *__re
     }
     uint32_t cidx[K];
     float cval[K];
-    uint32_t cn = 0, ci = 0;
+    uint32_t cn = 0, ci = 0, stalls = 0;
     while (__ballot(pending)) {
         bool progressed = false;
         if (pending) {
@@ -690,7 +720,14 @@ __device__ __forceinline__ void df_short_task(const TriDev &T, const float *__re
                 progressed = true;
             }
         }
-        if (!__ballot(progressed)) __builtin_amdgcn_s_sleep(1);
+        if (!__ballot(progressed)) {
+            if (++stalls >= kSpinCap && pending) {   // give up: the row completes with NaN
+                x[r] = df_nan();
+                df_set(flags, r);
+                pending = false;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
     }
 }
 
@@ -708,8 +745,7 @@ __device__ __forceinline__ void df_long_task(const TriDev &T, const float *__res
         for (uint32_t q = b + lane; q < e; q += kWave) {
             const uint32_t c = T.idx[q];
             const float v = vals[T.vpos ? T.vpos[q] : q];
-            while (!df_ready(flags, c)) __builtin_amdgcn_s_sleep(1);
-            sum += v * x[c];
+            sum += v * df_get<true>(flags, x, c);
         }
         sum = wave_sum(sum);
         if (lane == 0) {
@@ -730,8 +766,7 @@ __device__ __forceinline__ void df_long_task(const TriDev &T, const float *__res
         if (q < hi) {
             const uint32_t cc = T.idx[q];
             const float v = vals[T.vpos ? T.vpos[q] : q];
-            while (!df_ready(flags, cc)) __builtin_amdgcn_s_sleep(1);
-            p = v * x[cc];
+            p = v * df_get<true>(flags, x, cc);
         }
         lds[lane] = p;
         __builtin_amdgcn_wave_barrier();
@@ -1230,8 +1265,7 @@ __device__ __forceinline__ void df2_issue(const DfDev &D, const float *__restric
 // A task of lane rows some longer than kDfK entries: each lane walks its
 // row kDfK entries at a time (the slots of entry e at base + 64 e + lane),
 // consuming ready entries in the row's order as the short-row path does.
-// Polls are bounded (kSpinCap) so that every wave exits even if a table
-// were inconsistent.
+// Polls are bounded (kSpinCap); a row still pending then completes with NaN.
 __device__ __forceinline__ void df2_lane_long(const DfDev &D, const float *__restrict__ sv, float *x,
                                               uint32_t *flags, const DfTask &t, uint32_t lane)
 {
@@ -1304,6 +1338,10 @@ __device__ __forceinline__ void df2_lane_long(const DfDev &D, const float *__res
             }
         }
     }
+    if (pending) {   // gave up: the row completes with NaN
+        x[t.row] = df_nan();
+        df_set(flags, t.row);
+    }
 }
 
 // Waits spin on the LDS ready bits without s_sleep: measured on the n=20k
@@ -1324,8 +1362,7 @@ __device__ __forceinline__ void df2_run(const DfDev &D, const float *__restrict_
         bool pending = lane < t.cnt;
         float acc = pending ? x[t.row] : 0.f;
         uint32_t ci = 0;
-        while (__ballot(pending)) {
-            bool progressed = false;
+        while (__ballot(pending)) {   // unbounded: the plan's tables are checked on the host
             if (pending) {
                 // every entry's ready bit and x value read without branches
                 // (independent LDS reads, one wait each), then the ready
@@ -1347,17 +1384,14 @@ __device__ __forceinline__ void df2_run(const DfDev &D, const float *__restrict_
                     if (e == ci && ((mask >> e) & 1u)) {
                         acc = acc - t.val[e] * xv[e];
                         ++ci;
-                        progressed = true;
                     }
                 if (ci == t.len) {
                     if (D.div) acc = acc / t.dval;
                     x[t.row] = acc;
                     df_set(flags, t.row);
                     pending = false;
-                    progressed = true;
                 }
             }
-            
         }
         return;
     }
@@ -1698,8 +1732,12 @@ constexpr int kKrylovLds = kKrylovD + kMaxKrylov;
 //    modified Gram-Schmidt into v_{j+1}, extends the Givens QR of the
 //    Hessenberg matrix and solves min |beta e_1 - H y|_2; x and r move by
 //    the change of y: x += sum dy_c z_c, r -= sum dy_c w_c (r stays b - A x
-//    up to FP64 rounding, as every w_c is an FP64 product A z_c).  eta is
-//    checked after every solve, so a matrix stops as soon as it meets tol.
+//    up to FP64 rounding, as every w_c is an FP64 product A z_c).  eta of
+//    the updated r is checked after every solve: a matrix stops as soon as
+//    it meets tol with room for the rounding drift between the updated and
+//    the true residual (then the true eta <= tol); near tol, at maxit or on
+//    NaN the true residual b - A x is formed and its eta decides and is
+//    reported.  A restart cycle starts from the updated residual.
 // Every workgroup-wide sum has a fixed order: results do not depend on
 // timing.  xs is the FP32 sweep vector; flags the sweeps' ready bits.
 template <int kMode, bool kT = false>
@@ -1825,11 +1863,23 @@ __device__ __forceinline__ void wg_refine(const MatDesc *Md, const MatIO &io, do
                 const double rn = wg_max(rmax, sh);
                 const double xn = wg_max(xmax, sh);
                 eta = rn / (anorm * xn + bnorm);
-                if (!(eta > tol) || it >= maxit) {   // also stops on NaN
+                // The updated r drifts from b - A x by rounding only: each
+                // step adds at most ~(2 k + m + 3) u (|A||x| + |b|) per entry
+                // (k: most entries in a row of A), so the true eta is within
+                // `drift` of this one.  Accept outright when that cannot
+                // cross tol; otherwise (or at maxit, or NaN) decide on the
+                // true residual.
+                const double drift = (double)it * (2.0 * (double)Md->a_kmax + 8.0) * 0x1p-52;
+                if (eta + drift <= tol) {
                     done = true;
                     break;
                 }
-                if (!(hn > 0.0)) break;   // breakdown short of tol: restart from r
+                if (!(eta > tol) || it >= maxit) {
+                    eta = wg_residual<kT>(Md, io, anorm, bnorm, sh);
+                    if (!(eta > tol) || it >= maxit) done = true;
+                    break;   // not done: restart from the true residual
+                }
+                if (!(hn > 0.0)) break;   // breakdown short of tol: restart
             }
         }
     }

@@ -181,14 +181,16 @@ extern "C" glu_status glu_analyze_many(int count, const uint32_t *n, const uint3
             if (status[i] != GLU_OK) msg[i] = glu_last_error();
         }
     };
+    std::vector<std::thread> pool;
     try {
-        std::vector<std::thread> pool;
+        pool.reserve(nt - 1);
         for (int t = 1; t < nt; ++t) pool.emplace_back(work);
-        work();
-        for (std::thread &t : pool) t.join();
-    } catch (const std::system_error &) {
-        work();   // no threads available: finish on this one
+    } catch (const std::exception &) {
+        // fewer threads than asked for (none left, or no memory): the ones
+        // that started share the queue with this thread
     }
+    work();
+    for (std::thread &t : pool) t.join();
     for (int i = 0; i < count; ++i)
         if (status[i] != GLU_OK) return fail(status[i], "matrix " + std::to_string(i) + ": " + msg[i]);
     return GLU_OK;
@@ -298,6 +300,7 @@ struct glu_plan {
     int fuse_units = kDefaultFuseUnits;
     bool has_scaling = false, has_refine = false;
     uint32_t a_nlong = 0, a_nlong_t = 0;   // long rows / columns of A
+    uint32_t a_kmax = 0;                   // most entries in a row or a column of A
 
     std::vector<Segment> fsegs;
     TriHost tri[4];          // lo, up, tup, tlo
@@ -312,6 +315,7 @@ struct glu_plan {
     // only, so small orders get one whatever the cap, and n=500k MNA gains
     // up to ~16 (refactor 14.3 -> 12.4 ms; no change beyond)
     int max_windows = 64;
+    uint32_t win_values = 0;     // values one window may stage (0: what LDS holds)
     int krylov = kDefaultKrylov;  // batch refinement: FGMRES restart length (0: plain)
     int nwin = 0;                // LDS windows of the factorization
     int win_steps = 0;           // steps they cover
@@ -1074,6 +1078,72 @@ void build_df(const TriHost &H, Entries entries, bool div, Pivot pivot, bool lon
     }
 }
 
+// Host check of a dataflow sweep table, before any launch.  Every load the
+// sweep kernels issue stays inside the tables (a lane task's prefetch of
+// kDfK slots per lane and its chunk refills, a wave task's slots, rows and
+// pivots); every slot's value position is in the pattern or padding; every
+// row runs exactly once; and every dependency of a row is finished by an
+// earlier task or, in a lane task, by an earlier lane of the same task
+// (the rows of a wave task never depend on each other).  Tasks are taken in
+// order, so the lowest unfinished task always has its dependencies finished
+// or inside itself: every ready-bit wait of the sweeps ends.
+bool check_df(const DfHost &D, uint32_t n, uint32_t nnz, const char *name, std::string &err)
+{
+    const size_t nslots = D.idx.size();
+    std::vector<uint8_t> done(n, 0);
+    auto bad = [&](const std::string &what) {
+        err = std::string(name) + " sweep table: " + what;
+        return false;
+    };
+    if (D.rl.size() < (size_t)n + 64) return bad("row list not padded");
+    for (size_t s = 0; s < nslots; ++s)
+        if (D.vpos[s] != UINT32_MAX && D.vpos[s] >= nnz) return bad("value position out of range");
+    for (size_t k = 0; k < D.task.size(); ++k) {
+        const uint4 d = D.task[k];
+        const uint32_t base = d.x, p0 = d.y, z = d.z, m = d.w;
+        if ((size_t)p0 + 64 > D.rl.size()) return bad("task rows out of range");
+        if (!(z & kLongTask)) {
+            const uint32_t cnt = z;
+            if (cnt > 64 || (size_t)base + (size_t)kDfK * 64 > nslots) return bad("lane task out of range");
+            for (uint32_t lane = 0; lane < cnt; ++lane) {
+                const uint32_t row = D.rl[p0 + lane].x, len = D.rl[p0 + lane].y;
+                if (row >= n || len > m) return bad("lane row out of range");
+                // a long lane row loads its entries kDfK at a time: up to the
+                // end of its last chunk
+                const size_t chunks = std::max<size_t>(1, (len + kDfK - 1) / kDfK);
+                if (base + (chunks * kDfK - 1) * 64 + lane >= nslots) return bad("lane row slots out of range");
+                for (uint32_t e = 0; e < len; ++e) {
+                    const uint32_t dep = D.idx[base + (size_t)e * 64 + lane];
+                    if (dep >= n || !done[dep]) return bad("row runs before its dependency");
+                }
+                if (done[row]) return bad("row runs twice");
+                done[row] = 1;
+            }
+        } else {
+            const uint32_t lg = (z >> 24) & 0x7fu, cnt = z & 0xffffffu, gw = 1u << lg;
+            if (lg < 4 || lg > 6 || cnt == 0 || cnt * gw > 64) return bad("wave task shape");
+            if ((size_t)base + 64 > nslots) return bad("wave task slots out of range");
+            for (uint32_t g = 0; g < cnt; ++g) {
+                const uint32_t row = D.rl[p0 + g].x, len = D.rl[p0 + g].y;
+                if (row >= n || (gw < 64 && len > gw)) return bad("wave row out of range");
+                if (gw == 64 && (cnt != 1 || (size_t)base + len > nslots)) return bad("wave row slots out of range");
+                for (uint32_t j = 0; j < len; ++j) {
+                    const uint32_t dep = D.idx[base + (size_t)gw * g + j];
+                    if (dep >= n || !done[dep]) return bad("row runs before its dependency");
+                }
+            }
+            for (uint32_t g = 0; g < cnt; ++g) {
+                const uint32_t row = D.rl[p0 + g].x;
+                if (done[row]) return bad("row runs twice");
+                done[row] = 1;
+            }
+        }
+    }
+    for (uint32_t r = 0; r < n; ++r)
+        if (!done[r]) return bad("a row never runs");
+    return true;
+}
+
 struct Uploader {
     Arena arena;
     struct Item {
@@ -1123,6 +1193,8 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
         if (opts->krylov < -1 || opts->krylov > kMaxKrylov)
             return fail(GLU_ERR_ARGUMENT, "krylov must be in [-1, 4]");
         if (opts->krylov) p->krylov = opts->krylov < 0 ? 0 : opts->krylov;
+        if (opts->lds_window_values < 0) return fail(GLU_ERR_ARGUMENT, "lds_window_values must be >= 0");
+        p->win_values = (uint32_t)opts->lds_window_values;
     }
     p->n = P.n;
     p->nnz = P.nnz;
@@ -1139,8 +1211,9 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
     HIP_TRY(hipGetDevice(&p->device));
     p->lds_cap = launch::enable_lds();
     std::vector<Window> wins;
-    choose_windows(T, P.num_lev, P.nnz, (uint32_t)(p->lds_cap / sizeof(float)), (uint32_t)p->fuse_units,
-                   p->max_windows, wins);
+    uint32_t budget = (uint32_t)(p->lds_cap / sizeof(float));
+    if (p->win_values) budget = std::min(budget, p->win_values);
+    choose_windows(T, P.num_lev, P.nnz, budget, (uint32_t)p->fuse_units, p->max_windows, wins);
     p->nwin = (int)wins.size();
     for (const Window &w : wins) {
         p->win_steps += w.l1 - w.l0;
@@ -1189,6 +1262,9 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
         for (uint32_t q = P.sym_c_ptr[r + 1]; q > P.l_col_ptr[r] + 1; --q)
             out.push_back(make_uint2(P.sym_r_idx[q - 1], q - 1));
     }, false, [&](uint32_t) { return 0u; }, long_lanes, dtl);
+    if (!check_df(dlo, n, P.nnz, "L", err) || !check_df(dup, n, P.nnz, "U", err) ||
+        !check_df(dtu, n, P.nnz, "U'", err) || !check_df(dtl, n, P.nnz, "L'", err))
+        return fail(GLU_ERR_INVALID, err);
     // the copy runs in CSC order (contiguous reads, scattered stores): the
     // slot of each position, plus pairs for positions with a second slot;
     // padding slots are zeroed once when the copies are allocated
@@ -1281,6 +1357,8 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
         };
         long_first(a_rptr.data(), a_rows, p->a_nlong);
         long_first(an->in_ap.data(), a_cols, p->a_nlong_t);
+        for (uint32_t i = 0; i < n; ++i)
+            p->a_kmax = std::max({p->a_kmax, a_rptr[i + 1] - a_rptr[i], an->in_ap[i + 1] - an->in_ap[i]});
         p->has_scaling = p->has_refine = true;
     }
 
@@ -1509,6 +1587,7 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
     md.n = n;
     md.nnz = P.nnz;
     md.krylov = (uint32_t)p->krylov;
+    md.a_kmax = p->a_kmax;
     p->d_desc = reinterpret_cast<MatDesc *>(A + o_desc);
     HIP_TRY(hipMemcpy(p->d_desc, &md, sizeof(MatDesc), hipMemcpyHostToDevice));
     p->d_norm_bits = reinterpret_cast<unsigned int *>(A + o_normbits);
@@ -1639,7 +1718,7 @@ extern "C" glu_status glu_plan_create_from_analysis(const glu_analysis *a, const
 
 extern "C" void glu_plan_destroy(glu_plan *p) { delete p; }
 
-extern "C" glu_status glu_plan_info(const glu_plan *p, int64_t info[10])
+extern "C" glu_status glu_plan_info(const glu_plan *p, int64_t info[12])
 {
     if (!p || !info) return fail(GLU_ERR_ARGUMENT, "null argument");
     info[0] = p->n;
@@ -1650,8 +1729,10 @@ extern "C" glu_status glu_plan_info(const glu_plan *p, int64_t info[10])
     info[5] = (int64_t)p->fsegs.size();
     info[6] = (int64_t)p->arena_bytes;
     info[7] = p->units;
-    info[8] = p->win_steps;
-    info[9] = p->win_nslot;
+    info[8] = p->nwin;
+    info[9] = p->win_steps;
+    info[10] = p->win_nslot;
+    info[11] = 0;
     return GLU_OK;
 }
 
@@ -1986,6 +2067,7 @@ extern "C" glu_status glu_group_create(glu_plan *const *plans, int count, glu_gr
         descs[i].n = p->n;
         descs[i].nnz = p->nnz;
         descs[i].krylov = (uint32_t)p->krylov;
+        descs[i].a_kmax = p->a_kmax;
         g->koff.push_back(p->krylov ? ktot : ~(size_t)0);
         ktot += (3 * (size_t)p->krylov + 1) * p->n * (p->krylov != 0);
         g->voff.push_back(vtot);

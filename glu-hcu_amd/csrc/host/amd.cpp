@@ -8,6 +8,10 @@
 // :832-899, _I_NicsLU_PostTree :901-931).  Degree-list tie-breaks, the
 // garbage-collection trigger (pfree >= iwlen) and the 32-bit wrap of the
 // supervariable hash all influence the ordering, so they are kept as is.
+//
+// A derivative work of NICSLU, Copyright (c) 2011-2013 Tsinghua University,
+// distributed under the GNU Lesser General Public License, version 2.1 or
+// later (LICENSES/LGPL-2.1.txt and LICENSES/NOTICE in this directory).
 #include "glu_host.h"
 
 #include <climits>

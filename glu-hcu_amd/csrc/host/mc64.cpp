@@ -11,6 +11,10 @@
 // the shared use of the q array, and every comparison (< versus <=) are kept
 // exactly as the reference has them.  Orientation is NICSLU's: ap indexes
 // "rows" (caller columns), ai holds "columns" (caller rows).
+//
+// A derivative work of NICSLU, Copyright (c) 2011-2013 Tsinghua University,
+// distributed under the GNU Lesser General Public License, version 2.1 or
+// later (LICENSES/LGPL-2.1.txt and LICENSES/NOTICE in this directory).
 #include "glu_host.h"
 
 #include <cfloat>

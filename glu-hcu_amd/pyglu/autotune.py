@@ -14,8 +14,10 @@ None of these changes a result: factors are bit-identical under every
 schedule (only the order of independent work changes).
 
 Each candidate plan is timed on the device with HIP events over the real
-workload (single-matrix factorization, or one batched launch) and the
-fastest is returned.  Results of every candidate are kept for reporting.
+workload (single-matrix factorization, one same-pattern batched launch, or
+one fused launch over an uneven group of patterns, whose workgroup size is
+the tile every member runs on) and the fastest is returned.  Results of
+every candidate are kept for reporting.
 """
 
 from __future__ import annotations
@@ -125,14 +127,54 @@ def tune_batch(analysis, ax, b, tol=1e-10, maxit=10, threads=THREAD_CANDIDATES, 
     return TuneResult(best[0], best[1], best[2], table, krylov=best[3])
 
 
+def tune_group(analyses, axs, bs, tol=1e-10, maxit=10, threads=THREAD_CANDIDATES, krylov=KRYLOV_CANDIDATES,
+               windows=WINDOW_CANDIDATES, reps=3, plan_pool=None):
+    """Best group-wide (block_threads, krylov, lds_windows) for one fused
+    launch over distinct patterns (glu_group_solve): the launch's workgroup
+    size is the tile every member runs on, and each candidate rebuilds all
+    member plans.  Thread counts at the defaults first, then restart
+    lengths, then window counts, each at the best so far.  ``plan_pool`` (an
+    executor) builds the member plans in parallel."""
+    from .device import Group
+    table = []
+    best = None
+
+    def measure(t, m, wnd):
+        mk = lambda an: Plan(an, block_threads=t, krylov=m, lds_windows=wnd)
+        plans = list(plan_pool.map(mk, analyses)) if plan_pool is not None else [mk(a) for a in analyses]
+        g = Group(plans)
+        ms = _time(lambda: g.solve(axs, bs, tol=tol, maxit=maxit), reps, warmup=1)
+        table.append((0, t, 0, ms, (m, wnd)))
+        g.close()
+        for p in plans:
+            p.close()
+        return ms
+
+    for t in threads:
+        ms = measure(t, 0, 0)
+        if best is None or ms < best[2]:
+            best = (0, t, ms, 0, 0)
+    for m in krylov:
+        ms = measure(best[1], m, 0)
+        if ms < best[2]:
+            best = (0, best[1], ms, m, 0)
+    for wnd in windows:
+        if wnd == 0:
+            continue
+        ms = measure(best[1], best[3], wnd)
+        if ms < best[2]:
+            best = (0, best[1], ms, best[3], wnd)
+    return TuneResult(best[0], best[1], best[2], table, lds_windows=best[4], krylov=best[3])
+
+
 def report(res: TuneResult) -> str:
     lines = [f"{'fuse_units':>10} {'threads':>8} {'segments':>9} {'ms':>10} {'extra':>6}"]
     best = min(res.table, key=lambda r: r[3]) if res.table else None
     for row in res.table:
         f, t, s, ms, x = row
         mark = " *" if row is best else ""
-        lines.append(f"{f:>10} {t:>8} {s:>9} {ms:>10.4f} {x:>6}{mark}")
+        lines.append(f"{f:>10} {t:>8} {s:>9} {ms:>10.4f} {str(x):>6}{mark}")
     return "\n".join(lines)
 
 
-__all__ = ["tune_factor", "tune_batch", "report", "TuneResult", "np"]
+__all__ = ["tune_factor", "tune_batch", "tune_group", "report", "TuneResult", "np"]

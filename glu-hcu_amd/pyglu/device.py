@@ -50,19 +50,24 @@ class Plan:
     factor and solve launch sequences from captured hipGraphs (captured
     again when called with other buffers).  ``lds_windows``: how many runs of
     narrow levels, counted from the last, are factored with their values
-    staged in LDS (0 = default, 1; -1 = none).  ``solve_dataflow``: narrow
+    staged in LDS (0 = default, 64; -1 = none).  ``lds_window_values``: the
+    most values one window may stage (0 = what the workgroup's LDS holds;
+    smaller values give more, shorter windows).  ``solve_dataflow``: narrow
     levels of the single-matrix solves barrier-free (0 = default, 1 = on,
     -1 = off).  ``krylov``: refinement of the batched and grouped solves
     after the first solve (0 = default, restarted FGMRES(2); -1 = plain
     refinement; 1..4 = FGMRES(m)).
+
+    After creation, ``lds_window_count``, ``lds_window_steps`` and
+    ``lds_window_max_values`` describe the factorization's LDS windows.
     """
 
     def __init__(self, source, fuse_units: int = 0, block_threads: int = 0, use_graph: bool = False,
-                 lds_windows: int = 0, solve_dataflow: int = 0, krylov: int = 0):
+                 lds_windows: int = 0, solve_dataflow: int = 0, krylov: int = 0, lds_window_values: int = 0):
         _require_device()
         L = capi.lib()
         opts = capi.PlanOpts(fuse_units, block_threads, int(bool(use_graph)), int(lds_windows),
-                             int(solve_dataflow), int(krylov))
+                             int(solve_dataflow), int(krylov), int(lds_window_values))
         h = C.c_void_p()
         if isinstance(source, Analysis):
             capi.check(L.glu_plan_create_from_analysis(source.handle, C.byref(opts), C.byref(h)),
@@ -77,10 +82,11 @@ class Plan:
             capi.check(L.glu_plan_create(C.byref(pat), C.byref(opts), C.byref(h)), "glu_plan_create")
             self.analysis = None
         self._h = h
-        info = np.zeros(10, dtype=np.int64)
+        info = np.zeros(12, dtype=np.int64)
         capi.check(L.glu_plan_info(h, capi.ptr(info)), "glu_plan_info")
         self.n, self.nnz_lu, self.nnz_a, self.num_lev, self.updates, self.segments, \
-            self.device_bytes, self.units, self.tail_level, self.tail_values = (int(v) for v in info)
+            self.device_bytes, self.units, self.lds_window_count, self.lds_window_steps, \
+            self.lds_window_max_values, _ = (int(v) for v in info)
 
     @property
     def handle(self):

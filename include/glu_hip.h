@@ -133,7 +133,11 @@ typedef struct glu_plan_opts {
                                (x += LU^-1 r per step), m in 1..4 = restarted
                                flexible GMRES(m) in FP64 preconditioned by the
                                FP32 factors (m solves per restart cycle) */
-    int32_t reserved[2];
+    int32_t lds_window_values; /* values one LDS window may stage: 0 = as many
+                               as the workgroup's LDS holds (gfx950: ~39.8k),
+                               k > 0 = at most k (smaller windows, more of
+                               them: a schedule knob and a test hook) */
+    int32_t reserved[1];
 } glu_plan_opts;
 
 glu_status glu_plan_create(const glu_pattern *pat, const glu_plan_opts *opts, glu_plan **out);
@@ -145,9 +149,10 @@ void glu_plan_destroy(glu_plan *p);
 
 /* info[0]=n info[1]=nnz_lu info[2]=nnz_a info[3]=num_lev info[4]=updates
  * info[5]=segments in the factor schedule info[6]=device bytes held
- * info[7]=units info[8]=first step of the LDS-resident tail (num_lev + 1:
- * none) info[9]=values the tail stages in LDS */
-glu_status glu_plan_info(const glu_plan *p, int64_t info[10]);
+ * info[7]=64-lane column tasks info[8]=LDS windows of the factorization
+ * info[9]=steps inside them info[10]=values the largest window stages in
+ * LDS info[11]=reserved (0) */
+glu_status glu_plan_info(const glu_plan *p, int64_t info[12]);
 
 /* ------------------------------------------------------------------------
  * Numeric factorization (FP32, in place, L unit-lower strictly below the

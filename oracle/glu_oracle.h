@@ -29,6 +29,12 @@ typedef struct oracle_pattern {
  * Returns the GESP value used (0 when perturb == 0). */
 float oracle_factor_f32(const oracle_pattern *P, float *val, int perturb);
 
+/* The same factorization visiting only the rows of L(:,k) in each updated
+ * column (the dense scatter's other updates add a signed zero).  Equal to
+ * oracle_factor_f32 bit for bit unless an entry holds -0.0 or a factor is
+ * non-finite; its cost is the scalar update count (for large orders). */
+float oracle_factor_f32_sparse(const oracle_pattern *P, float *val, int perturb);
+
 /* Number of level-local updates each entry receives from more than one
  * column of the same level (the entries whose FP32 summation order the
  * reference leaves to its atomics).  mark[p] = 1 for such entries. */
@@ -61,6 +67,18 @@ void oracle_solve_f64(const oracle_pattern *P, const double *lu, const uint32_t 
 void oracle_abft_cca(const oracle_pattern *P, const float *val, float *cca);
 int32_t oracle_abft_check(const oracle_pattern *P, const float *lu, const float *cca, float tol, float *abft,
                           float *max_diff);
+
+/* Values of a refactorization with the frozen analysis: NICSLU's
+ * _I_NicsLU_MC64ScaleForRefact (nicslu_mc64scale.c:58-85) on the caller's
+ * CSC values ax0 (nnz_a entries, ap/ai the caller's CSC), my_DumpA
+ * (preprocess.c:63-78) and predictLU (symbolic.cc:82-103).  order 0 is
+ * NICSLU's refactor product (ax0 * rs) * csp, order 1 the analysis product
+ * ax0 * (rs * csp).  dump_ax (nnz_a, may be NULL) receives the dumped FP64
+ * values, val (nnz_lu) the FP32 values in the LU pattern. */
+void oracle_refresh(const oracle_pattern *P, uint32_t nnz_a, const uint32_t *ap, const uint32_t *ai,
+                    const double *ax0, const uint32_t *row_perm, const uint32_t *col_perm_inv,
+                    const double *row_scale, const double *col_scale_perm, int order, double *dump_ax,
+                    float *val);
 
 #ifdef __cplusplus
 }

@@ -10,6 +10,9 @@ Pinning status of each function (see tests/test_oracle.py):
                 matrix (n <= 300), FP64 oracle agreement at FP32 accuracy,
                 the ABFT column-sum identity of src/symbolic.cc:172-197, the
                 GESP quirks read from src/numeric.cu:133-146.
+  factor_f32_sparse
+                pinned: bit-exact against factor_f32 (+-GESP) on every
+                analysed test matrix and on random patterns.
   solve_f32     pinned: bit-exact against the reference's own
                 Symbolic_Matrix::solve compiled into oracle/_ref.
   factor_f64    pinned: dense Doolittle, scipy splu (natural order, no
@@ -21,6 +24,11 @@ Pinning status of each function (see tests/test_oracle.py):
                 (src/symbolic.cc:163-197, oracle/_ref built with GLU_DEBUG),
                 on passing factors and on planted errors; closed forms
                 (diagonal and unit-triangular matrices).
+  refresh       pinned: bit-exact (FP64 dump and FP32 values) against the
+                reference's own _I_NicsLU_MC64ScaleForRefact + my_DumpA +
+                predictLU on drifted value sets; order 1 (the analysis
+                product) must differ from it somewhere, so the check tells
+                the two orders apart.
 """
 
 from __future__ import annotations
@@ -57,6 +65,8 @@ def _load():
         P = C.c_void_p
         L.oracle_factor_f32.argtypes = [C.POINTER(Pattern), P, C.c_int]
         L.oracle_factor_f32.restype = C.c_float
+        L.oracle_factor_f32_sparse.argtypes = [C.POINTER(Pattern), P, C.c_int]
+        L.oracle_factor_f32_sparse.restype = C.c_float
         L.oracle_conflicts.argtypes = [C.POINTER(Pattern), P]
         L.oracle_conflicts.restype = C.c_uint32
         L.oracle_solve_f32.argtypes = [C.POINTER(Pattern), P, P, P, P, P, P, P]
@@ -65,6 +75,7 @@ def _load():
         L.oracle_abft_cca.argtypes = [C.POINTER(Pattern), P, P]
         L.oracle_abft_check.argtypes = [C.POINTER(Pattern), P, P, C.c_float, P, P]
         L.oracle_abft_check.restype = C.c_int32
+        L.oracle_refresh.argtypes = [C.POINTER(Pattern), C.c_uint32] + [P] * 7 + [C.c_int, P, P]
         _lib = L
     return _lib
 
@@ -103,6 +114,15 @@ class SymPattern:
 def factor_f32(pat: SymPattern, val, perturb=False):
     v = np.array(val, dtype=np.float32, copy=True)
     pert = _load().oracle_factor_f32(pat.c, _p(v), int(bool(perturb)))
+    return v, float(pert)
+
+
+def factor_f32_sparse(pat: SymPattern, val, perturb=False):
+    """factor_f32 without the dense scatter's signed-zero updates: the same
+    bits unless an entry holds -0.0 or a factor is non-finite, at the cost
+    of the scalar update count (used for the largest orders)."""
+    v = np.array(val, dtype=np.float32, copy=True)
+    pert = _load().oracle_factor_f32_sparse(pat.c, _p(v), int(bool(perturb)))
     return v, float(pert)
 
 
@@ -172,6 +192,25 @@ def pattern_to_dense(pat: SymPattern, val):
         s, e = pat.sym_c_ptr[c], pat.sym_c_ptr[c + 1]
         D[pat.sym_r_idx[s:e], c] = val[s:e]
     return D
+
+
+def refresh(pat: SymPattern, an, indices, indptr, ax0, order=0):
+    """Refactor values of ax0 (caller CSC order) with the frozen analysis
+    `an` (row_perm, col_perm_inv, row_scale, col_scale_perm): NICSLU's
+    refactor scaling, my_DumpA, predictLU.  order 0 = (a * rs) * csp (NICSLU's
+    refactor product), 1 = a * (rs * csp) (the analysis product).  Returns
+    (dumped FP64 values, FP32 LU-pattern values)."""
+    ap = np.ascontiguousarray(indptr, dtype=np.uint32)
+    ai = np.ascontiguousarray(indices, dtype=np.uint32)
+    ax0 = np.ascontiguousarray(ax0, dtype=np.float64)
+    arrs = [np.ascontiguousarray(an.row_perm, dtype=np.uint32), np.ascontiguousarray(an.col_perm_inv, dtype=np.uint32),
+            np.ascontiguousarray(an.row_scale, dtype=np.float64),
+            np.ascontiguousarray(an.col_scale_perm, dtype=np.float64)]
+    dump = np.empty(ax0.size)
+    val = np.empty(pat.nnz, dtype=np.float32)
+    _load().oracle_refresh(pat.c, ax0.size, _p(ap), _p(ai), _p(ax0), *[_p(a) for a in arrs], int(order),
+                           _p(dump), _p(val))
+    return dump, val
 
 
 def abft_cca(pat: SymPattern, val):

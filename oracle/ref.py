@@ -39,6 +39,8 @@ def _load():
         lib.ref_solve.argtypes = [P] * 4
         lib.ref_abft.argtypes = [P] * 4
         lib.ref_abft.restype = C.c_int
+        lib.ref_refact_values.argtypes = [P] * 4
+        lib.ref_refact_values.restype = C.c_int
         lib.ref_nicslu_time.argtypes = [C.c_uint, C.c_uint, P, P, P, P, C.c_int, C.c_int, P, P]
         lib.ref_nicslu_time.restype = C.c_int
         _lib = lib
@@ -92,6 +94,20 @@ class RefAnalysis:
         cca = np.empty(self.n, dtype=np.float32)
         first = _load().ref_abft(C.c_void_p(self._handle), _p(val0), _p(lu), _p(cca))
         return cca, int(first)
+
+    def refact_values(self, ax0: np.ndarray):
+        """The reference's values for a refactorization with new values ax0
+        (caller CSC order): NICSLU's _I_NicsLU_MC64ScaleForRefact +
+        _I_NicsLU_Scale, my_DumpA, then predictLU.  Returns (dumped FP64
+        values, FP32 values in the LU pattern)."""
+        ax0 = np.ascontiguousarray(ax0, dtype=np.float64)
+        assert ax0.size == self.nnz_a
+        dump = np.empty(self.nnz_a)
+        val = np.empty(self.nnz_lu, dtype=np.float32)
+        rc = _load().ref_refact_values(C.c_void_p(self._handle), _p(ax0), _p(dump), _p(val))
+        if rc != 0:
+            raise RefError(rc)
+        return dump, val
 
     def close(self):
         if self._handle:

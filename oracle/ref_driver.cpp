@@ -21,6 +21,7 @@
 
 extern "C" {
 #include "nicslu.h"
+#include "nicslu_internal.h"
 #include "preprocess.h"
 }
 #include "symbolic.h"
@@ -169,6 +170,37 @@ int ref_abft(void *hp, const float *val0, const float *lu, float *cca)
     unsigned col = 0;
     if (sscanf(e.c_str(), "Column %u:", &col) == 1) return (int)col;
     return o.find("passed ABFT check") != std::string::npos ? -1 : -2;
+}
+
+// The reference's refactor values for a new value set ax0 (caller CSC
+// order): NICSLU's own _I_NicsLU_MC64ScaleForRefact + _I_NicsLU_Scale, as
+// NicsLU_ReFactorize applies them (nicslu_refact.c:74-77), then my_DumpA
+// and Symbolic_Matrix::predictLU on the frozen pattern.  dump_ax receives
+// the dumped FP64 values (nnz(A)), val the FP32 values (nnz(L+U)).
+// Returns 0 or the error code of the step that failed.
+int ref_refact_values(void *hp, const double *ax0, double *dump_ax, float *val)
+{
+    RefHandle *h = static_cast<RefHandle *>(hp);
+    const unsigned nnz = (unsigned)h->nicslu.nnz;
+    std::vector<double> a0(ax0, ax0 + nnz);
+    _I_NicsLU_MC64ScaleForRefact(&h->nicslu, a0.data());
+    int ret = _I_NicsLU_Scale(&h->nicslu);
+    if (ret != NICS_OK) return ret;
+    double *dax = nullptr;
+    unsigned *dai = nullptr, *dap = nullptr;
+    ret = my_DumpA(&h->nicslu, &dax, &dai, &dap);
+    if (ret != 0) return ret;
+    std::copy(dax, dax + nnz, dump_ax);
+    Symbolic_Matrix *s = h->sym;
+    std::vector<float> saved;
+    saved.swap(s->val);   // predictLU appends to val
+    s->predictLU(dai, dap, dax);
+    std::copy(s->val.begin(), s->val.end(), val);
+    s->val.swap(saved);
+    free(dax);
+    free(dai);
+    free(dap);
+    return 0;
 }
 
 static double now_s()

@@ -136,6 +136,49 @@ int main(int argc, char **argv)
     build_df(up, [&](uint32_t r, std::vector<uint2> &out) {
         for (uint32_t q = rptr[r + 1]; q > dptr[r] + 1; --q) out.push_back(make_uint2(cidx[q - 1], q - 1));
     }, true, [&](uint32_t r) { return dptr[r]; }, long_lanes, dup);
+    // the plan builder's own host check accepts these tables and refuses
+    // planted mistakes: a task run early (swapped with its successor), a row
+    // dropped, a dependency on a row that runs later
+    std::string err;
+    if (!check_df(dlo, n, (uint32_t)nnz, "L", err) || !check_df(dup, n, (uint32_t)nnz, "U", err)) {
+        printf("check_df refused valid tables: %s\n", err.c_str());
+        return 1;
+    }
+    int planted = 0, caught = 0;
+    for (const DfHost *src : {&dlo, &dup}) {
+        for (size_t k = 0; k + 1 < src->task.size(); ++k) {   // first swap that reorders dependent tasks
+            DfHost bad = *src;
+            std::swap(bad.task[k], bad.task[k + 1]);
+            std::string e2;
+            if (!check_df(bad, n, (uint32_t)nnz, "swap", e2)) {
+                ++planted;
+                ++caught;
+                break;
+            }
+        }
+        if (!src->task.empty()) {
+            DfHost bad = *src;
+            bad.task.pop_back();
+            std::string e2;
+            ++planted;
+            caught += !check_df(bad, n, (uint32_t)nnz, "drop", e2);
+        }
+        for (size_t k = 0; k < src->task.size(); ++k) {   // a slot of a row made to read the task's last row
+            const uint4 d = src->task[k];
+            if ((d.z & kLongTask) || d.z < 2) continue;
+            const uint32_t first_len = src->rl[d.y].y;
+            if (first_len == 0) continue;
+            DfHost bad = *src;
+            bad.idx[d.x] = src->rl[d.y + d.z - 1].x;   // lane 0's first entry
+            std::string e2;
+            ++planted;
+            caught += !check_df(bad, n, (uint32_t)nnz, "dep", e2);
+            break;
+        }
+    }
+    printf("check_df: planted %d mistakes, refused %d\n", planted, caught);
+    if (caught != planted) return 1;
+
     std::vector<float> y, x;
     if (emulate("L", dlo, n, lu, false, b, y)) return 1;
     if (emulate("U", dup, n, lu, true, y, x)) return 1;

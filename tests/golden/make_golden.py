@@ -6,7 +6,9 @@ For every input matrix it records the SHA-256 of each array the reference
 produces (permutations, scale factors, the permuted/scaled matrix, the LU
 pattern, the predictLU values and the level sets) and of the reference's
 FP32 solve output for the oracle's FP32 factors and a seeded right-hand
-side.  Run from the repository root:  python tests/golden/make_golden.py
+side, and of the refactor values (NICSLU's _I_NicsLU_MC64ScaleForRefact,
+my_DumpA, predictLU) of a drifted value set.  Run from the repository root:
+python tests/golden/make_golden.py
 """
 
 from __future__ import annotations
@@ -32,6 +34,7 @@ from pyglu import mna  # noqa: E402
 ARRAYS = ["row_perm", "row_perm_inv", "col_perm", "col_perm_inv", "row_scale", "col_scale_perm",
           "dump_ax", "dump_ai", "dump_ap", "sym_c_ptr", "sym_r_idx", "l_col_ptr", "csr_r_ptr",
           "csr_c_idx", "csr_diag_ptr", "val", "level_ptr", "level_idx"]
+REFACT_SEED = 9000   # drift of the refactor value set: data * (1 + U[-0.1, 0.1])
 
 
 def sha(a: np.ndarray) -> str:
@@ -70,6 +73,11 @@ def main():
         rhs = mna.rhs(n, 77).astype(np.float32)
         rec["oracle_lu32"] = sha(lu32)
         rec["ref_solve_f32"] = sha(r.solve(lu32, rhs))
+        # refactor values of a drifted value set (NICSLU's refactor scaling,
+        # my_DumpA, predictLU)
+        dump, val = r.refact_values(mna.perturb_values(d, REFACT_SEED))
+        rec["refact_dump_ax"] = sha(dump)
+        rec["refact_val"] = sha(val)
         out[name] = rec
         print(name, n, r.nnz_lu, r.num_lev)
     with open(os.path.join(HERE, "analysis_sha256.json"), "w") as f:

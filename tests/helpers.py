@@ -37,6 +37,86 @@ def mna_case(name):
     return mna.mna_csc(spec, seed)
 
 
+def golden_inputs():
+    """The inputs of tests/golden/make_golden.py, same order: add32, two MNA
+    matrices and eight random patterns (half with equal magnitudes, so that
+    MC64/AMD tie-breaks decide everything)."""
+    yield "add32", add32()
+    yield "mna_small", mna.mna_csc(mna.SPEC_SMALL, mna.SEED_SMALL)
+    yield "mna_2k", mna.mna_csc(mna.MNASpec(n_nodes=1980, n_vsrc=20, n_rails=3, rail_deg=120), 7)
+    rng = np.random.default_rng(2024)
+    for t in range(8):
+        n = int(rng.integers(20, 300))
+        A = sp.random(n, n, density=float(rng.uniform(0.01, 0.06)), random_state=int(rng.integers(1 << 30)),
+                      format="csc")
+        if t % 2 == 0:
+            A.data[:] = 1.0
+        A = A + sp.csc_matrix((np.ones(n), (rng.permutation(n), np.arange(n))), shape=(n, n))
+        A = sp.csc_matrix(A)
+        A.sum_duplicates()
+        A.sort_indices()
+        yield f"random{t}", (A.data.copy(), A.indices.astype(np.int32), A.indptr.astype(np.int32), n)
+
+
+def golden_digests():
+    """tests/golden/analysis_sha256.json: per input, SHA-256 digests of the
+    reference's analysis arrays, solve output and refactor values."""
+    import json
+    with open(os.path.join(GOLDEN, "analysis_sha256.json")) as f:
+        return json.load(f)
+
+
+REFACT_SEED = 9000   # drifted value set of the golden refactor values (make_golden.py)
+
+
+def order_sensitive_values(an, indices, indptr, data, span=32):
+    """A value set near `data` on which the two FP64 product orders of the
+    MC64 scaling, (a * rs) * csp (NICSLU's refactor) and a * (rs * csp) (its
+    analysis), round to different FP32 values wherever that can happen.
+
+    Entry k of the caller's CSC (column j, row i) is scaled by rs[j] and
+    csp[col_perm_inv[i]].  Each entry is moved to a value whose scaled
+    product sits at the midpoint of two adjacent FP32 numbers, then stepped
+    one FP64 ulp at a time (up to `span` steps each way) until the two orders
+    land on opposite sides of it.  Returns (values, mask of the entries where
+    the orders differ in FP32).  Only entries whose scales are both not
+    powers of two can differ at all."""
+    n = len(indptr) - 1
+    col = np.repeat(np.arange(n), np.diff(np.asarray(indptr, dtype=np.int64)))
+    s = np.asarray(an.row_scale)[col]
+    c = np.asarray(an.col_scale_perm)[np.asarray(an.col_perm_inv)[indices]]
+    f = (data * s * c).astype(np.float32)
+    mid = (f.astype(np.float64) + np.nextafter(f, np.float32(np.inf)).astype(np.float64)) / 2
+    a = mid / (s * c)
+    out = np.array(data, dtype=np.float64, copy=True)
+    found = np.zeros(out.size, dtype=bool)
+    lo, hi = a.copy(), a.copy()
+    with np.errstate(all="ignore"):
+        for _ in range(span):
+            for cand in (lo, hi):
+                o0 = ((cand * s) * c).astype(np.float32)
+                o1 = (cand * (s * c)).astype(np.float32)
+                m = (o0 != o1) & ~found & np.isfinite(cand) & (data != 0)
+                out[m] = cand[m]
+                found |= m
+            lo = np.nextafter(lo, -np.inf)
+            hi = np.nextafter(hi, np.inf)
+    return out, found
+
+
+def unsym_case(n=400, seed=31):
+    """A random unsymmetric matrix with magnitudes spread over six decades:
+    MC64's row and column scales are rarely powers of two on it."""
+    rng = np.random.default_rng(seed)
+    A = sp.random(n, n, density=0.02, random_state=int(rng.integers(1 << 30)), format="csc")
+    A.data[:] = 10.0 ** rng.uniform(-3, 3, A.data.size) * rng.choice([-1.0, 1.0], A.data.size)
+    A = A + sp.csc_matrix((10.0 ** rng.uniform(-1, 2, n), (rng.permutation(n), np.arange(n))), shape=(n, n))
+    A = sp.csc_matrix(A)
+    A.sum_duplicates()
+    A.sort_indices()
+    return A.data.copy(), A.indices.astype(np.int32), A.indptr.astype(np.int32), n
+
+
 def entry_rel_err(got, want, col_ptr):
     """Per-entry relative error |got - want| / max(|want|, 1e-3 * |col|_inf):
     relative to the entry itself, floored at a thousandth of its column's

@@ -145,14 +145,38 @@ def test_batch_matches_single(torch_mod):
         assert float(eta[b]) <= 1e-10 and e <= 2e-10
 
 
-def test_refresh_matches_analysis(torch_mod):
-    """Refreshing the analysed values reproduces predictLU's FP32 values up to
-    the product order of the two MC64 scaling passes (<= 1 ulp)."""
+@pytest.mark.parametrize("name", ["small", "batch", "unsym"])
+def test_refresh_matches_oracle(torch_mod, name):
+    """glu_refresh and the batch kernels' value phase equal the oracle's
+    restatement of NICSLU's refactor scaling (_I_NicsLU_MC64ScaleForRefact,
+    (a * rs) * csp) + my_DumpA + predictLU bit for bit: on drifted value sets
+    and on a value set where the analysis order a * (rs * csp) rounds to
+    other FP32 values (the oracle with that order must then differ).  The
+    batch kernels' phase 1 is checked through their factors, which must be
+    the oracle's factorization of the oracle's refreshed values."""
+    from helpers import order_sensitive_values, unsym_case
+    from oracle import oracle as O
+    from pyglu import mna
+    from pyglu.analysis import Analysis
+    from pyglu.device import Plan
     torch = torch_mod
-    data, indices, indptr, n, an, plan, pat = _setup("small")
-    lu = plan.refresh(torch.from_numpy(data).cuda()).cpu().numpy().reshape(-1)
-    ulps = np.abs(lu.view(np.int32).astype(np.int64) - an.val.view(np.int32).astype(np.int64))
-    assert ulps.max() <= 1 and (lu == 0).sum() == (an.val == 0).sum()
+    data, indices, indptr, n = unsym_case() if name == "unsym" else mna_case(name)
+    an = Analysis(data, indices, indptr, n)
+    plan, pat = Plan(an), O.SymPattern(an)
+    adv, sensitive = order_sensitive_values(an, indices, indptr, data)
+    sets = np.stack([mna.perturb_values(data, 3000 + b) for b in range(3)] + [adv])
+    print(f"[{name}] n={n} order-sensitive entries {int(sensitive.sum())}", flush=True)
+    assert sensitive.any()
+    axd = torch.from_numpy(sets).cuda()
+    got = plan.refresh(axd).cpu().numpy()
+    lub = plan.batch_factor(axd).cpu().numpy()
+    for b in range(sets.shape[0]):
+        _, want = O.refresh(pat, an, indices, indptr, sets[b])
+        assert np.array_equal(got[b].view(np.uint32), want.view(np.uint32)), f"refresh, set {b}"
+        fw, _ = O.factor_f32(pat, want)
+        assert np.array_equal(lub[b].view(np.uint32), fw.view(np.uint32)), f"batch factor, set {b}"
+    _, wrong = O.refresh(pat, an, indices, indptr, adv, order=1)
+    assert np.count_nonzero(got[-1].view(np.uint32) != wrong.view(np.uint32)) > 0
 
 
 def test_group_matches_single(torch_mod):

@@ -318,3 +318,83 @@ def test_abft_matches_reference(name):
     bad[pat.sym_c_ptr[c1]] += 1.0          # first (U or pivot) entry of column c1
     first_b, _, _ = O.abft_check(pat, bad, cca)
     assert first_b == r.abft(r.val, bad)[1] == c1
+
+
+def test_refresh_golden_digests():
+    """oracle_refresh on a drifted value set of every golden input equals the
+    reference's own refactor values (NICSLU's _I_NicsLU_MC64ScaleForRefact,
+    my_DumpA, predictLU), recorded as digests by tests/golden/make_golden.py:
+    the dumped FP64 values and the FP32 LU-pattern values, bit for bit."""
+    import hashlib
+
+    from helpers import REFACT_SEED, golden_digests, golden_inputs
+    from pyglu import mna
+    gold = golden_digests()
+    sha = lambda a: hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+    for name, (d, i, p, n) in golden_inputs():
+        an = Analysis(d, i, p, n)
+        dump, val = O.refresh(O.SymPattern(an), an, i, p, mna.perturb_values(d, REFACT_SEED))
+        assert sha(dump) == gold[name]["refact_dump_ax"], name
+        assert sha(val) == gold[name]["refact_val"], name
+
+
+@pytest.mark.parametrize("name", ["mna600", "add32", "mna2k", "small", "unsym"])
+def test_refresh_bit_exact_with_reference(name):
+    """Live against the reference build: drifted value sets and a value set
+    on which the two product orders round to different FP32 values.  On the
+    latter, the analysis order a * (rs * csp) (order=1) must differ from the
+    reference in FP32, so the check tells the two orders apart; and with
+    order=1 the unchanged values reproduce the analysis's own dump and
+    predictLU values bit for bit (both are that product)."""
+    from helpers import order_sensitive_values, unsym_case
+    from oracle import ref
+    from pyglu import mna
+    if not ref.available():
+        pytest.skip("oracle/_ref not built")
+    d, i, p, n = unsym_case() if name == "unsym" else mna_case(name)
+    r = ref.analyze(d, i, p, n)
+    pat = O.SymPattern(r)
+    adv, sensitive = order_sensitive_values(r, i, p, d)
+    for ax0 in [mna.perturb_values(d, 7100 + s) for s in range(3)] + [adv]:
+        dump_ref, val_ref = r.refact_values(ax0)
+        dump, val = O.refresh(pat, r, i, p, ax0)
+        assert np.array_equal(dump.view(np.uint64), dump_ref.view(np.uint64))
+        assert np.array_equal(val.view(np.uint32), val_ref.view(np.uint32))
+    if name != "add32":   # add32's column scales are all powers of two: both orders agree
+        _, val1 = O.refresh(pat, r, i, p, adv, order=1)
+        assert sensitive.sum() >= (100 if name == "unsym" else 1)
+        assert np.count_nonzero(val1.view(np.uint32) != val_ref.view(np.uint32)) == sensitive.sum()
+    dump1, val1 = O.refresh(pat, r, i, p, d, order=1)
+    assert np.array_equal(dump1.view(np.uint64), r.dump_ax.view(np.uint64))
+    assert np.array_equal(val1.view(np.uint32), r.val.view(np.uint32))
+
+
+@pytest.mark.parametrize("name", ["mna600", "mna2k", "small", "add32", "batch"])
+def test_factor_sparse_walk_equals_dense_scatter(name):
+    """oracle_factor_f32_sparse (updates of L(:,k)'s rows only) equals the
+    dense-scatter restatement bit for bit, with and without GESP, on the
+    analysed matrices and on random patterns with dense rows and ties."""
+    d, i, p, n = mna_case(name)
+    an = Analysis(d, i, p, n)
+    pat = O.SymPattern(an)
+    for perturb in (False, True):
+        a, pa = O.factor_f32(pat, an.val, perturb=perturb)
+        b, pb = O.factor_f32_sparse(pat, an.val, perturb=perturb)
+        assert pa == pb and np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_factor_sparse_walk_random_patterns():
+    rng = np.random.default_rng(77)
+    for t in range(12):
+        n = int(rng.integers(5, 120))
+        M = rng.uniform(-1, 1, (n, n)) * (rng.random((n, n)) < rng.uniform(0.03, 0.2))
+        if t % 3 == 0:
+            M[int(rng.integers(n)), :] = rng.uniform(-1, 1, n)
+        M += np.diag(rng.uniform(0.5, 2.0, n) * rng.choice([-1, 1], n))
+        P = DensePattern(M != 0)
+        sp_ = O.SymPattern(P)
+        v = P.values(M).astype(np.float32)
+        for perturb in (False, True):
+            a, _ = O.factor_f32(sp_, v, perturb=perturb)
+            b, _ = O.factor_f32_sparse(sp_, v, perturb=perturb)
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), (t, perturb)
