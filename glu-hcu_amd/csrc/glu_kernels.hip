@@ -392,6 +392,15 @@ __global__ void k_zero_u64(unsigned long long *__restrict__ p, int count)
     if ((int)threadIdx.x < count) p[threadIdx.x] = 0ull;
 }
 
+// p[0, count) = 0 across the grid (a kernel, not a memset, so that the
+// clear may sit inside a captured graph).
+__global__ void __launch_bounds__(256) k_fill_zero(uint32_t *__restrict__ p, uint64_t count)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+
 // One level across the grid; the host sizes the grid with the lanes the
 // level needs (column tasks and long merges a wave each).
 __global__ void __launch_bounds__(256) k_factor_grid(FactorDev F, float *__restrict__ val,
@@ -2430,6 +2439,13 @@ void adjoint_values(const RefineDev &R, const double *lam, const double *x, doub
 void zero_u64(unsigned long long *p, int count, hipStream_t st)
 {
     k_zero_u64<<<1, 64, 0, st>>>(p, count);
+}
+
+void fill_zero(uint32_t *p, uint64_t words, hipStream_t st)
+{
+    if (words == 0) return;
+    const unsigned grid = (unsigned)std::min<uint64_t>(blocks_for(words, 256), 2048);
+    k_fill_zero<<<grid, 256, 0, st>>>(p, words);
 }
 
 size_t matio_size() { return sizeof(MatIO); }

@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <atomic>
 #include <system_error>
@@ -64,6 +65,7 @@ size_t enable_lds();
 void factor_window(const FactorDev &F, int w, uint32_t nslot, float *val, const float *d_pert, int perturb,
                    int threads, hipStream_t st);
 void zero_u64(unsigned long long *p, int count, hipStream_t st);
+void fill_zero(uint32_t *p, uint64_t words, hipStream_t st);
 void solve_one(const MatDesc *desc, uint32_t nnz, uint32_t nextra, const float *lu, float *lucsr, const float *b,
                float *x, float *t32, int threads, int mode, size_t lds, hipStream_t st);
 void refine_one(const MatDesc *desc, uint32_t nnz, uint32_t nextra, const double *ax, const double *b, double *x,
@@ -1923,6 +1925,103 @@ extern "C" glu_status glu_refine(glu_plan *p, const double *d_ax, const float *d
         ++it;
         if (!(eta > tol) || it >= maxit) break;
     }
+    if (h_iters) *h_iters = it;
+    if (h_eta) *h_eta = eta;
+    if (!(eta == eta)) return fail(GLU_ERR_NUMERIC, "refinement produced NaN");
+    return GLU_OK;
+}
+
+// ---------------------------------------------------------------------------
+// glu_refine_noalloc: glu_refine's one-workgroup path with its scratch in a
+// file-scope device array instead of plan buffers grown on first use.
+// ---------------------------------------------------------------------------
+
+namespace {
+
+constexpr size_t kScratchBytes = GLU_NOALLOC_SCRATCH_BYTES;
+constexpr int kScratchDevices = 64;
+
+// One copy per device, held until the process ends.  Host code never uses
+// this name as an address: each device's copy is reached through the
+// pointer hipGetSymbolAddress gives for it (scratch_base).
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+std::mutex g_scratch_mu;
+unsigned char *g_scratch_ptr[kScratchDevices] = {};
+
+// The calling thread's current device's copy of `scratch`, looked up once
+// per device (with that device current).
+glu_status scratch_base(unsigned char **out)
+{
+    int dev = -1;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kScratchDevices) return fail(GLU_ERR_DEVICE, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(GLU_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = static_cast<unsigned char *>(ptr);
+    }
+    *out = g_scratch_ptr[dev];
+    return GLU_OK;
+}
+
+size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+}  // namespace
+
+extern "C" glu_status glu_refine_noalloc(glu_plan *p, const double *d_ax, const float *d_lu, const double *d_b,
+                                         double *d_x, double tol, int maxit, int transpose, int *h_iters,
+                                         double *h_eta, void *stream)
+{
+    if (!p || !d_ax || !d_lu || !d_b || !d_x || maxit <= 0) return fail(GLU_ERR_ARGUMENT, "bad argument");
+    if (!p->has_refine) return fail(GLU_ERR_ARGUMENT, "plan has no analysis");
+    const uint32_t n = p->n;
+    const int mode = sweep_mode(n, launch::enable_lds());
+    if (mode == 0)
+        return fail(GLU_ERR_ARGUMENT, "order " + std::to_string(n) +
+                                          " is beyond the one-workgroup refinement (its ready bits exceed LDS): "
+                                          "use glu_refine");
+    // scratch layout: sweep value copy, correction vector, residual, norms,
+    // FGMRES vectors, iteration count + eta
+    const uint32_t nval = transpose ? p->tdf_nval : p->df_nval;
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        const size_t o = off;
+        off = align256(off + bytes);
+        return o;
+    };
+    const size_t o_val = take(sizeof(float) * (size_t)nval);
+    const size_t o_t32 = take(sizeof(float) * (size_t)n);
+    const size_t o_r64 = take(sizeof(double) * (size_t)n);
+    const size_t o_norms = take(2 * sizeof(double));
+    const size_t o_kry = take(p->krylov ? sizeof(double) * (3 * (size_t)p->krylov + 1) * n : 0);
+    const size_t o_out = take(16);
+    if (off > kScratchBytes)
+        return fail(GLU_ERR_ARGUMENT, "refinement needs " + std::to_string(off) + " bytes of scratch; the static " +
+                                          "scratch holds " + std::to_string(kScratchBytes) + ": use glu_refine");
+    unsigned char *base = nullptr;
+    glu_status s = scratch_base(&base);
+    if (s != GLU_OK) return s;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    float *lucsr = reinterpret_cast<float *>(base + o_val);
+    int32_t *d_it = reinterpret_cast<int32_t *>(base + o_out);
+    double *d_eta = reinterpret_cast<double *>(base + o_out + 8);
+    // the copy's padding slots must read 0 (the array keeps the last call's data)
+    launch::fill_zero(reinterpret_cast<uint32_t *>(lucsr), nval, st);
+    launch::refine_one(p->d_desc, p->nnz, transpose ? p->tdf_nextra : p->df_nextra, d_ax, d_b, d_x, d_lu, lucsr,
+                       reinterpret_cast<float *>(base + o_t32), reinterpret_cast<double *>(base + o_r64),
+                       reinterpret_cast<double *>(base + o_norms),
+                       p->krylov ? reinterpret_cast<double *>(base + o_kry) : nullptr, d_it, d_eta, tol, maxit,
+                       p->fused_threads, mode, launch::batch_lds_bytes(mode, n), transpose, st);
+    HIP_TRY(hipGetLastError());
+    int32_t it = 0;
+    double eta = 0.0;
+    HIP_TRY(hipMemcpyAsync(&it, d_it, sizeof it, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(&eta, d_eta, sizeof eta, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
     if (h_iters) *h_iters = it;
     if (h_eta) *h_eta = eta;
     if (!(eta == eta)) return fail(GLU_ERR_NUMERIC, "refinement produced NaN");

@@ -31,7 +31,7 @@ EXPORTED = [
     "glu_plan_create", "glu_plan_create_from_analysis", "glu_plan_destroy", "glu_plan_info",
     "glu_factor", "glu_factor_host", "glu_solve", "glu_refresh", "glu_refine", "glu_adjoint_values",
     "glu_batch_solve", "glu_batch_factor", "glu_abft_sums", "glu_abft_check",
-    "glu_group_create", "glu_group_destroy", "glu_group_solve",
+    "glu_group_create", "glu_group_destroy", "glu_group_solve", "glu_refine_noalloc",
 ]
 
 
@@ -86,6 +86,7 @@ def lib():
         "glu_solve": (st, [P, P, P, P, I, P]),
         "glu_refresh": (st, [P, P, P, I, P]),
         "glu_refine": (st, [P, P, P, P, P, D, I, I, P, P, P]),
+        "glu_refine_noalloc": (st, [P, P, P, P, P, D, I, I, P, P, P]),
         "glu_adjoint_values": (st, [P, P, P, P, P]),
         "glu_batch_solve": (st, [P, I, P, P, P, P, D, I, P, P, C.c_uint, P]),
         "glu_batch_factor": (st, [P, I, P, P, C.c_uint, P]),

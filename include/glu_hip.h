@@ -186,6 +186,23 @@ glu_status glu_refine(glu_plan *p, const double *d_ax, const float *d_lu, const 
                       double *d_x, double tol, int maxit, int transpose, int *h_iters,
                       double *h_eta, void *stream);
 
+/* glu_refine without any allocation: the same refinement and results (the
+ * same kernels), with its scratch (the sweep-layout copy of the factors,
+ * the correction and residual vectors, the FGMRES vectors, the iteration
+ * count and eta) in a static device array of GLU_NOALLOC_SCRATCH_BYTES per
+ * device, which each device that runs it holds until the process ends.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  It calls no hipMalloc/hipFree.  Limits: the
+ * order must fit the one-workgroup refinement (n up to ~1.2M) and its
+ * scratch must fit the array, else GLU_ERR_ARGUMENT (use glu_refine).
+ * Every call on one device shares that device's array: calls that overlap
+ * on one device (other streams or threads) race and must not be issued. */
+#define GLU_NOALLOC_SCRATCH_BYTES (64u << 20)
+glu_status glu_refine_noalloc(glu_plan *p, const double *d_ax, const float *d_lu, const double *d_b,
+                              double *d_x, double tol, int maxit, int transpose, int *h_iters,
+                              double *h_eta, void *stream);
+
 /* Adjoint of x = A^-1 b with respect to the stored values of A:
  * grad_ax[k] = -lam[row(k)] * x[col(k)] for every entry k of the caller's
  * CSC (lam = A^-T g).  All pointers are device buffers. */
