@@ -160,15 +160,13 @@ struct SolveDev {
     TriDev lo, up;             // A x = b:   L then U
     TriDev tup, tlo;           // A' x = b:  U' then L'
     DfDev lo_df, up_df;        // L, U over the per-matrix value copy
-    const uint32_t *df_dst;    // value copy: slot of CSC position p in it (~0u: none) [nnz_lu]
-    const uint2 *df_extra;     // (slot, position) of positions that take a second slot [df_nextra]
-    uint32_t df_nval, df_nextra;   // slots per matrix (padding slots hold 0); extra pairs
+    const uint32_t *df_src;    // value copy: CSC position of each slot (~0u: padding, holds 0) [df_nval]
+    uint32_t df_nval;          // slots per matrix
     // A' x = b over a second value copy (U' slots, U' pivots, L' slots),
     // the same kind of tables
     DfDev tup_df, tlo_df;
-    const uint32_t *tdf_dst;
-    const uint2 *tdf_extra;
-    uint32_t tdf_nval, tdf_nextra;
+    const uint32_t *tdf_src;
+    uint32_t tdf_nval;
     const uint32_t *row_perm, *row_perm_inv, *col_perm, *col_perm_inv;
     const double *row_scale, *col_scale_perm;
 };

@@ -392,15 +392,6 @@ __global__ void k_zero_u64(unsigned long long *__restrict__ p, int count)
     if ((int)threadIdx.x < count) p[threadIdx.x] = 0ull;
 }
 
-// p[0, count) = 0 across the grid (a kernel, not a memset, so that the
-// clear may sit inside a captured graph).
-__global__ void __launch_bounds__(256) k_fill_zero(uint32_t *__restrict__ p, uint64_t count)
-{
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
-         i += (uint64_t)gridDim.x * blockDim.x)
-        p[i] = 0u;
-}
-
 // One level across the grid; the host sizes the grid with the lanes the
 // level needs (column tasks and long merges a wave each).
 __global__ void __launch_bounds__(256) k_factor_grid(FactorDev F, float *__restrict__ val,
@@ -1595,26 +1586,24 @@ __device__ __forceinline__ void wg_factor(const MatDesc *Md, const MatIO &io, in
         factor_all_wg(F, io.lu, g_lds_vec, pert, perturb, wlds);
     }
 
-    // 4. the factors copied into the sweeps' task-contiguous layout, read in
-    //    CSC order (contiguous) and stored to their slots
+    // 4. the factors gathered into the sweeps' task-contiguous layout: slot
+    //    order (contiguous, whole-line stores), each slot reading its CSC
+    //    position; padding slots get 0
     if (kCopy) {
-        const uint32_t *dst = glb(Md->s.df_dst);
+        const uint32_t *src = glb(Md->s.df_src);
+        const uint32_t nval = Md->s.df_nval;
         constexpr int U = 8;
-        for (uint32_t p0 = tid; p0 < nnz; p0 += U * nt) {
-            uint32_t q[U];
+        for (uint32_t q0 = tid; q0 < nval; q0 += U * nt) {
+            uint32_t p[U];
             float v[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const uint32_t p = p0 + u * nt;
-                q[u] = p < nnz ? dst[p] : ~0u;
-                v[u] = p < nnz ? io.lu[p] : 0.f;
-            }
+            for (int u = 0; u < U; ++u) p[u] = q0 + u * nt < nval ? src[q0 + u * nt] : ~0u;
+#pragma unroll
+            for (int u = 0; u < U; ++u) v[u] = p[u] != ~0u ? io.lu[p[u]] : 0.f;
 #pragma unroll
             for (int u = 0; u < U; ++u)
-                if (q[u] != ~0u) io.lucsr[q[u]] = v[u];
+                if (q0 + u * nt < nval) io.lucsr[q0 + u * nt] = v[u];
         }
-        const uint2 *extra = glb(Md->s.df_extra);
-        for (uint32_t e = tid; e < Md->s.df_nextra; e += nt) io.lucsr[extra[e].x] = io.lu[extra[e].y];
         __syncthreads();
     }
     if (tid == 0 && io.norms) io.norms[1] = bnorm;
@@ -1980,22 +1969,16 @@ __global__ void __launch_bounds__(1024) k_batch_same_refine(const MatDesc *__res
     wg_refine<kMode>(desc, io, tol, maxit, sh, hs, xs, flags);
 }
 
-// One matrix's factors into the sweeps' layout (grid): lucsr[df_dst[p]] =
-// lu[p]; padding slots keep the zeros they were allocated with.
+// One matrix's factors gathered into the sweeps' layout (grid, a thread per
+// slot): lucsr[q] = lu[df_src[q]], 0 for padding slots.
 template <bool kT>
 __global__ void __launch_bounds__(256) k_copy_df(const MatDesc *__restrict__ desc, const float *__restrict__ lu,
                                                  float *__restrict__ lucsr)
 {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t *dst = glb(kT ? desc->s.tdf_dst : desc->s.df_dst);
-    if (p < desc->nnz) {
-        const uint32_t q = dst[p];
-        if (q != ~0u) lucsr[q] = lu[p];
-    }
-    if (p < (kT ? desc->s.tdf_nextra : desc->s.df_nextra)) {
-        const uint2 e = glb(kT ? desc->s.tdf_extra : desc->s.df_extra)[p];
-        lucsr[e.x] = lu[e.y];
-    }
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (kT ? desc->s.tdf_nval : desc->s.df_nval)) return;
+    const uint32_t p = glb(kT ? desc->s.tdf_src : desc->s.df_src)[q];
+    lucsr[q] = p != ~0u ? lu[p] : 0.f;
 }
 
 // Refinement of one factored matrix (glu_refine; kT: A' x = b): x = 0,
@@ -2027,6 +2010,34 @@ __global__ void __launch_bounds__(1024) k_refine_one(const MatDesc *__restrict__
     float *xs;
     batch_lds<kMode>(n, io.t32, flags, xs);
     wg_refine<kMode, kT>(desc, io, tol, maxit, sh, hs, xs, flags);
+}
+
+// The preconditioner's two sweeps alone, in one workgroup, over a
+// correction vector t32 in global memory that grid kernels scaled before
+// and read back after (glu_refine for orders whose vector does not fit in
+// LDS: the FP64 vector work and the products with A run across the grid).
+// kT: A' x = b (U' then L').  Dynamic LDS: the ready bits, ceil(n/32) words.
+template <bool kT>
+__global__ void __launch_bounds__(1024) k_sweeps_df(const MatDesc *__restrict__ desc,
+                                                    const float *__restrict__ lucsr, float *__restrict__ t32)
+{
+    uint32_t *flags = reinterpret_cast<uint32_t *>(g_lds_vec);
+    const uint32_t nwords = (desc->n + 31) / 32;
+    for (uint32_t i = threadIdx.x; i < nwords; i += blockDim.x) flags[i] = 0u;
+    __syncthreads();
+    {
+        DfDev lo = kT ? desc->s.tup_df : desc->s.lo_df;
+        globalize(lo);
+        df2_sweep(lo, lucsr + lo.voff, t32, flags);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < nwords; i += blockDim.x) flags[i] = 0u;
+    __syncthreads();
+    {
+        DfDev up = kT ? desc->s.tlo_df : desc->s.up_df;
+        globalize(up);
+        df2_sweep(up, lucsr + up.voff, t32, flags);
+    }
 }
 
 // glu_solve in one workgroup (A x = b): xs = float(b[col_perm] *
@@ -2291,32 +2302,45 @@ void batch_same(const MatDesc *desc, int batch, const double *ax, const double *
                                                                             norms, kry, iters, eta, tol, maxit);
 }
 
-// glu_refine in one workgroup: the factors into the sweeps' layout, then
-// the refinement (mode: kSweepLds or kSweepFlags).
-void refine_one(const MatDesc *desc, uint32_t nnz, uint32_t nextra, const double *ax, const double *b, double *x,
+// glu_refine in one workgroup (orders whose sweep vector fits in LDS): the
+// factors into the sweeps' layout, then the refinement.
+void refine_one(const MatDesc *desc, uint32_t nval, const double *ax, const double *b, double *x,
                 const float *lu, float *lucsr, float *t32, double *r64, double *norms, double *kry, int32_t *iters,
-                double *eta, double tol, int maxit, int threads, int mode, size_t ref_lds, int transpose,
-                hipStream_t st)
+                double *eta, double tol, int maxit, int threads, size_t ref_lds, int transpose, hipStream_t st)
 {
-    const unsigned grid = blocks_for(std::max(nnz, nextra), 256);
-    if (transpose) k_copy_df<true><<<grid, 256, 0, st>>>(desc, lu, lucsr);
-    else k_copy_df<false><<<grid, 256, 0, st>>>(desc, lu, lucsr);
-#define GLU_REFINE_ONE(M, T)                                                                                   \
-    k_refine_one<M, T><<<1, threads, ref_lds, st>>>(desc, ax, b, x, lu, lucsr, t32, r64, norms, kry, iters, eta, \
-                                                    tol, maxit)
-    if (mode == kSweepLds && !transpose) GLU_REFINE_ONE(kSweepLds, false);
-    else if (mode == kSweepLds) GLU_REFINE_ONE(kSweepLds, true);
-    else if (!transpose) GLU_REFINE_ONE(kSweepFlags, false);
-    else GLU_REFINE_ONE(kSweepFlags, true);
-#undef GLU_REFINE_ONE
+    const unsigned grid = blocks_for(nval, 256);
+    if (transpose) {
+        k_copy_df<true><<<grid, 256, 0, st>>>(desc, lu, lucsr);
+        k_refine_one<kSweepLds, true><<<1, threads, ref_lds, st>>>(desc, ax, b, x, lu, lucsr, t32, r64, norms, kry,
+                                                                   iters, eta, tol, maxit);
+    } else {
+        k_copy_df<false><<<grid, 256, 0, st>>>(desc, lu, lucsr);
+        k_refine_one<kSweepLds, false><<<1, threads, ref_lds, st>>>(desc, ax, b, x, lu, lucsr, t32, r64, norms, kry,
+                                                                    iters, eta, tol, maxit);
+    }
+}
+
+// The factors into the sweeps' layout (A or A' tables), grid.
+void copy_df(const MatDesc *desc, uint32_t nval, const float *lu, float *lucsr, int transpose, hipStream_t st)
+{
+    if (transpose) k_copy_df<true><<<blocks_for(nval, 256), 256, 0, st>>>(desc, lu, lucsr);
+    else k_copy_df<false><<<blocks_for(nval, 256), 256, 0, st>>>(desc, lu, lucsr);
+}
+
+// Both sweeps of the preconditioner in one workgroup over t32 (global).
+void sweeps_df(const MatDesc *desc, const float *lucsr, float *t32, int threads, size_t lds, int transpose,
+               hipStream_t st)
+{
+    if (transpose) k_sweeps_df<true><<<1, threads, lds, st>>>(desc, lucsr, t32);
+    else k_sweeps_df<false><<<1, threads, lds, st>>>(desc, lucsr, t32);
 }
 
 // glu_solve in one workgroup: the factors into the sweep layout, then the
 // exact sweeps (mode: kSweepLds or kSweepFlags).
-void solve_one(const MatDesc *desc, uint32_t nnz, uint32_t nextra, const float *lu, float *lucsr, const float *b,
-               float *x, float *t32, int threads, int mode, size_t lds, hipStream_t st)
+void solve_one(const MatDesc *desc, uint32_t nval, const float *lu, float *lucsr, const float *b, float *x,
+               float *t32, int threads, int mode, size_t lds, hipStream_t st)
 {
-    k_copy_df<false><<<blocks_for(std::max(nnz, nextra), 256), 256, 0, st>>>(desc, lu, lucsr);
+    k_copy_df<false><<<blocks_for(nval, 256), 256, 0, st>>>(desc, lu, lucsr);
     if (mode == kSweepLds) k_solve_one<kSweepLds><<<1, threads, lds, st>>>(desc, lucsr, b, x, t32);
     else k_solve_one<kSweepFlags><<<1, threads, lds, st>>>(desc, lucsr, b, x, t32);
 }
@@ -2397,9 +2421,9 @@ size_t enable_lds()
         reinterpret_cast<const void *>(&k_solve_one<kSweepLds>),
         reinterpret_cast<const void *>(&k_solve_one<kSweepFlags>),
         reinterpret_cast<const void *>(&k_refine_one<kSweepLds, false>),
-        reinterpret_cast<const void *>(&k_refine_one<kSweepFlags, false>),
         reinterpret_cast<const void *>(&k_refine_one<kSweepLds, true>),
-        reinterpret_cast<const void *>(&k_refine_one<kSweepFlags, true>),
+        reinterpret_cast<const void *>(&k_sweeps_df<false>),
+        reinterpret_cast<const void *>(&k_sweeps_df<true>),
         reinterpret_cast<const void *>(&k_batch_same_refine<kSweepLds>),
         reinterpret_cast<const void *>(&k_batch_same_refine<kSweepFlags>),
         reinterpret_cast<const void *>(&k_batch_same_refine<kSweepLevels>),
@@ -2439,13 +2463,6 @@ void adjoint_values(const RefineDev &R, const double *lam, const double *x, doub
 void zero_u64(unsigned long long *p, int count, hipStream_t st)
 {
     k_zero_u64<<<1, 64, 0, st>>>(p, count);
-}
-
-void fill_zero(uint32_t *p, uint64_t words, hipStream_t st)
-{
-    if (words == 0) return;
-    const unsigned grid = (unsigned)std::min<uint64_t>(blocks_for(words, 256), 2048);
-    k_fill_zero<<<grid, 256, 0, st>>>(p, words);
 }
 
 size_t matio_size() { return sizeof(MatIO); }

@@ -65,13 +65,14 @@ size_t enable_lds();
 void factor_window(const FactorDev &F, int w, uint32_t nslot, float *val, const float *d_pert, int perturb,
                    int threads, hipStream_t st);
 void zero_u64(unsigned long long *p, int count, hipStream_t st);
-void fill_zero(uint32_t *p, uint64_t words, hipStream_t st);
-void solve_one(const MatDesc *desc, uint32_t nnz, uint32_t nextra, const float *lu, float *lucsr, const float *b,
-               float *x, float *t32, int threads, int mode, size_t lds, hipStream_t st);
-void refine_one(const MatDesc *desc, uint32_t nnz, uint32_t nextra, const double *ax, const double *b, double *x,
+void solve_one(const MatDesc *desc, uint32_t nval, const float *lu, float *lucsr, const float *b, float *x,
+               float *t32, int threads, int mode, size_t lds, hipStream_t st);
+void refine_one(const MatDesc *desc, uint32_t nval, const double *ax, const double *b, double *x,
                 const float *lu, float *lucsr, float *t32, double *r64, double *norms, double *kry, int32_t *iters,
-                double *eta, double tol, int maxit, int threads, int mode, size_t ref_lds, int transpose,
-                hipStream_t st);
+                double *eta, double tol, int maxit, int threads, size_t ref_lds, int transpose, hipStream_t st);
+void copy_df(const MatDesc *desc, uint32_t nval, const float *lu, float *lucsr, int transpose, hipStream_t st);
+void sweeps_df(const MatDesc *desc, const float *lucsr, float *t32, int threads, size_t lds, int transpose,
+               hipStream_t st);
 void adjoint_values(const RefineDev &R, const double *lam, const double *x, double *grad, uint32_t n,
                     hipStream_t st);
 size_t matio_size();
@@ -310,8 +311,7 @@ struct glu_plan {
     int solve_df_opt = 0;        // glu_plan_opts.solve_dataflow
     size_t lds_cap = 0;          // dynamic LDS per workgroup (bytes)
     uint32_t df_nval = 0;        // values of the per-matrix sweep copy
-    uint32_t df_nextra = 0;      // its positions with a second slot
-    uint32_t tdf_nval = 0, tdf_nextra = 0;   // the same for A' x = b
+    uint32_t tdf_nval = 0;       // the same for A' x = b
     float *d_tlucsr = nullptr;   // glu_refine's A' x = b value copy
     // LDS windows allowed (from the last level); windows hold narrow steps
     // only, so small orders get one whatever the cap, and n=500k MNA gains
@@ -1267,40 +1267,20 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
     if (!check_df(dlo, n, P.nnz, "L", err) || !check_df(dup, n, P.nnz, "U", err) ||
         !check_df(dtu, n, P.nnz, "U'", err) || !check_df(dtl, n, P.nnz, "L'", err))
         return fail(GLU_ERR_INVALID, err);
-    // the copy runs in CSC order (contiguous reads, scattered stores): the
-    // slot of each position, plus pairs for positions with a second slot;
-    // padding slots are zeroed once when the copies are allocated
-    auto copy_map = [&](const std::vector<uint32_t> &vpos, std::vector<uint32_t> &dst, std::vector<uint2> &extra,
-                        uint32_t &nextra) {
-        dst.assign(P.nnz, UINT32_MAX);
-        extra.clear();
-        for (uint32_t q = 0; q < (uint32_t)vpos.size(); ++q) {
-            const uint32_t pos = vpos[q];
-            if (pos == UINT32_MAX) continue;
-            if (dst[pos] == UINT32_MAX) dst[pos] = q;
-            else extra.push_back(make_uint2(q, pos));
-        }
-        nextra = (uint32_t)extra.size();
-        if (extra.empty()) extra.push_back(make_uint2(UINT32_MAX, 0u));   // keeps the table non-empty
-    };
+    // the value copies are gathered in slot order (whole-line stores): the
+    // CSC position of each slot, ~0 for padding slots (which get 0)
     std::vector<uint32_t> tdf_vpos(dtu.vpos);   // U' slots, U' pivots (+64 pad), L' slots
     tdf_vpos.insert(tdf_vpos.end(), dtu.pvpos.begin(), dtu.pvpos.end());
     tdf_vpos.resize(tdf_vpos.size() + 64, UINT32_MAX);
     const uint32_t tlo_voff = (uint32_t)tdf_vpos.size();
     tdf_vpos.insert(tdf_vpos.end(), dtl.vpos.begin(), dtl.vpos.end());
     p->tdf_nval = (uint32_t)tdf_vpos.size();
-    std::vector<uint32_t> tdf_dst;
-    std::vector<uint2> tdf_extra;
-    copy_map(tdf_vpos, tdf_dst, tdf_extra, p->tdf_nextra);
     // per-matrix value copy: L slots, U slots, U pivots
     std::vector<uint32_t> df_vpos(dlo.vpos);
     df_vpos.insert(df_vpos.end(), dup.vpos.begin(), dup.vpos.end());
     df_vpos.insert(df_vpos.end(), dup.pvpos.begin(), dup.pvpos.end());
     df_vpos.resize(df_vpos.size() + 64, UINT32_MAX);   // pivot reads of the last positions' lanes
     p->df_nval = (uint32_t)df_vpos.size();
-    std::vector<uint32_t> df_dst;
-    std::vector<uint2> df_extra;
-    copy_map(df_vpos, df_dst, df_extra, p->df_nextra);
 
     // Refinement / refresh tables from the analysis.
     std::vector<uint32_t> a_rptr, a_cidx, a_src, lu_src, lu_col, a_rows, a_cols;
@@ -1407,10 +1387,8 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
     const size_t o_dup_task = U.put(dup.task), o_dup_rl = U.put(dup.rl), o_dup_idx = U.put(dup.idx);
     const size_t o_dtu_task = U.put(dtu.task), o_dtu_rl = U.put(dtu.rl), o_dtu_idx = U.put(dtu.idx);
     const size_t o_dtl_task = U.put(dtl.task), o_dtl_rl = U.put(dtl.rl), o_dtl_idx = U.put(dtl.idx);
-    const size_t o_tdfdst = U.put(tdf_dst);
-    const size_t o_tdfextra = U.put(tdf_extra);
-    const size_t o_dfdst = U.put(df_dst);
-    const size_t o_dfextra = U.put(df_extra);
+    const size_t o_tdfsrc = U.put(tdf_vpos);
+    const size_t o_dfsrc = U.put(df_vpos);
     size_t o_trows[4], o_tptr[4], o_tlong[4], o_ttask[4];
     for (int s = 0; s < 4; ++s) {
         o_trows[s] = U.put(p->tri[s].rows);
@@ -1546,14 +1524,10 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
     S.up_df = df(dup, o_dup_task, o_dup_rl, o_dup_idx, 1, (uint32_t)dlo.idx.size());
     S.tup_df = df(dtu, o_dtu_task, o_dtu_rl, o_dtu_idx, 1, 0);
     S.tlo_df = df(dtl, o_dtl_task, o_dtl_rl, o_dtl_idx, 0, tlo_voff);
-    S.tdf_dst = u32p(o_tdfdst);
-    S.tdf_extra = reinterpret_cast<const uint2 *>(A + o_tdfextra);
+    S.tdf_src = u32p(o_tdfsrc);
     S.tdf_nval = p->tdf_nval;
-    S.tdf_nextra = p->tdf_nextra;
-    S.df_dst = u32p(o_dfdst);
-    S.df_extra = reinterpret_cast<const uint2 *>(A + o_dfextra);
+    S.df_src = u32p(o_dfsrc);
     S.df_nval = p->df_nval;
-    S.df_nextra = p->df_nextra;
     p->tri_dataflow = p->solve_df_opt >= 0 && p->lds_cap >= 4 * (size_t)((n + 31) / 32);
     if (an) {
         S.row_perm = u32p(o_rp);
@@ -1686,7 +1660,6 @@ glu_status ensure_batch(glu_plan *p, int batch)
     HIP_TRY(hipMalloc(&p->d_bt32, sizeof(float) * (size_t)p->n * batch));
     HIP_TRY(hipMalloc(&p->d_br64, sizeof(double) * (size_t)p->n * batch));
     HIP_TRY(hipMalloc(&p->d_blucsr, sizeof(float) * (size_t)p->df_nval * batch));
-    HIP_TRY(hipMemset(p->d_blucsr, 0, sizeof(float) * (size_t)p->df_nval * batch));   // padding slots
     HIP_TRY(hipMalloc(&p->d_bnorms, 2 * sizeof(double) * (size_t)batch));
     if (p->krylov)
         HIP_TRY(hipMalloc(&p->d_bkry, sizeof(double) * (3 * (size_t)p->krylov + 1) * p->n * (size_t)batch));
@@ -1817,7 +1790,7 @@ extern "C" glu_status glu_solve(glu_plan *p, const float *d_lu, const float *d_b
     auto issue = [&](hipStream_t s) {
         const SolveDev &S = p->S;
         if (mode != 0) {
-            launch::solve_one(p->d_desc, p->nnz, p->df_nextra, d_lu, p->d_blucsr, d_b, d_x, p->d_bt32,
+            launch::solve_one(p->d_desc, p->df_nval, d_lu, p->d_blucsr, d_b, d_x, p->d_bt32,
                               p->fused_threads, mode, launch::batch_lds_bytes(mode, p->n), s);
         } else if (!transpose) {
             launch::perm_scale_f32(d_b, p->d_t32, S.col_perm, S.col_scale_perm, 0, p->n, s);
@@ -1853,68 +1826,44 @@ static double bits_to_double(unsigned long long b)
     return d;
 }
 
-extern "C" glu_status glu_refine(glu_plan *p, const double *d_ax, const float *d_lu, const double *d_b,
-                                 double *d_x, double tol, int maxit, int transpose, int *h_iters, double *h_eta,
-                                 void *stream)
+namespace {
+
+// Plain iterative refinement driven from the host, one preconditioner
+// solve per iteration: x = 0, r = b; then t = float(r scaled into the
+// factors' frame), the two sweeps on t, x += t scaled back, r = b - A x with
+// |r|, |x| (and |A|, |b| on the first pass), eta on the host.  The FP64 work
+// runs in grid kernels; the sweeps run in one workgroup over the
+// sweep-layout copy `lucsr` (dataflow, ready bits in LDS) or, when lucsr is
+// null, level by level over the factors.
+glu_status refine_host_loop(glu_plan *p, const double *d_ax, const float *d_lu, const double *d_b, double *d_x,
+                            double tol, int maxit, int transpose, const float *lucsr, float *t32, double *r64,
+                            unsigned long long *norms, int *h_iters, double *h_eta, hipStream_t st)
 {
-    if (!p || !d_ax || !d_lu || !d_b || !d_x || maxit <= 0) return fail(GLU_ERR_ARGUMENT, "bad argument");
-    if (!p->has_refine) return fail(GLU_ERR_ARGUMENT, "plan has no analysis");
-    hipStream_t st = static_cast<hipStream_t>(stream);
     const SolveDev &S = p->S;
     const uint32_t n = p->n;
-    {
-        // one workgroup: the batch pipeline's dataflow sweeps and FGMRES
-        // (plan.krylov), when the sweeps' ready bits fit in LDS
-        const size_t cap = launch::enable_lds();
-        const int mode = sweep_mode(n, cap);
-        if (mode != 0) {
-            glu_status s = ensure_batch(p, 1);
-            if (s != GLU_OK) return s;
-            if (!p->d_rout) HIP_TRY(hipMalloc(&p->d_rout, 16));
-            float *lucsr = p->d_blucsr;
-            if (transpose) {
-                if (!p->d_tlucsr) {
-                    HIP_TRY(hipMalloc(&p->d_tlucsr, sizeof(float) * p->tdf_nval));
-                    HIP_TRY(hipMemset(p->d_tlucsr, 0, sizeof(float) * p->tdf_nval));   // padding slots
-                }
-                lucsr = p->d_tlucsr;
-            }
-            int32_t *d_it = reinterpret_cast<int32_t *>(p->d_rout);
-            double *d_eta = reinterpret_cast<double *>(p->d_rout + 8);
-            launch::refine_one(p->d_desc, p->nnz, transpose ? p->tdf_nextra : p->df_nextra, d_ax, d_b, d_x, d_lu,
-                               lucsr, p->d_bt32, p->d_br64, p->d_bnorms, p->d_bkry, d_it, d_eta, tol, maxit,
-                               p->fused_threads, mode, launch::batch_lds_bytes(mode, n), transpose, st);
-            HIP_TRY(hipGetLastError());
-            int32_t it = 0;
-            double eta = 0.0;
-            HIP_TRY(hipMemcpyAsync(&it, d_it, sizeof it, hipMemcpyDeviceToHost, st));
-            HIP_TRY(hipMemcpyAsync(&eta, d_eta, sizeof eta, hipMemcpyDeviceToHost, st));
-            HIP_TRY(hipStreamSynchronize(st));
-            if (h_iters) *h_iters = it;
-            if (h_eta) *h_eta = eta;
-            if (!(eta == eta)) return fail(GLU_ERR_NUMERIC, "refinement produced NaN");
-            return GLU_OK;
-        }
-    }
     HIP_TRY(hipMemsetAsync(d_x, 0, sizeof(double) * n, st));
-    HIP_TRY(hipMemcpyAsync(p->d_r64, d_b, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    HIP_TRY(hipMemcpyAsync(r64, d_b, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
     double anorm = 0.0, bnorm = 0.0, eta = 0.0;
     int it = 0;
     for (;;) {
         if (!transpose)
-            launch::perm_scale_f64(p->d_r64, p->d_t32, S.col_perm, S.col_scale_perm, 0, n, st);
+            launch::perm_scale_f64(r64, t32, S.col_perm, S.col_scale_perm, 0, n, st);
         else
-            launch::perm_scale_f64(p->d_r64, p->d_t32, S.row_perm, S.row_scale, 1, n, st);
-        run_sweeps(p, d_lu, p->d_t32, transpose, st);
+            launch::perm_scale_f64(r64, t32, S.row_perm, S.row_scale, 1, n, st);
+        if (lucsr)
+            launch::sweeps_df(p->d_desc, lucsr, t32, p->fused_threads, launch::batch_lds_bytes(1, n), transpose, st);
+        else
+            run_sweeps(p, d_lu, t32, transpose, st);
         if (!transpose)
-            launch::update_x(p->d_t32, d_x, S.row_perm_inv, S.row_scale, 0, n, st);
+            launch::update_x(t32, d_x, S.row_perm_inv, S.row_scale, 0, n, st);
         else
-            launch::update_x(p->d_t32, d_x, S.col_perm_inv, S.col_scale_perm, 1, n, st);
-        launch::zero_u64(p->d_norms, 4, st);
+            launch::update_x(t32, d_x, S.col_perm_inv, S.col_scale_perm, 1, n, st);
+        launch::zero_u64(norms, 4, st);
         const uint32_t nl = transpose ? p->a_nlong_t : p->a_nlong;
-        launch::residual(p->R, d_ax, d_b, d_x, p->d_r64, n, transpose, it == 0, p->d_norms, (n - nl) + nl * 64, st);
+        launch::residual(p->R, d_ax, d_b, d_x, r64, n, transpose, it == 0, norms, (n - nl) + nl * 64, st);
+        HIP_TRY(hipGetLastError());
         unsigned long long nb[4];
-        HIP_TRY(hipMemcpyAsync(nb, p->d_norms, sizeof nb, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(nb, norms, sizeof nb, hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
         if (it == 0) {
             anorm = bits_to_double(nb[2]);
@@ -1929,6 +1878,67 @@ extern "C" glu_status glu_refine(glu_plan *p, const double *d_ax, const float *d
     if (h_eta) *h_eta = eta;
     if (!(eta == eta)) return fail(GLU_ERR_NUMERIC, "refinement produced NaN");
     return GLU_OK;
+}
+
+// The one-workgroup refinement (sweeps, FGMRES and the products with A in
+// one workgroup; sweep vector and ready bits in LDS), then its iteration
+// count and eta read back.
+glu_status refine_one_wg(glu_plan *p, const double *d_ax, const float *d_lu, const double *d_b, double *d_x,
+                         double tol, int maxit, int transpose, float *lucsr, float *t32, double *r64,
+                         double *bnorms, double *kry, char *out16, int *h_iters, double *h_eta,
+                         hipStream_t st)
+{
+    const uint32_t n = p->n;
+    int32_t *d_it = reinterpret_cast<int32_t *>(out16);
+    double *d_eta = reinterpret_cast<double *>(out16 + 8);
+    launch::refine_one(p->d_desc, transpose ? p->tdf_nval : p->df_nval, d_ax, d_b, d_x, d_lu, lucsr, t32, r64,
+                       bnorms, kry, d_it, d_eta, tol, maxit, p->fused_threads, launch::batch_lds_bytes(2, n),
+                       transpose, st);
+    HIP_TRY(hipGetLastError());
+    int32_t it = 0;
+    double eta = 0.0;
+    HIP_TRY(hipMemcpyAsync(&it, d_it, sizeof it, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(&eta, d_eta, sizeof eta, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if (h_iters) *h_iters = it;
+    if (h_eta) *h_eta = eta;
+    if (!(eta == eta)) return fail(GLU_ERR_NUMERIC, "refinement produced NaN");
+    return GLU_OK;
+}
+
+}  // namespace
+
+extern "C" glu_status glu_refine(glu_plan *p, const double *d_ax, const float *d_lu, const double *d_b,
+                                 double *d_x, double tol, int maxit, int transpose, int *h_iters, double *h_eta,
+                                 void *stream)
+{
+    if (!p || !d_ax || !d_lu || !d_b || !d_x || maxit <= 0) return fail(GLU_ERR_ARGUMENT, "bad argument");
+    if (!p->has_refine) return fail(GLU_ERR_ARGUMENT, "plan has no analysis");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const uint32_t n = p->n;
+    // 2: the sweep vector fits in LDS -> everything in one workgroup (FGMRES,
+    // plan.krylov); 1: only the ready bits fit -> dataflow sweeps in one
+    // workgroup, FP64 work across the grid (plain refinement); 0: level by
+    // level.
+    const int mode = sweep_mode(n, launch::enable_lds());
+    float *lucsr = nullptr;
+    if (mode != 0) {
+        glu_status s = ensure_batch(p, 1);
+        if (s != GLU_OK) return s;
+        lucsr = p->d_blucsr;
+        if (transpose) {
+            if (!p->d_tlucsr) HIP_TRY(hipMalloc(&p->d_tlucsr, sizeof(float) * p->tdf_nval));
+            lucsr = p->d_tlucsr;
+        }
+    }
+    if (mode == 2) {
+        if (!p->d_rout) HIP_TRY(hipMalloc(&p->d_rout, 16));
+        return refine_one_wg(p, d_ax, d_lu, d_b, d_x, tol, maxit, transpose, lucsr, p->d_bt32, p->d_br64,
+                             p->d_bnorms, p->d_bkry, p->d_rout, h_iters, h_eta, st);
+    }
+    if (mode == 1) launch::copy_df(p->d_desc, transpose ? p->tdf_nval : p->df_nval, d_lu, lucsr, transpose, st);
+    return refine_host_loop(p, d_ax, d_lu, d_b, d_x, tol, maxit, transpose, lucsr, p->d_t32, p->d_r64, p->d_norms,
+                            h_iters, h_eta, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -1982,10 +1992,11 @@ extern "C" glu_status glu_refine_noalloc(glu_plan *p, const double *d_ax, const 
     const int mode = sweep_mode(n, launch::enable_lds());
     if (mode == 0)
         return fail(GLU_ERR_ARGUMENT, "order " + std::to_string(n) +
-                                          " is beyond the one-workgroup refinement (its ready bits exceed LDS): "
+                                          " is beyond the dataflow sweeps (their ready bits exceed LDS): "
                                           "use glu_refine");
-    // scratch layout: sweep value copy, correction vector, residual, norms,
-    // FGMRES vectors, iteration count + eta
+    // scratch layout: sweep value copy, correction vector, residual, norms
+    // (one-workgroup path: FP64 |b| slots, FGMRES vectors, iteration count +
+    // eta; grid path: the four norm words)
     const uint32_t nval = transpose ? p->tdf_nval : p->df_nval;
     size_t off = 0;
     auto take = [&](size_t bytes) {
@@ -1996,8 +2007,8 @@ extern "C" glu_status glu_refine_noalloc(glu_plan *p, const double *d_ax, const 
     const size_t o_val = take(sizeof(float) * (size_t)nval);
     const size_t o_t32 = take(sizeof(float) * (size_t)n);
     const size_t o_r64 = take(sizeof(double) * (size_t)n);
-    const size_t o_norms = take(2 * sizeof(double));
-    const size_t o_kry = take(p->krylov ? sizeof(double) * (3 * (size_t)p->krylov + 1) * n : 0);
+    const size_t o_norms = take(4 * sizeof(unsigned long long));
+    const size_t o_kry = take(mode == 2 && p->krylov ? sizeof(double) * (3 * (size_t)p->krylov + 1) * n : 0);
     const size_t o_out = take(16);
     if (off > kScratchBytes)
         return fail(GLU_ERR_ARGUMENT, "refinement needs " + std::to_string(off) + " bytes of scratch; the static " +
@@ -2007,25 +2018,16 @@ extern "C" glu_status glu_refine_noalloc(glu_plan *p, const double *d_ax, const 
     if (s != GLU_OK) return s;
     hipStream_t st = static_cast<hipStream_t>(stream);
     float *lucsr = reinterpret_cast<float *>(base + o_val);
-    int32_t *d_it = reinterpret_cast<int32_t *>(base + o_out);
-    double *d_eta = reinterpret_cast<double *>(base + o_out + 8);
-    // the copy's padding slots must read 0 (the array keeps the last call's data)
-    launch::fill_zero(reinterpret_cast<uint32_t *>(lucsr), nval, st);
-    launch::refine_one(p->d_desc, p->nnz, transpose ? p->tdf_nextra : p->df_nextra, d_ax, d_b, d_x, d_lu, lucsr,
-                       reinterpret_cast<float *>(base + o_t32), reinterpret_cast<double *>(base + o_r64),
-                       reinterpret_cast<double *>(base + o_norms),
-                       p->krylov ? reinterpret_cast<double *>(base + o_kry) : nullptr, d_it, d_eta, tol, maxit,
-                       p->fused_threads, mode, launch::batch_lds_bytes(mode, n), transpose, st);
-    HIP_TRY(hipGetLastError());
-    int32_t it = 0;
-    double eta = 0.0;
-    HIP_TRY(hipMemcpyAsync(&it, d_it, sizeof it, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(&eta, d_eta, sizeof eta, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
-    if (h_iters) *h_iters = it;
-    if (h_eta) *h_eta = eta;
-    if (!(eta == eta)) return fail(GLU_ERR_NUMERIC, "refinement produced NaN");
-    return GLU_OK;
+    float *t32 = reinterpret_cast<float *>(base + o_t32);
+    double *r64 = reinterpret_cast<double *>(base + o_r64);
+    if (mode == 2)   // as glu_refine: everything in one workgroup
+        return refine_one_wg(p, d_ax, d_lu, d_b, d_x, tol, maxit, transpose, lucsr, t32, r64,
+                             reinterpret_cast<double *>(base + o_norms),
+                             p->krylov ? reinterpret_cast<double *>(base + o_kry) : nullptr,
+                             reinterpret_cast<char *>(base + o_out), h_iters, h_eta, st);
+    launch::copy_df(p->d_desc, nval, d_lu, lucsr, transpose, st);   // as glu_refine: sweeps in one workgroup
+    return refine_host_loop(p, d_ax, d_lu, d_b, d_x, tol, maxit, transpose, lucsr, t32, r64,
+                            reinterpret_cast<unsigned long long *>(base + o_norms), h_iters, h_eta, st);
 }
 
 extern "C" glu_status glu_adjoint_values(glu_plan *p, const double *d_lam, const double *d_x, double *d_grad_ax,
@@ -2187,7 +2189,6 @@ extern "C" glu_status glu_group_create(glu_plan *const *plans, int count, glu_gr
     HIP_TRY(hipMalloc(&g->d_t32, sizeof(float) * vtot));
     HIP_TRY(hipMalloc(&g->d_r64, sizeof(double) * vtot));
     HIP_TRY(hipMalloc(&g->d_lucsr, sizeof(float) * ftot));
-    HIP_TRY(hipMemset(g->d_lucsr, 0, sizeof(float) * ftot));   // padding slots
     HIP_TRY(hipMalloc(&g->d_norms, 2 * sizeof(double) * count));
     if (ktot) HIP_TRY(hipMalloc(&g->d_kry, sizeof(double) * ktot));
     HIP_TRY(hipMemcpy(g->d_descs, descs.data(), sizeof(MatDesc) * count, hipMemcpyHostToDevice));

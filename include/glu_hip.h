@@ -181,7 +181,12 @@ glu_status glu_refresh(glu_plan *p, const double *d_ax, float *d_lu, int batch, 
 /* Iterative refinement of A x = b (or A^T x = b): FP32 solves with d_lu,
  * FP64 residual r = b - A x on the original A values d_ax and FP64 update,
  * until eta = |r|_inf / (|A|_inf |x|_inf + |b|_inf) <= tol or maxit.
- * x starts from 0.  h_iters/h_eta (host, may be NULL) receive the result. */
+ * x starts from 0.  h_iters/h_eta (host, may be NULL) receive the result
+ * (h_iters counts preconditioner solves).  Orders whose FP32 sweep vector
+ * fits in LDS run FGMRES(plan krylov) entirely in one workgroup; larger
+ * orders run plain refinement with the two sweeps in one workgroup and the
+ * FP64 work in grid kernels (or, beyond the sweeps' LDS ready bits, level
+ * by level).  Synchronous. */
 glu_status glu_refine(glu_plan *p, const double *d_ax, const float *d_lu, const double *d_b,
                       double *d_x, double tol, int maxit, int transpose, int *h_iters,
                       double *h_eta, void *stream);

@@ -1,9 +1,9 @@
 """glu_refine_noalloc (the refinement with its scratch in a static device
 array, for callers that cannot allocate) against glu_refine, through the
 C-ABI: the same solution bits, iteration count and eta on two calls in a
-row, the second on a smaller matrix (the array keeps the first call's
-contents), for A x = b and A' x = b; and the size limit rejected with a
-clear error."""
+row, each on a smaller matrix than the one before (the array keeps the
+previous call's contents), for A x = b and A' x = b, through both of
+glu_refine's paths; and the size limit rejected with a clear error."""
 
 import ctypes as C
 
@@ -36,7 +36,10 @@ def test_noalloc_matches_refine(transpose):
     from pyglu import mna
     from pyglu.analysis import Analysis
     from pyglu.device import Plan
-    for name in ("small", "mna2k"):   # n = 4,960, then the smaller n = 2,000
+    # n = 50,000 (sweep vector beyond LDS: dataflow sweeps in one workgroup,
+    # FP64 work across the grid), then the smaller n = 4,960 and n = 2,000
+    # (everything in one workgroup)
+    for name in ("grad", "small", "mna2k"):
         d, i, p, n = mna_case(name)
         plan = Plan(Analysis(d, i, p, n))
         ax = torch.from_numpy(mna.perturb_values(d, 11)).cuda()
@@ -61,11 +64,16 @@ def test_noalloc_rejects_beyond_limit():
     from pyglu import _capi as capi
     from pyglu.analysis import Analysis
     from pyglu.device import Plan
-    # diagonal, n = 1.2M: the sweeps' ready bits fit LDS (one-workgroup
-    # refinement), its scratch (FGMRES vectors alone 67 MB) does not fit 64 MiB
-    n = 1_200_000
-    A = sp.diags([np.random.default_rng(1).uniform(1.0, 2.0, n)], [0], format="csc")
-    A.sort_indices()
+    # n = 1M in dense 16 x 16 diagonal blocks (16M stored entries, no fill):
+    # the sweeps' ready bits fit LDS, the refinement's scratch (the sweep
+    # copy of 16M factor values alone is 64 MB) does not fit 64 MiB
+    n, k = 1_000_000, 16
+    rng = np.random.default_rng(1)
+    indptr = np.arange(0, n * k + 1, k, dtype=np.int64)
+    indices = ((np.arange(n) // k * k)[:, None] + np.arange(k)[None, :]).reshape(-1)
+    data = rng.uniform(-1.0, 1.0, n * k)
+    data[indices == np.repeat(np.arange(n), k)] += 2.0 * k   # diagonally dominant
+    A = sp.csc_matrix((data, indices, indptr), shape=(n, n))
     plan = Plan(Analysis(A.data, A.indices.astype(np.int32), A.indptr.astype(np.int32), n))
     ax = torch.from_numpy(A.data.copy()).cuda()
     lu = plan.refresh(ax).reshape(-1)
