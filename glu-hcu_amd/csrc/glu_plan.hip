@@ -15,6 +15,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <exception>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -501,16 +502,69 @@ struct Tables {
     std::vector<uint32_t> dvd_ptr;     // [num_lev + 1]
 };
 
+// Runs work(worker, i) for i in [0, count) on up to `threads` host threads
+// (the calling one included), items taken in increasing order from a shared
+// counter.  The first exception thrown in any worker is rethrown here after
+// every thread has finished.
+template <typename Work>
+void parallel_items(size_t count, unsigned threads, Work work)
+{
+    threads = (unsigned)std::max<size_t>(1, std::min<size_t>(threads, count));
+    std::atomic<size_t> next(0);
+    std::exception_ptr failure;
+    std::mutex fail_mu;
+    auto run = [&](unsigned worker) {
+        try {
+            for (size_t i = next.fetch_add(1); i < count; i = next.fetch_add(1)) work(worker, i);
+        } catch (...) {
+            std::lock_guard<std::mutex> lock(fail_mu);
+            if (!failure) failure = std::current_exception();
+            next.store(count);
+        }
+    };
+    std::vector<std::thread> pool;
+    try {
+        pool.reserve(threads - 1);
+        for (unsigned t = 1; t < threads; ++t) pool.emplace_back(run, t);
+    } catch (const std::exception &) {
+        // fewer threads than asked for: the ones that started share the work
+    }
+    run(0);
+    for (std::thread &t : pool) t.join();
+    if (failure) std::rethrow_exception(failure);
+}
+
+// Host threads for building plan tables: the cores available, at most 16.
+unsigned table_threads()
+{
+    const unsigned hw = std::thread::hardware_concurrency();
+    return std::max(1u, std::min(16u, hw ? hw : 1u));
+}
+
+// One level's factorization items (the level's slice of Tables).
+struct LevelItems {
+    std::vector<uint4> cln;
+    std::vector<uint32_t> ctgt, cup;
+    std::vector<uint32_t> ment, msrc_cnt;
+    uint32_t ment_long = 0;
+    std::vector<MergeSrc> msrc;
+    std::vector<uint2> dvd;
+};
+
 // Lane items of the factorization, level by level.  For pivot column k of
 // level l, every (i in L_k, j in U_k) update targets the CSC position of
 // (i,j) in column j; the fill closure of the pattern guarantees it exists
 // (checked).  Targets that two or more columns of the level update become
 // merge entries whose sources keep the level's column order (level_idx lists
 // a level's columns in increasing order); all other updates are items.
+// The update map is built per target column and the items per level, each
+// on host threads; levels are then concatenated in order, so the tables do
+// not depend on the thread count.
 bool build_factor_tables(const glu_pattern &P, Tables &T, int64_t &updates, std::string &err)
 {
     const uint32_t n = P.n, nnz = P.nnz;
     const int nl = P.num_lev;
+    const unsigned threads = table_threads();
     T.upos.resize(nnz);
     std::vector<uint32_t> csc2csr(nnz);
     {
@@ -522,7 +576,8 @@ bool build_factor_tables(const glu_pattern &P, Tables &T, int64_t &updates, std:
                 csc2csr[p] = q;
             }
     }
-    // Scatter targets per (k, jj, ii), built per target column j.
+    // Scatter targets per (k, jj, ii), built per target column j (columns
+    // write disjoint ranges of umap).
     std::vector<uint64_t> moff(n + 1);
     uint64_t off = 0;
     for (uint32_t k = 0; k < n; ++k) {
@@ -541,42 +596,70 @@ bool build_factor_tables(const glu_pattern &P, Tables &T, int64_t &updates, std:
         return false;
     }
     {
-        std::vector<uint32_t> stamp(n, UINT32_MAX), where(n);
-        for (uint32_t j = 0; j < n; ++j) {
-            for (uint32_t p = P.sym_c_ptr[j]; p < P.sym_c_ptr[j + 1]; ++p) {
-                stamp[P.sym_r_idx[p]] = j;
-                where[P.sym_r_idx[p]] = p;
+        constexpr uint32_t kCols = 256;   // target columns per work item
+        std::vector<std::vector<uint32_t>> stamps(threads), wheres(threads);
+        std::atomic<uint32_t> bad_col(UINT32_MAX);
+        parallel_items((n + kCols - 1) / kCols, threads, [&](unsigned w, size_t item) {
+            std::vector<uint32_t> &stamp = stamps[w], &where = wheres[w];
+            if (stamp.empty()) {
+                stamp.assign(n, UINT32_MAX);
+                where.assign(n, 0);
             }
-            for (uint32_t p = P.sym_c_ptr[j]; p < P.l_col_ptr[j]; ++p) {
-                const uint32_t k = P.sym_r_idx[p];
-                const uint32_t L = P.sym_c_ptr[k + 1] - P.l_col_ptr[k] - 1;
-                if (L == 0) continue;
-                const uint32_t jj = csc2csr[p] - P.csr_diag_ptr[k] - 1;
-                const uint64_t base = moff[k] + (uint64_t)jj * L;
-                const uint32_t l0 = P.l_col_ptr[k] + 1;
-                for (uint32_t ii = 0; ii < L; ++ii) {
-                    const uint32_t i = P.sym_r_idx[l0 + ii];
-                    if (stamp[i] != j) {
-                        err = "pattern is not closed under fill: (" + std::to_string(i) + "," + std::to_string(j) +
-                              ") missing";
-                        return false;
+            const uint32_t j1 = std::min<uint32_t>(n, (uint32_t)(item + 1) * kCols);
+            for (uint32_t j = (uint32_t)item * kCols; j < j1; ++j) {
+                for (uint32_t p = P.sym_c_ptr[j]; p < P.sym_c_ptr[j + 1]; ++p) {
+                    stamp[P.sym_r_idx[p]] = j;
+                    where[P.sym_r_idx[p]] = p;
+                }
+                for (uint32_t p = P.sym_c_ptr[j]; p < P.l_col_ptr[j]; ++p) {
+                    const uint32_t k = P.sym_r_idx[p];
+                    const uint32_t L = P.sym_c_ptr[k + 1] - P.l_col_ptr[k] - 1;
+                    if (L == 0) continue;
+                    const uint32_t jj = csc2csr[p] - P.csr_diag_ptr[k] - 1;
+                    const uint64_t base = moff[k] + (uint64_t)jj * L;
+                    const uint32_t l0 = P.l_col_ptr[k] + 1;
+                    for (uint32_t ii = 0; ii < L; ++ii) {
+                        const uint32_t i = P.sym_r_idx[l0 + ii];
+                        if (stamp[i] != j) {
+                            uint32_t none = UINT32_MAX;
+                            bad_col.compare_exchange_strong(none, j);
+                            return;
+                        }
+                        umap[base + ii] = where[i];
                     }
-                    umap[base + ii] = where[i];
                 }
             }
+        });
+        if (bad_col.load() != UINT32_MAX) {
+            err = "pattern is not closed under fill (column " + std::to_string(bad_col.load()) + ")";
+            return false;
         }
     }
-    // Items per level.
-    T.ctask_ptr.assign(nl + 2, 0);
-    T.ment_ptr.assign(nl + 2, 0);
-    T.ment_long.assign(nl + 1, 0);
-    T.dvd_ptr.assign(nl + 1, 0);
-    T.msrc_ptr.assign(1, 0);
-    std::vector<uint32_t> stamp(nnz, UINT32_MAX), cnt(nnz, 0), slot(nnz, 0), touched;
-    for (int l = 0; l < nl; ++l) {
-        T.ctask_ptr[l] = (uint32_t)(T.cln.size() / 64);
-        T.ment_ptr[l] = (uint32_t)T.ment.size();
-        T.dvd_ptr[l] = (uint32_t)T.dvd.size();
+    // Items per level, on host threads (each with its own marks over the
+    // positions), heaviest levels first.
+    std::vector<LevelItems> lv(nl);
+    std::vector<uint64_t> lwork(nl, 0);
+    for (int l = 0; l < nl; ++l)
+        for (int t = P.level_ptr[l]; t < P.level_ptr[l + 1]; ++t) {
+            const uint32_t k = (uint32_t)P.level_idx[t];
+            lwork[l] += (moff[k + 1] - moff[k]) + 1;
+        }
+    std::vector<int> lorder(nl);
+    for (int l = 0; l < nl; ++l) lorder[l] = l;
+    std::stable_sort(lorder.begin(), lorder.end(), [&](int a, int b) { return lwork[a] > lwork[b]; });
+    struct Marks {
+        std::vector<uint32_t> stamp, cnt, slot, touched;
+    };
+    std::vector<Marks> marks(threads);
+    parallel_items((size_t)nl, threads, [&](unsigned w, size_t item) {
+        const int l = lorder[item];
+        Marks &M = marks[w];
+        if (M.stamp.empty()) {
+            M.stamp.assign(nnz, UINT32_MAX);
+            M.cnt.assign(nnz, 0);
+            M.slot.assign(nnz, 0);
+        }
+        LevelItems &out = lv[l];
         auto each_update = [&](auto f) {
             for (int t = P.level_ptr[l]; t < P.level_ptr[l + 1]; ++t) {
                 const uint32_t k = (uint32_t)P.level_idx[t];
@@ -592,30 +675,30 @@ bool build_factor_tables(const glu_pattern &P, Tables &T, int64_t &updates, std:
                 }
             }
         };
-        touched.clear();
+        M.touched.clear();
         each_update([&](uint32_t tgt, uint32_t, uint32_t, uint32_t) {
-            if (stamp[tgt] != (uint32_t)l) {
-                stamp[tgt] = (uint32_t)l;
-                cnt[tgt] = 0;
-                touched.push_back(tgt);
+            if (M.stamp[tgt] != (uint32_t)l) {
+                M.stamp[tgt] = (uint32_t)l;
+                M.cnt[tgt] = 0;
+                M.touched.push_back(tgt);
             }
-            ++cnt[tgt];
+            ++M.cnt[tgt];
         });
-        std::sort(touched.begin(), touched.end());
-        const uint32_t first = (uint32_t)T.ment.size();
+        std::sort(M.touched.begin(), M.touched.end());
         // long merge entries (a wave each) first, then the rest
+        std::vector<uint32_t> fillpos;
         for (int pass = 0; pass < 2; ++pass)
-            for (uint32_t tgt : touched) {
-                if (cnt[tgt] < 2 || (cnt[tgt] > kLongMerge) != (pass == 0)) continue;
-                slot[tgt] = (uint32_t)T.ment.size();
-                T.ment.push_back(tgt);
-                T.msrc_ptr.push_back(T.msrc_ptr.back() + cnt[tgt]);
-                if (pass == 0) ++T.ment_long[l];
+            for (uint32_t tgt : M.touched) {
+                if (M.cnt[tgt] < 2 || (M.cnt[tgt] > kLongMerge) != (pass == 0)) continue;
+                M.slot[tgt] = (uint32_t)out.ment.size();
+                out.ment.push_back(tgt);
+                out.msrc_cnt.push_back(M.cnt[tgt]);
+                fillpos.push_back((uint32_t)out.msrc.size());
+                out.msrc.resize(out.msrc.size() + M.cnt[tgt]);
+                if (pass == 0) ++out.ment_long;
             }
-        std::vector<uint32_t> fillpos(T.msrc_ptr.begin() + first, T.msrc_ptr.end() - 1);
-        T.msrc.resize(T.msrc_ptr.back());
         each_update([&](uint32_t tgt, uint32_t lpos, uint32_t up, uint32_t dpos) {
-            if (cnt[tgt] >= 2) T.msrc[fillpos[slot[tgt] - first]++] = MergeSrc{lpos, up, dpos, 0};
+            if (M.cnt[tgt] >= 2) out.msrc[fillpos[M.slot[tgt]]++] = MergeSrc{lpos, up, dpos, 0};
         });
         // Column tasks: a lane takes one L entry A(i,k) of a pivot column
         // and up to four consecutive entries of its U list; lanes sharing a
@@ -633,17 +716,17 @@ bool build_factor_tables(const glu_pattern &P, Tables &T, int64_t &updates, std:
                 const uint32_t nu = std::min<uint32_t>(4, U - c0);
                 for (uint32_t ii = 0; ii < L; ++ii) {
                     if (used == 64) {
-                        T.cln.resize(T.cln.size() + 64, make_uint4(0u, 0u, 0u, 0u));
-                        T.ctgt.resize(T.ctgt.size() + 256, UINT32_MAX);
-                        T.cup.resize(T.cup.size() + 256, 0u);
+                        out.cln.resize(out.cln.size() + 64, make_uint4(0u, 0u, 0u, 0u));
+                        out.ctgt.resize(out.ctgt.size() + 256, UINT32_MAX);
+                        out.cup.resize(out.cup.size() + 256, 0u);
                         used = 0;
                     }
-                    const size_t lane = T.cln.size() - 64 + used, slot0 = T.ctgt.size() - 256 + used;
-                    T.cln[lane] = make_uint4(dpos + 1 + ii, dpos, nu, 0u);
+                    const size_t lane = out.cln.size() - 64 + used, slot0 = out.ctgt.size() - 256 + used;
+                    out.cln[lane] = make_uint4(dpos + 1 + ii, dpos, nu, 0u);
                     for (uint32_t e = 0; e < nu; ++e) {
                         const uint32_t tgt = umap[moff[k] + (uint64_t)(c0 + e) * L + ii];
-                        T.ctgt[slot0 + 64 * e] = cnt[tgt] >= 2 ? UINT32_MAX : tgt;
-                        T.cup[slot0 + 64 * e] = T.upos[uf + c0 + e];
+                        out.ctgt[slot0 + 64 * e] = M.cnt[tgt] >= 2 ? UINT32_MAX : tgt;
+                        out.cup[slot0 + 64 * e] = T.upos[uf + c0 + e];
                     }
                     ++used;
                 }
@@ -653,8 +736,44 @@ bool build_factor_tables(const glu_pattern &P, Tables &T, int64_t &updates, std:
             const uint32_t k = (uint32_t)P.level_idx[t];
             const uint32_t dpos = P.l_col_ptr[k];
             for (uint32_t p = dpos + 1; p < P.sym_c_ptr[k + 1]; ++p)
-                T.dvd.push_back(make_uint2(p, dpos | (p == dpos + 1 ? kFirstBit : 0u)));
+                out.dvd.push_back(make_uint2(p, dpos | (p == dpos + 1 ? kFirstBit : 0u)));
         }
+    });
+    umap = std::vector<uint32_t>();
+    // concatenation in level order
+    T.ctask_ptr.assign(nl + 2, 0);
+    T.ment_ptr.assign(nl + 2, 0);
+    T.ment_long.assign(nl + 1, 0);
+    T.dvd_ptr.assign(nl + 1, 0);
+    T.msrc_ptr.assign(1, 0);
+    size_t ncln = 0, nment = 0, nmsrc = 0, ndvd = 0;
+    for (const LevelItems &o : lv) {
+        ncln += o.cln.size();
+        nment += o.ment.size();
+        nmsrc += o.msrc.size();
+        ndvd += o.dvd.size();
+    }
+    T.cln.reserve(ncln);
+    T.ctgt.reserve(ncln * 4);
+    T.cup.reserve(ncln * 4);
+    T.ment.reserve(nment);
+    T.msrc_ptr.reserve(nment + 1);
+    T.msrc.reserve(nmsrc);
+    T.dvd.reserve(ndvd);
+    for (int l = 0; l < nl; ++l) {
+        LevelItems &o = lv[l];
+        T.ctask_ptr[l] = (uint32_t)(T.cln.size() / 64);
+        T.ment_ptr[l] = (uint32_t)T.ment.size();
+        T.dvd_ptr[l] = (uint32_t)T.dvd.size();
+        T.ment_long[l] = o.ment_long;
+        T.cln.insert(T.cln.end(), o.cln.begin(), o.cln.end());
+        T.ctgt.insert(T.ctgt.end(), o.ctgt.begin(), o.ctgt.end());
+        T.cup.insert(T.cup.end(), o.cup.begin(), o.cup.end());
+        T.ment.insert(T.ment.end(), o.ment.begin(), o.ment.end());
+        for (uint32_t c : o.msrc_cnt) T.msrc_ptr.push_back(T.msrc_ptr.back() + c);
+        T.msrc.insert(T.msrc.end(), o.msrc.begin(), o.msrc.end());
+        T.dvd.insert(T.dvd.end(), o.dvd.begin(), o.dvd.end());
+        o = LevelItems();
     }
     T.ctask_ptr[nl] = T.ctask_ptr[nl + 1] = (uint32_t)(T.cln.size() / 64);
     T.ment_ptr[nl] = T.ment_ptr[nl + 1] = (uint32_t)T.ment.size();
@@ -666,9 +785,6 @@ bool build_factor_tables(const glu_pattern &P, Tables &T, int64_t &updates, std:
     return true;
 }
 
-// Level sets of one triangular sweep.  dir > 0: rows ascending, dependencies
-// on smaller indices; dir < 0: descending.  deps(r, f) calls f(c) for each
-// dependency c of row r.
 // Level sets of one triangular sweep.  dir > 0: rows ascending, dependencies
 // on smaller indices; dir < 0: descending.  deps(r, f) calls f(c) for each
 // dependency c of row r; len(r) is the row's entry count.  Within a level,
@@ -1246,27 +1362,44 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
     // sweep vector beyond LDS (the richest sweep mode does not fit): long
     // lane rows in wide levels
     const bool long_lanes = launch::batch_lds_bytes(2, n) > p->lds_cap;
-    DfHost dlo, dup;
-    build_df(p->tri[0], [&](uint32_t r, std::vector<uint2> &out) {
-        for (uint32_t q = P.csr_r_ptr[r]; q < P.csr_diag_ptr[r]; ++q) out.push_back(make_uint2(P.csr_c_idx[q], T.upos[q]));
-    }, false, [&](uint32_t) { return 0u; }, long_lanes, dlo);
-    build_df(p->tri[1], [&](uint32_t r, std::vector<uint2> &out) {
-        for (uint32_t q = P.csr_r_ptr[r + 1]; q > P.csr_diag_ptr[r] + 1; --q)
-            out.push_back(make_uint2(P.csr_c_idx[q - 1], T.upos[q - 1]));
-    }, true, [&](uint32_t r) { return P.l_col_ptr[r]; }, long_lanes, dup);
-    // A' x = b: rows of U' (columns of U, pivot last) forward, rows of L'
-    // (columns of L) backward
-    DfHost dtu, dtl;
-    build_df(p->tri[2], [&](uint32_t r, std::vector<uint2> &out) {
-        for (uint32_t q = P.sym_c_ptr[r]; q < P.l_col_ptr[r]; ++q) out.push_back(make_uint2(P.sym_r_idx[q], q));
-    }, true, [&](uint32_t r) { return P.l_col_ptr[r]; }, long_lanes, dtu);
-    build_df(p->tri[3], [&](uint32_t r, std::vector<uint2> &out) {
-        for (uint32_t q = P.sym_c_ptr[r + 1]; q > P.l_col_ptr[r] + 1; --q)
-            out.push_back(make_uint2(P.sym_r_idx[q - 1], q - 1));
-    }, false, [&](uint32_t) { return 0u; }, long_lanes, dtl);
-    if (!check_df(dlo, n, P.nnz, "L", err) || !check_df(dup, n, P.nnz, "U", err) ||
-        !check_df(dtu, n, P.nnz, "U'", err) || !check_df(dtl, n, P.nnz, "L'", err))
-        return fail(GLU_ERR_INVALID, err);
+    // the four sweeps' dataflow tables (L, U; U', L' for A' x = b: rows of
+    // U' are columns of U with the pivot last, forward; rows of L' columns
+    // of L, backward), built and checked on four host threads
+    DfHost dlo, dup, dtu, dtl;
+    std::string errs[4];
+    bool ok[4] = {false, false, false, false};
+    parallel_items(4, std::min(4u, table_threads()), [&](unsigned, size_t which) {
+        switch (which) {
+        case 0:
+            build_df(p->tri[0], [&](uint32_t r, std::vector<uint2> &out) {
+                for (uint32_t q = P.csr_r_ptr[r]; q < P.csr_diag_ptr[r]; ++q)
+                    out.push_back(make_uint2(P.csr_c_idx[q], T.upos[q]));
+            }, false, [&](uint32_t) { return 0u; }, long_lanes, dlo);
+            ok[0] = check_df(dlo, n, P.nnz, "L", errs[0]);
+            break;
+        case 1:
+            build_df(p->tri[1], [&](uint32_t r, std::vector<uint2> &out) {
+                for (uint32_t q = P.csr_r_ptr[r + 1]; q > P.csr_diag_ptr[r] + 1; --q)
+                    out.push_back(make_uint2(P.csr_c_idx[q - 1], T.upos[q - 1]));
+            }, true, [&](uint32_t r) { return P.l_col_ptr[r]; }, long_lanes, dup);
+            ok[1] = check_df(dup, n, P.nnz, "U", errs[1]);
+            break;
+        case 2:
+            build_df(p->tri[2], [&](uint32_t r, std::vector<uint2> &out) {
+                for (uint32_t q = P.sym_c_ptr[r]; q < P.l_col_ptr[r]; ++q) out.push_back(make_uint2(P.sym_r_idx[q], q));
+            }, true, [&](uint32_t r) { return P.l_col_ptr[r]; }, long_lanes, dtu);
+            ok[2] = check_df(dtu, n, P.nnz, "U'", errs[2]);
+            break;
+        default:
+            build_df(p->tri[3], [&](uint32_t r, std::vector<uint2> &out) {
+                for (uint32_t q = P.sym_c_ptr[r + 1]; q > P.l_col_ptr[r] + 1; --q)
+                    out.push_back(make_uint2(P.sym_r_idx[q - 1], q - 1));
+            }, false, [&](uint32_t) { return 0u; }, long_lanes, dtl);
+            ok[3] = check_df(dtl, n, P.nnz, "L'", errs[3]);
+        }
+    });
+    for (int q = 0; q < 4; ++q)
+        if (!ok[q]) return fail(GLU_ERR_INVALID, errs[q]);
     // the value copies are gathered in slot order (whole-line stores): the
     // CSC position of each slot, ~0 for padding slots (which get 0)
     std::vector<uint32_t> tdf_vpos(dtu.vpos);   // U' slots, U' pivots (+64 pad), L' slots
