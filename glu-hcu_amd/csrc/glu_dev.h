@@ -162,6 +162,12 @@ struct SolveDev {
     DfDev lo_df, up_df;        // L, U over the per-matrix value copy
     const uint32_t *df_src;    // value copy: CSC position of each slot (~0u: padding, holds 0) [df_nval]
     uint32_t df_nval;          // slots per matrix
+    // the same copy staged in LDS, df_cpy_range slots at a time: per range
+    // r, entries [df_cpy_ptr[r], df_cpy_ptr[r + 1]) of df_cpy, {CSC
+    // position, slot - r df_cpy_range} in increasing position order
+    const uint2 *df_cpy;
+    const uint32_t *df_cpy_ptr;
+    uint32_t df_cpy_range;
     // A' x = b over a second value copy (U' slots, U' pivots, L' slots),
     // the same kind of tables
     DfDev tup_df, tlo_df;

@@ -486,8 +486,32 @@ __device__ __forceinline__ void factor_all_wg(const FactorDev &F, float *__restr
         l = W.l1;
     }
     factor_levels_wg<true, false>(F, val, l, F.num_lev + 1, pert, perturb, wlds);
-    const uint32_t nd = F.dvd_ptr[F.num_lev];
-    for (uint32_t t = threadIdx.x; t < nd; t += blockDim.x) factor_divide(val, F.dvd[t], pert, perturb);
+    // the divisions, eight items a lane at a time (their loads issued
+    // together); items are independent: an item writes its L entry and, for
+    // a perturbed pivot, the pivot, which the column's other items divide
+    // by either way (see factor_divide)
+    const uint32_t nd = F.dvd_ptr[F.num_lev], tid = threadIdx.x, nt = blockDim.x;
+    constexpr int U = 8;
+    for (uint32_t t0 = tid; t0 < nd; t0 += U * nt) {
+        uint2 it[U];
+        float d[U], x[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) it[u] = t0 + u * nt < nd ? F.dvd[t0 + u * nt] : make_uint2(~0u, 0u);
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (it[u].x != ~0u) {
+                d[u] = val[it[u].y & ~kFirstBit];
+                x[u] = val[it[u].x];
+            }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (it[u].x != ~0u) {
+                const uint32_t dpos = it[u].y & ~kFirstBit;
+                const bool pt = perturb && fabsf(d[u]) < pert;
+                val[it[u].x] = x[u] / (pt ? pert : d[u]);
+                if (pt && (it[u].y & kFirstBit)) val[dpos] = pert;
+            }
+    }
     __syncthreads();
 }
 
@@ -1586,25 +1610,37 @@ __device__ __forceinline__ void wg_factor(const MatDesc *Md, const MatIO &io, in
         factor_all_wg(F, io.lu, g_lds_vec, pert, perturb, wlds);
     }
 
-    // 4. the factors gathered into the sweeps' task-contiguous layout: slot
-    //    order (contiguous, whole-line stores), each slot reading its CSC
-    //    position; padding slots get 0
+    // 4. the factors copied into the sweeps' task-contiguous layout, built
+    //    in LDS (free until step 5) one range of slots at a time: the
+    //    range's sources read in increasing CSC position (few lines per
+    //    wave) and placed in LDS, then the range written out whole
+    //    (contiguous, whole-line stores); padding slots get 0
     if (kCopy) {
-        const uint32_t *src = glb(Md->s.df_src);
-        const uint32_t nval = Md->s.df_nval;
-        constexpr int U = 8;
-        for (uint32_t q0 = tid; q0 < nval; q0 += U * nt) {
-            uint32_t p[U];
-            float v[U];
+        const uint2 *cpy = glb(Md->s.df_cpy);
+        const uint32_t *cptr = glb(Md->s.df_cpy_ptr);
+        const uint32_t nval = Md->s.df_nval, range = Md->s.df_cpy_range;
+        float *buf = g_lds_vec;
+        for (uint32_t base = 0, r = 0; base < nval; base += range, ++r) {
+            const uint32_t len = min(range, nval - base);
+            for (uint32_t i = tid; i < len; i += nt) buf[i] = 0.f;
+            __syncthreads();
+            const uint32_t e1 = cptr[r + 1];
+            constexpr int U = 8;
+            for (uint32_t k0 = cptr[r] + tid; k0 < e1; k0 += U * nt) {
+                uint2 e[U];
+                float v[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) p[u] = q0 + u * nt < nval ? src[q0 + u * nt] : ~0u;
+                for (int u = 0; u < U; ++u) e[u] = k0 + u * nt < e1 ? cpy[k0 + u * nt] : make_uint2(0u, ~0u);
 #pragma unroll
-            for (int u = 0; u < U; ++u) v[u] = p[u] != ~0u ? io.lu[p[u]] : 0.f;
+                for (int u = 0; u < U; ++u) v[u] = e[u].y != ~0u ? io.lu[e[u].x] : 0.f;
 #pragma unroll
-            for (int u = 0; u < U; ++u)
-                if (q0 + u * nt < nval) io.lucsr[q0 + u * nt] = v[u];
+                for (int u = 0; u < U; ++u)
+                    if (e[u].y != ~0u) buf[e[u].y] = v[u];
+            }
+            __syncthreads();
+            for (uint32_t i = tid; i < len; i += nt) io.lucsr[base + i] = buf[i];
+            __syncthreads();
         }
-        __syncthreads();
     }
     if (tid == 0 && io.norms) io.norms[1] = bnorm;
 }

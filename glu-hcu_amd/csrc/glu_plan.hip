@@ -262,6 +262,7 @@ constexpr int kGridThreads = 256;
 constexpr int kDefaultFusedThreads = 1024;
 constexpr int kDefaultKrylov = 2;         // FGMRES(2) cycles after the first solve
 constexpr int kDefaultFuseUnits = 2048;   // lanes
+constexpr uint32_t kCopyRange = 32768;    // slots (128 KB of LDS) per range of the batch value copy
 
 struct Segment {
     int l0, l1;      // levels [l0, l1)
@@ -323,6 +324,7 @@ struct glu_plan {
     int nwin = 0;                // LDS windows of the factorization
     int win_steps = 0;           // steps they cover
     uint32_t win_nslot = 0;      // the largest window's values
+    uint32_t copy_range = 0;     // slots the batch kernels' value copy stages in LDS at a time
     std::vector<uint32_t> win_slots;   // values of each window
 
     // hipGraph replay of the launch sequences (use_graph): one executable
@@ -1414,6 +1416,23 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
     df_vpos.insert(df_vpos.end(), dup.pvpos.begin(), dup.pvpos.end());
     df_vpos.resize(df_vpos.size() + 64, UINT32_MAX);   // pivot reads of the last positions' lanes
     p->df_nval = (uint32_t)df_vpos.size();
+    // The batch kernels build the copy in LDS, copy_range slots at a time:
+    // per range, {CSC position, slot - range start} of its non-padding
+    // slots in increasing position order (so that the reads sweep the
+    // factors in order), then the range is written out whole.
+    p->copy_range = std::min<uint32_t>(p->df_nval, kCopyRange);
+    std::vector<uint2> df_cpy;
+    std::vector<uint32_t> df_cpy_ptr(1, 0);
+    df_cpy.reserve(p->df_nval);
+    for (uint32_t b = 0; b < p->df_nval; b += p->copy_range) {
+        const uint32_t e = std::min<uint32_t>(p->df_nval, b + p->copy_range);
+        const size_t first = df_cpy.size();
+        for (uint32_t q = b; q < e; ++q)
+            if (df_vpos[q] != UINT32_MAX) df_cpy.push_back(make_uint2(df_vpos[q], q - b));
+        std::sort(df_cpy.begin() + first, df_cpy.end(),
+                  [](const uint2 &x, const uint2 &y) { return x.x < y.x || (x.x == y.x && x.y < y.y); });
+        df_cpy_ptr.push_back((uint32_t)df_cpy.size());
+    }
 
     // Refinement / refresh tables from the analysis.
     std::vector<uint32_t> a_rptr, a_cidx, a_src, lu_src, lu_col, a_rows, a_cols;
@@ -1522,6 +1541,7 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
     const size_t o_dtl_task = U.put(dtl.task), o_dtl_rl = U.put(dtl.rl), o_dtl_idx = U.put(dtl.idx);
     const size_t o_tdfsrc = U.put(tdf_vpos);
     const size_t o_dfsrc = U.put(df_vpos);
+    const size_t o_dfcpy = U.put(df_cpy), o_dfcptr = U.put(df_cpy_ptr);
     size_t o_trows[4], o_tptr[4], o_tlong[4], o_ttask[4];
     for (int s = 0; s < 4; ++s) {
         o_trows[s] = U.put(p->tri[s].rows);
@@ -1661,6 +1681,9 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
     S.tdf_nval = p->tdf_nval;
     S.df_src = u32p(o_dfsrc);
     S.df_nval = p->df_nval;
+    S.df_cpy = reinterpret_cast<const uint2 *>(A + o_dfcpy);
+    S.df_cpy_ptr = u32p(o_dfcptr);
+    S.df_cpy_range = p->copy_range;
     p->tri_dataflow = p->solve_df_opt >= 0 && p->lds_cap >= 4 * (size_t)((n + 31) / 32);
     if (an) {
         S.row_perm = u32p(o_rp);
@@ -2187,7 +2210,7 @@ extern "C" glu_status glu_batch_solve(glu_plan *p, int batch, const double *d_ax
     const int mode = sweep_mode(p->n, cap);
     launch::batch_same(p->d_desc, batch, d_ax, d_b, d_x, d_lu, p->d_blucsr, p->d_bt32, p->d_br64, p->d_bnorms,
                        p->d_bkry, d_iters, d_eta, tol, maxit, (int)(flags & 1u), p->fused_threads, mode,
-                       sizeof(float) * p->win_nslot, launch::batch_lds_bytes(mode, p->n),
+                       sizeof(float) * std::max(p->win_nslot, p->copy_range), launch::batch_lds_bytes(mode, p->n),
                        static_cast<hipStream_t>(stream));
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(GLU_ERR_DEVICE, std::string("batch launch: ") + hipGetErrorString(e));
@@ -2351,7 +2374,7 @@ extern "C" glu_status glu_group_solve(glu_group *g, const double *const *d_ax, c
     size_t nmax = 0, tmax = 0;
     for (glu_plan *p : g->plans) {
         nmax = std::max<size_t>(nmax, p->n);
-        tmax = std::max<size_t>(tmax, sizeof(float) * p->win_nslot);
+        tmax = std::max<size_t>(tmax, sizeof(float) * std::max(p->win_nslot, p->copy_range));
     }
     const size_t cap = launch::enable_lds();
     const int mode = sweep_mode((uint32_t)nmax, cap);
