@@ -215,6 +215,9 @@ def bench_uneven(dev, log, tol, maxit):
         return (time.perf_counter() - t0) / reps * 1e3, res
 
     default_ms, _ = launch_ms(g)
+    g0 = Group(plans, split=0)   # every member in the fused launch
+    fused_only_ms, _ = launch_ms(g0)
+    g0.close()
     # group-wide tiles (workgroup size), refinement restart and LDS windows,
     # autotuned on this group; the tuned group is the one measured
     from pyglu import autotune
@@ -224,7 +227,7 @@ def bench_uneven(dev, log, tol, maxit):
         tuned_plans = list(ex.map(lambda a: Plan(a, block_threads=tune.block_threads, krylov=tune.krylov,
                                                 lds_windows=tune.lds_windows), ans))
     t_tune = time.perf_counter() - t0
-    gt = Group(tuned_plans)
+    gt = Group(tuned_plans, split=None if tune.split < 0 else tune.split)
     ms, (xs, _, iters, eta) = launch_ms(gt)
     it_h, eta_h = iters.cpu().numpy(), eta.cpu().numpy()
     accs = [accuracy(ans[m], d, i, p, mna.rhs(n, 4500 + m), xs[m].cpu().numpy())
@@ -240,16 +243,18 @@ def bench_uneven(dev, log, tol, maxit):
     out = {"count": len(plans), "sum_n": int(sum(p.n for p in plans)), "max_n": int(max(p.n for p in plans)),
            "max_levels": int(max(p.num_lev for p in plans)), "launch_ms": round(ms, 3),
            "refactors_per_s": round(len(plans) / ms * 1e3, 1), "plain_loop_ms": round(loop_ms, 3),
-           "default_params_launch_ms": round(default_ms, 3),
+           "default_params_launch_ms": round(default_ms, 3), "fused_only_launch_ms": round(fused_only_ms, 3),
            "autotuned": {"block_threads": tune.block_threads, "krylov": tune.krylov,
-                         "lds_windows": tune.lds_windows, "candidates": len(tune.table),
+                         "lds_windows": tune.lds_windows, "split": tune.split, "candidates": len(tune.table),
                          "gain_vs_default": round(default_ms / ms, 3), "tune_s": round(t_tune, 2)},
            "refine_iters_max": int(it_h.max()), "eta_max": float(eta_h.max()),
            "accuracy": summarize(accs),
            "correct": bool(np.all(eta_h <= tol)), "host_analysis_s": round(t_an, 2),
            "host_analysis_threads": threads, "plans_s": round(t_plan, 2)}
     log(f"[bench] uneven group: {ms:.3f} ms per launch autotuned (threads {tune.block_threads}, krylov "
-        f"{tune.krylov}, windows {tune.lds_windows}; defaults {default_ms:.3f} ms; plain loop {loop_ms:.1f} ms), "
+        f"{tune.krylov}, windows {tune.lds_windows}, split {tune.split}; defaults {default_ms:.3f} ms, "
+        f"all members in the fused launch {fused_only_ms:.3f} ms; "
+        f"plain loop {loop_ms:.1f} ms), "
         f"iters max {it_h.max()}, eta max {eta_h.max():.2e}, forward error max {out['accuracy']['fwd_max']:.1e}")
     g.close()
     gt.close()

@@ -263,6 +263,7 @@ constexpr int kDefaultFusedThreads = 1024;
 constexpr int kDefaultKrylov = 2;         // FGMRES(2) cycles after the first solve
 constexpr int kDefaultFuseUnits = 2048;   // lanes
 constexpr uint32_t kCopyRange = 32768;    // slots (128 KB of LDS) per range of the batch value copy
+constexpr int kMaxGroupSplit = 2;         // group members on the single-matrix path, at most
 
 struct Segment {
     int l0, l1;      // levels [l0, l1)
@@ -1420,7 +1421,8 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
     // per range, {CSC position, slot - range start} of its non-padding
     // slots in increasing position order (so that the reads sweep the
     // factors in order), then the range is written out whole.
-    p->copy_range = std::min<uint32_t>(p->df_nval, kCopyRange);
+    p->copy_range = std::min<uint32_t>({p->df_nval, kCopyRange, (uint32_t)(p->lds_cap / sizeof(float)) & ~63u});
+    if (p->copy_range == 0) return fail(GLU_ERR_DEVICE, "no LDS for the batch value copy");
     std::vector<uint2> df_cpy;
     std::vector<uint32_t> df_cpy_ptr(1, 0);
     df_cpy.reserve(p->df_nval);
@@ -2291,9 +2293,18 @@ struct glu_group {
     std::vector<size_t> voff;   // per-matrix offset into the scratch vectors
     std::vector<size_t> koff;   // per-matrix offset into d_kry (~0: plain refinement)
     std::vector<size_t> foff;   // per-matrix offset into d_lucsr
+    std::vector<uint32_t> order;   // members, heaviest first (d_order)
     int threads = kDefaultFusedThreads;
+    int split_req = -1;            // heaviest members on the single-matrix path (-1: automatic)
+    hipStream_t aux[kMaxGroupSplit] = {};
+    hipEvent_t ev_in = nullptr, ev_aux[kMaxGroupSplit] = {};
     ~glu_group()
     {
+        for (int j = 0; j < kMaxGroupSplit; ++j) {
+            if (aux[j]) (void)hipStreamDestroy(aux[j]);
+            if (ev_aux[j]) (void)hipEventDestroy(ev_aux[j]);
+        }
+        if (ev_in) (void)hipEventDestroy(ev_in);
         if (d_lucsr) (void)hipFree(d_lucsr);
         if (d_norms) (void)hipFree(d_norms);
         if (d_kry) (void)hipFree(d_kry);
@@ -2337,7 +2348,8 @@ extern "C" glu_status glu_group_create(glu_plan *const *plans, int count, glu_gr
                    (uint32_t)i};
     }
     std::sort(work.begin(), work.end());
-    std::vector<uint32_t> order(count);
+    std::vector<uint32_t> &order = g->order;
+    order.resize(count);
     for (int i = 0; i < count; ++i) order[i] = work[i].second;
     HIP_TRY(hipMalloc(&g->d_descs, sizeof(MatDesc) * count));
     HIP_TRY(hipMalloc(&g->d_ios, sizeof(MatIOHost) * count));
@@ -2355,6 +2367,13 @@ extern "C" glu_status glu_group_create(glu_plan *const *plans, int count, glu_gr
 
 extern "C" void glu_group_destroy(glu_group *g) { delete g; }
 
+extern "C" glu_status glu_group_set_split(glu_group *g, int count)
+{
+    if (!g || count < -1 || count > kMaxGroupSplit) return fail(GLU_ERR_ARGUMENT, "split count must be -1..2");
+    g->split_req = count;
+    return GLU_OK;
+}
+
 extern "C" glu_status glu_group_solve(glu_group *g, const double *const *d_ax, const double *const *d_b,
                                       double *const *d_x, float *const *d_lu, double tol, int maxit,
                                       int32_t *d_iters, double *d_eta, unsigned flags, void *stream)
@@ -2371,25 +2390,62 @@ extern "C" glu_status glu_group_solve(glu_group *g, const double *const *d_ax, c
     }
     hipStream_t st = static_cast<hipStream_t>(stream);
     HIP_TRY(hipMemcpyAsync(g->d_ios, ios.data(), sizeof(MatIOHost) * count, hipMemcpyHostToDevice, st));
+    const size_t cap = launch::enable_lds();
+    const int perturb = (int)(flags & 1u);
+    // The heaviest members (as many as asked; automatic: one in a group of
+    // four or more) run on the single-matrix path on streams of their own,
+    // alongside the fused launch of the rest: the grid factorization on the
+    // CUs the fused launch leaves idle, then the one-workgroup refinement,
+    // the same kernel the fused launch runs (same workgroup size and restart
+    // length), so the results are the same bits.  Only members whose
+    // refinement keeps its vector in LDS qualify.
+    int want = g->split_req >= 0 ? g->split_req : (count >= 4 ? 1 : 0);
+    want = std::min(want, count - 1);
+    int nsplit = 0;
+    while (nsplit < want && sweep_mode(g->plans[g->order[nsplit]]->n, cap) == 2) ++nsplit;
+    if (nsplit > 0) {
+        if (!g->ev_in) HIP_TRY(hipEventCreateWithFlags(&g->ev_in, hipEventDisableTiming));
+        HIP_TRY(hipEventRecord(g->ev_in, st));
+    }
+    for (int j = 0; j < nsplit; ++j) {
+        const uint32_t m = g->order[j];
+        glu_plan *p = g->plans[m];
+        glu_status s = ensure_batch(p, 1);
+        if (s != GLU_OK) return s;
+        if (!g->aux[j]) HIP_TRY(hipStreamCreateWithFlags(&g->aux[j], hipStreamNonBlocking));
+        if (!g->ev_aux[j]) HIP_TRY(hipEventCreateWithFlags(&g->ev_aux[j], hipEventDisableTiming));
+        hipStream_t as = g->aux[j];
+        HIP_TRY(hipStreamWaitEvent(as, g->ev_in, 0));
+        launch::refresh(p->R, d_ax[m], d_lu[m], p->nnz, 1, as);
+        s = run_factor(p, d_lu[m], perturb, as);
+        if (s != GLU_OK) return s;
+        launch::refine_one(p->d_desc, p->df_nval, d_ax[m], d_b[m], d_x[m], d_lu[m], p->d_blucsr, p->d_bt32,
+                           p->d_br64, p->d_bnorms, p->d_bkry, d_iters + m, d_eta + m, tol, maxit, g->threads,
+                           launch::batch_lds_bytes(2, p->n), 0, as);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventRecord(g->ev_aux[j], as));
+    }
     size_t nmax = 0, tmax = 0;
-    for (glu_plan *p : g->plans) {
+    for (int i = nsplit; i < count; ++i) {
+        const glu_plan *p = g->plans[g->order[i]];
         nmax = std::max<size_t>(nmax, p->n);
         tmax = std::max<size_t>(tmax, sizeof(float) * std::max(p->win_nslot, p->copy_range));
     }
-    const size_t cap = launch::enable_lds();
     const int mode = sweep_mode((uint32_t)nmax, cap);
     // ready bits only for the largest order: each matrix whose vector fits
     // keeps it in LDS too (the launch's LDS: the largest such need)
     size_t ref_lds = launch::batch_lds_bytes(mode, (uint32_t)nmax);
     if (mode == 1)
-        for (glu_plan *p : g->plans) {
+        for (int i = nsplit; i < count; ++i) {
+            const glu_plan *p = g->plans[g->order[i]];
             const size_t full = launch::batch_lds_bytes(2, p->n);
             ref_lds = std::max(ref_lds, full <= cap ? full : launch::batch_lds_bytes(1, p->n));
         }
-    launch::batch_group(g->d_descs, g->d_ios, g->d_order, count, tol, maxit, (int)(flags & 1u), g->threads, mode, tmax,
-                        ref_lds, st);
+    launch::batch_group(g->d_descs, g->d_ios, g->d_order + nsplit, count - nsplit, tol, maxit, perturb, g->threads,
+                        mode, tmax, ref_lds, st);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(GLU_ERR_DEVICE, std::string("group launch: ") + hipGetErrorString(e));
+    for (int j = 0; j < nsplit; ++j) HIP_TRY(hipStreamWaitEvent(st, g->ev_aux[j], 0));
     // ios is read by the kernel from device memory; the host copy may go.
     HIP_TRY(hipStreamSynchronize(st));
     return GLU_OK;

@@ -31,7 +31,7 @@ EXPORTED = [
     "glu_plan_create", "glu_plan_create_from_analysis", "glu_plan_destroy", "glu_plan_info",
     "glu_factor", "glu_factor_host", "glu_solve", "glu_refresh", "glu_refine", "glu_adjoint_values",
     "glu_batch_solve", "glu_batch_factor", "glu_abft_sums", "glu_abft_check",
-    "glu_group_create", "glu_group_destroy", "glu_group_solve", "glu_refine_noalloc",
+    "glu_group_create", "glu_group_destroy", "glu_group_set_split", "glu_group_solve", "glu_refine_noalloc",
 ]
 
 
@@ -94,6 +94,7 @@ def lib():
         "glu_abft_check": (st, [P, I, P, P, C.c_float, P, P, P]),
         "glu_group_create": (st, [P, I, C.POINTER(P)]),
         "glu_group_destroy": (None, [P]),
+        "glu_group_set_split": (st, [P, I]),
         "glu_group_solve": (st, [P, P, P, P, P, D, I, P, P, C.c_uint, P]),
     }
     for name, (res, args) in sig.items():

@@ -35,6 +35,9 @@ THREAD_CANDIDATES = (256, 512, 1024)
 
 WINDOW_CANDIDATES = (0, -1, 1)
 KRYLOV_CANDIDATES = (2, -1, 1, 4)
+# uneven groups: members on the single-matrix path, against the automatic
+# choice the other stages run with (one in a group of four or more)
+SPLIT_CANDIDATES = (0, 2)
 
 
 @dataclass
@@ -45,6 +48,7 @@ class TuneResult:
     table: list = field(default_factory=list)
     lds_windows: int = 0
     krylov: int = 0
+    split: int = -1   # uneven groups: members on the single-matrix path (-1: automatic)
 
     def plan(self, analysis):
         """A plan with the chosen parameters."""
@@ -128,23 +132,25 @@ def tune_batch(analysis, ax, b, tol=1e-10, maxit=10, threads=THREAD_CANDIDATES, 
 
 
 def tune_group(analyses, axs, bs, tol=1e-10, maxit=10, threads=THREAD_CANDIDATES, krylov=KRYLOV_CANDIDATES,
-               windows=WINDOW_CANDIDATES, reps=3, plan_pool=None):
-    """Best group-wide (block_threads, krylov, lds_windows) for one fused
-    launch over distinct patterns (glu_group_solve): the launch's workgroup
-    size is the tile every member runs on, and each candidate rebuilds all
-    member plans.  Thread counts at the defaults first, then restart
-    lengths, then window counts, each at the best so far.  ``plan_pool`` (an
+               windows=WINDOW_CANDIDATES, splits=SPLIT_CANDIDATES, reps=3, plan_pool=None):
+    """Best group-wide (block_threads, krylov, lds_windows, split) for one
+    fused launch over distinct patterns (glu_group_solve): the launch's
+    workgroup size is the tile every member runs on, and each candidate
+    rebuilds all member plans.  Thread counts at the defaults first, then
+    restart lengths, then window counts, each at the best so far, then how
+    many of the heaviest members run on the single-matrix path beside the
+    launch (results are the same bits for every split).  ``plan_pool`` (an
     executor) builds the member plans in parallel."""
     from .device import Group
     table = []
     best = None
 
-    def measure(t, m, wnd):
+    def measure(t, m, wnd, split=None):
         mk = lambda an: Plan(an, block_threads=t, krylov=m, lds_windows=wnd)
         plans = list(plan_pool.map(mk, analyses)) if plan_pool is not None else [mk(a) for a in analyses]
-        g = Group(plans)
+        g = Group(plans, split=split)
         ms = _time(lambda: g.solve(axs, bs, tol=tol, maxit=maxit), reps, warmup=1)
-        table.append((0, t, 0, ms, (m, wnd)))
+        table.append((0, t, 0, ms, (m, wnd, -1 if split is None else split)))
         g.close()
         for p in plans:
             p.close()
@@ -164,7 +170,13 @@ def tune_group(analyses, axs, bs, tol=1e-10, maxit=10, threads=THREAD_CANDIDATES
         ms = measure(best[1], best[3], wnd)
         if ms < best[2]:
             best = (0, best[1], ms, best[3], wnd)
-    return TuneResult(best[0], best[1], best[2], table, lds_windows=best[4], krylov=best[3])
+    split = -1
+    for sp in splits:
+        ms = measure(best[1], best[3], best[4], sp)
+        if ms < best[2]:
+            best = (0, best[1], ms, best[3], best[4])
+            split = sp
+    return TuneResult(best[0], best[1], best[2], table, lds_windows=best[4], krylov=best[3], split=split)
 
 
 def report(res: TuneResult) -> str:

@@ -228,12 +228,17 @@ class Plan:
 class Group:
     """Distinct patterns refactored + refined together in one fused launch."""
 
-    def __init__(self, plans):
+    def __init__(self, plans, split=None):
+        """split: how many of the heaviest members run on the single-matrix
+        path beside the fused launch (0..2; None: the library's automatic
+        choice, one in a group of four or more).  Same results either way."""
         self.plans = list(plans)
         arr = (C.c_void_p * len(self.plans))(*[p.handle.value for p in self.plans])
         h = C.c_void_p()
         capi.check(capi.lib().glu_group_create(arr, len(self.plans), C.byref(h)), "glu_group_create")
         self._h = h
+        if split is not None:
+            capi.check(capi.lib().glu_group_set_split(h, int(split)), "glu_group_set_split")
 
     def solve(self, axs, bs, tol=1e-10, maxit=10, perturb=False):
         cnt = len(self.plans)

@@ -255,6 +255,13 @@ typedef struct glu_group glu_group;
 
 glu_status glu_group_create(glu_plan *const *plans, int count, glu_group **out);
 void glu_group_destroy(glu_group *g);
+/* Members run through the single-matrix path alongside the fused launch:
+ * the `count` heaviest (0..2; -1, the default: one in a group of four or
+ * more), on streams of their own, while the fused launch takes the rest.
+ * Results are the same bits either way.  Only members whose refinement
+ * vector fits in LDS are moved.  (No counterpart in the reference: part of
+ * the uneven batch this library adds.) */
+glu_status glu_group_set_split(glu_group *g, int count);
 /* Host arrays of `count` device pointers each, in plan order. */
 glu_status glu_group_solve(glu_group *g, const double *const *d_ax, const double *const *d_b,
                            double *const *d_x, float *const *d_lu, double tol, int maxit,

@@ -204,6 +204,44 @@ def test_group_matches_single(torch_mod):
         assert float(eta[k]) <= 1e-10 and e <= 2e-10
 
 
+def test_group_split_same_bits(torch_mod):
+    """The heaviest members of an uneven group moved to the single-matrix
+    path beside the fused launch (glu_group_set_split 1 and 2) give the same
+    factors, solutions, iteration counts and eta, bit for bit, as the fused
+    launch alone (split 0), on two solves in a row."""
+    from pyglu import mna
+    from pyglu.analysis import Analysis
+    from pyglu.device import Group, Plan
+    torch = torch_mod
+    plans, axs, bs = [], [], []
+    for spec, seed in mna.uneven_specs(6, seed0=4200):
+        spec = mna.MNASpec(n_nodes=min(spec.n_nodes, 4000), n_vsrc=min(spec.n_vsrc, 20),
+                           extra_hi=spec.extra_hi, n_rails=spec.n_rails, rail_deg=min(spec.rail_deg, 120))
+        d, i, p, n = mna.mna_csc(spec, seed)
+        plans.append(Plan(Analysis(d, i, p, n)))
+        axs.append(torch.from_numpy(mna.perturb_values(d, seed + 3)).cuda())
+        bs.append(torch.from_numpy(mna.rhs(n, seed + 1)).cuda())
+    runs = {}
+    for split in (0, 1, 2):
+        g = Group(plans, split=split)
+        for rep in range(2):
+            xs, lus, iters, eta = g.solve(axs, bs, tol=1e-10, maxit=20)
+            torch.cuda.synchronize()
+            runs[(split, rep)] = ([x.cpu().numpy() for x in xs], [lu.cpu().numpy() for lu in lus],
+                                  iters.cpu().numpy(), eta.cpu().numpy())
+        g.close()
+    want = runs[(0, 0)]
+    assert np.all(want[3] <= 1e-10)
+    for key, got in runs.items():
+        print(f"  split {key[0]} solve {key[1]}: iters {got[2].tolist()}", flush=True)
+        for k in range(len(plans)):
+            assert np.array_equal(got[0][k].view(np.uint64), want[0][k].view(np.uint64)), (key, k)
+            assert np.array_equal(got[1][k].view(np.uint32), want[1][k].view(np.uint32)), (key, k)
+        assert np.array_equal(got[2], want[2]) and np.array_equal(got[3].view(np.uint64), want[3].view(np.uint64)), key
+    with pytest.raises(Exception):
+        Group(plans, split=3)
+
+
 def test_autograd_against_dense_fp64(torch_mod):
     """dL/dA (on A's pattern) and dL/db of L = |x - x*|^2 against the dense
     FP64 reference gradient and central finite differences."""
