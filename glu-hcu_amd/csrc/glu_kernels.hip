@@ -1238,6 +1238,32 @@ __device__ __forceinline__ float group_sum(float v, uint32_t gw)
     return v;
 }
 
+// The sweep vector as the dataflow sweeps see it: get(c), a finished row's
+// value; in(r), row r's right-hand side; put(r, v), row r's result.
+// XWhole: the whole vector in one place (LDS or global memory).
+struct XWhole {
+    float *x;
+    __device__ __forceinline__ float get(uint32_t c) const { return x[c]; }
+    __device__ __forceinline__ float in(uint32_t r) const { return x[r]; }
+    __device__ __forceinline__ void put(uint32_t r, float v) const { x[r] = v; }
+};
+
+// XSplit: the vector in global memory, except that rows r >= r0 put their
+// results into LDS (xl[r - r0]) and are read from there: the LDS the ready
+// bits leave free holds the last rows' values.  The caller copies xl back
+// once the sweep is done.
+struct XSplit {
+    float *x, *xl;
+    uint32_t r0;
+    __device__ __forceinline__ float get(uint32_t c) const { return c >= r0 ? xl[c - r0] : x[c]; }
+    __device__ __forceinline__ float in(uint32_t r) const { return x[r]; }
+    __device__ __forceinline__ void put(uint32_t r, float v) const
+    {
+        if (r >= r0) xl[r - r0] = v;
+        else x[r] = v;
+    }
+};
+
 struct DfTask {
     uint32_t base, p0, cnt, m, lg;   // lg: log2 lanes per row of a wave task
     bool wave;
@@ -1290,11 +1316,12 @@ __device__ __forceinline__ void df2_issue(const DfDev &D, const float *__restric
 // row kDfK entries at a time (the slots of entry e at base + 64 e + lane),
 // consuming ready entries in the row's order as the short-row path does.
 // Polls are bounded (kSpinCap); a row still pending then completes with NaN.
-__device__ __forceinline__ void df2_lane_long(const DfDev &D, const float *__restrict__ sv, float *x,
+template <typename XV>
+__device__ __forceinline__ void df2_lane_long(const DfDev &D, const float *__restrict__ sv, const XV &x,
                                               uint32_t *flags, const DfTask &t, uint32_t lane)
 {
     bool pending = lane < t.cnt;
-    float acc = pending ? x[t.row] : 0.f;
+    float acc = pending ? x.in(t.row) : 0.f;
     // the row's entries kDfK at a time: [c0, c0 + kDfK) in idx / val (the
     // first chunk read again from the slots the task's prefetch brought in:
     // taking it from t would push the short-row path's arrays to scratch)
@@ -1347,7 +1374,7 @@ __device__ __forceinline__ void df2_lane_long(const DfDev &D, const float *__res
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
             float xv[kDfK];
 #pragma unroll
-            for (uint32_t e = 0; e < kDfK; ++e) xv[e] = x[idx[e]];
+            for (uint32_t e = 0; e < kDfK; ++e) xv[e] = x.get(idx[e]);
 #pragma unroll
             for (uint32_t e = 0; e < kDfK; ++e)
                 if (c0 + e == ci && ((mask >> e) & 1u)) {
@@ -1356,14 +1383,14 @@ __device__ __forceinline__ void df2_lane_long(const DfDev &D, const float *__res
                 }
             if (ci == t.len) {
                 if (D.div) acc = acc / t.dval;
-                x[t.row] = acc;
+                x.put(t.row, acc);
                 df_set(flags, t.row);
                 pending = false;
             }
         }
     }
     if (pending) {   // gave up: the row completes with NaN
-        x[t.row] = df_nan();
+        x.put(t.row, df_nan());
         df_set(flags, t.row);
     }
 }
@@ -1374,8 +1401,8 @@ __device__ __forceinline__ void df2_lane_long(const DfDev &D, const float *__res
 // kExact: every row's products subtracted one at a time in the row's order
 // (group rows through the wave's LDS slice `lds`, by the group's first
 // lane) — the arithmetic of Symbolic_Matrix::solve, for glu_solve.
-template <bool kExact = false>
-__device__ __forceinline__ void df2_run(const DfDev &D, const float *__restrict__ sv, float *x, uint32_t *flags,
+template <bool kExact = false, typename XV = XWhole>
+__device__ __forceinline__ void df2_run(const DfDev &D, const float *__restrict__ sv, const XV &x, uint32_t *flags,
                                         const DfTask &t, uint32_t lane, float *lds = nullptr)
 {
     if (!t.wave) {
@@ -1384,7 +1411,7 @@ __device__ __forceinline__ void df2_run(const DfDev &D, const float *__restrict_
             return;
         }
         bool pending = lane < t.cnt;
-        float acc = pending ? x[t.row] : 0.f;
+        float acc = pending ? x.in(t.row) : 0.f;
         uint32_t ci = 0;
         while (__ballot(pending)) {   // unbounded: the plan's tables are checked on the host
             if (pending) {
@@ -1402,7 +1429,7 @@ __device__ __forceinline__ void df2_run(const DfDev &D, const float *__restrict_
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
                 float xv[kDfK];
 #pragma unroll
-                for (uint32_t e = 0; e < kDfK; ++e) xv[e] = x[t.idx[e]];
+                for (uint32_t e = 0; e < kDfK; ++e) xv[e] = x.get(t.idx[e]);
 #pragma unroll
                 for (uint32_t e = 0; e < kDfK; ++e)
                     if (e == ci && ((mask >> e) & 1u)) {
@@ -1411,7 +1438,7 @@ __device__ __forceinline__ void df2_run(const DfDev &D, const float *__restrict_
                     }
                 if (ci == t.len) {
                     if (D.div) acc = acc / t.dval;
-                    x[t.row] = acc;
+                    x.put(t.row, acc);
                     df_set(flags, t.row);
                     pending = false;
                 }
@@ -1431,7 +1458,7 @@ __device__ __forceinline__ void df2_run(const DfDev &D, const float *__restrict_
         // the row's end subtract +0, which changes no value.  The group's
         // last lane ends with acc - p[0] - p[1] - ... - p[len - 1].
         const bool on = (lane >> t.lg) < t.cnt;
-        float carry = on ? x[t.row] : 0.f;   // the running value entering a pass, in lane 0 of the group
+        float carry = on ? x.in(t.row) : 0.f;   // the running value entering a pass, in lane 0 of the group
         float a = 0.f;
         for (uint32_t c0 = 0; c0 < t.len || c0 == 0; c0 += kWave) {   // a pass of gw (64: of 64) entries
             uint32_t col = t.idx[0];
@@ -1443,7 +1470,7 @@ __device__ __forceinline__ void df2_run(const DfDev &D, const float *__restrict_
             float pr = 0.f;
             if (c0 + j < t.len) {
                 while (!df_ready(flags, col)) {}
-                pr = v * x[col];
+                pr = v * x.get(col);
             }
             a = j == 0 ? carry - pr : pr;
             for (uint32_t r = 0; r < gw / 16u; ++r) {
@@ -1461,7 +1488,7 @@ __device__ __forceinline__ void df2_run(const DfDev &D, const float *__restrict_
         }
         if (on && j == gw - 1u) {
             if (D.div) a = a / t.dval;
-            x[t.row] = a;
+            x.put(t.row, a);
             df_set(flags, t.row);
         }
         return;
@@ -1469,7 +1496,7 @@ __device__ __forceinline__ void df2_run(const DfDev &D, const float *__restrict_
     float sum = 0.f;
     if (j < t.len) {
         while (!df_ready(flags, t.idx[0])) {}
-        sum = t.val[0] * x[t.idx[0]];
+        sum = t.val[0] * x.get(t.idx[0]);
     }
     for (uint32_t c0 = kWave; c0 < t.len; c0 += 4 * kWave) {   // gw == 64 here
         uint32_t col[4];
@@ -1484,14 +1511,14 @@ __device__ __forceinline__ void df2_run(const DfDev &D, const float *__restrict_
         for (uint32_t u = 0; u < 4; ++u)
             if (c0 + u * kWave + j < t.len) {
                 while (!df_ready(flags, col[u])) {}
-                sum += v[u] * x[col[u]];
+                sum += v[u] * x.get(col[u]);
             }
     }
     sum = group_sum(sum, gw);
     if (j == 0 && (lane >> t.lg) < t.cnt) {
-        float acc = x[t.row] - sum;
+        float acc = x.in(t.row) - sum;
         if (D.div) acc = acc / t.dval;
-        x[t.row] = acc;
+        x.put(t.row, acc);
         df_set(flags, t.row);
     }
 }
@@ -1502,8 +1529,8 @@ __device__ __forceinline__ void df2_run(const DfDev &D, const float *__restrict_
 // scalar/LDS wait counter, so waiting for one never waits for the vector
 // loads in flight), fetched two tasks ahead; a task's rows, columns and
 // values are vector loads issued one task ahead.
-template <bool kExact = false>
-__device__ __forceinline__ void df2_sweep(const DfDev &D, const float *__restrict__ sv, float *x, uint32_t *flags,
+template <bool kExact = false, typename XV = XWhole>
+__device__ __forceinline__ void df2_sweep(const DfDev &D, const float *__restrict__ sv, const XV &x, uint32_t *flags,
                                           float *lds = nullptr)
 {
     const uint32_t lane = threadIdx.x & (kWave - 1), nw = blockDim.x / kWave;
@@ -1528,7 +1555,7 @@ __device__ __forceinline__ void df2_sweep(const DfDev &D, const float *__restric
         dn = dn2;
         dn2 = desc(j + 2);
         df2_issue(D, sv, dn, lane, nxt);
-        df2_run<kExact>(D, sv, x, flags, cur, lane, lds);
+        df2_run<kExact, XV>(D, sv, x, flags, cur, lane, lds);
     }
 }
 
@@ -1695,7 +1722,7 @@ __device__ __forceinline__ void wg_precond(const MatDesc *Md, const MatIO &io, c
         {
             DfDev lo = kT ? Md->s.tup_df : Md->s.lo_df;
             globalize(lo);
-            df2_sweep(lo, io.lucsr + lo.voff, xs, flags);
+            df2_sweep(lo, io.lucsr + lo.voff, XWhole{xs}, flags);
         }
         __syncthreads();
         for (uint32_t i = tid; i < nwords; i += nt) flags[i] = 0u;
@@ -1703,7 +1730,7 @@ __device__ __forceinline__ void wg_precond(const MatDesc *Md, const MatIO &io, c
         {
             DfDev up = kT ? Md->s.tlo_df : Md->s.up_df;
             globalize(up);
-            df2_sweep(up, io.lucsr + up.voff, xs, flags);
+            df2_sweep(up, io.lucsr + up.voff, XWhole{xs}, flags);
         }
         __syncthreads();
     }
@@ -2052,27 +2079,25 @@ __global__ void __launch_bounds__(1024) k_refine_one(const MatDesc *__restrict__
 // correction vector t32 in global memory that grid kernels scaled before
 // and read back after (glu_refine for orders whose vector does not fit in
 // LDS: the FP64 vector work and the products with A run across the grid).
-// kT: A' x = b (U' then L').  Dynamic LDS: the ready bits, ceil(n/32) words.
+// kT: A' x = b (U' then L').  Dynamic LDS: the ready bits, ceil(n/32) words,
+// then the results of rows r0..n-1 (XSplit), copied to t32 after each sweep.
 template <bool kT>
 __global__ void __launch_bounds__(1024) k_sweeps_df(const MatDesc *__restrict__ desc,
-                                                    const float *__restrict__ lucsr, float *__restrict__ t32)
+                                                    const float *__restrict__ lucsr, float *__restrict__ t32,
+                                                    uint32_t r0)
 {
+    const uint32_t n = desc->n, nwords = (n + 31) / 32, tid = threadIdx.x, nt = blockDim.x;
     uint32_t *flags = reinterpret_cast<uint32_t *>(g_lds_vec);
-    const uint32_t nwords = (desc->n + 31) / 32;
-    for (uint32_t i = threadIdx.x; i < nwords; i += blockDim.x) flags[i] = 0u;
-    __syncthreads();
-    {
-        DfDev lo = kT ? desc->s.tup_df : desc->s.lo_df;
-        globalize(lo);
-        df2_sweep(lo, lucsr + lo.voff, t32, flags);
-    }
-    __syncthreads();
-    for (uint32_t i = threadIdx.x; i < nwords; i += blockDim.x) flags[i] = 0u;
-    __syncthreads();
-    {
-        DfDev up = kT ? desc->s.tlo_df : desc->s.up_df;
-        globalize(up);
-        df2_sweep(up, lucsr + up.voff, t32, flags);
+    const XSplit xv{t32, g_lds_vec + nwords, r0};
+    for (int sweep = 0; sweep < 2; ++sweep) {
+        for (uint32_t i = tid; i < nwords; i += nt) flags[i] = 0u;
+        __syncthreads();
+        DfDev d = sweep == 0 ? (kT ? desc->s.tup_df : desc->s.lo_df) : (kT ? desc->s.tlo_df : desc->s.up_df);
+        globalize(d);
+        df2_sweep(d, lucsr + d.voff, xv, flags);
+        __syncthreads();
+        for (uint32_t r = r0 + tid; r < n; r += nt) t32[r] = xv.xl[r - r0];   // the LDS rows back
+        __syncthreads();
     }
 }
 
@@ -2102,7 +2127,7 @@ __global__ void __launch_bounds__(1024) k_solve_one(const MatDesc *__restrict__ 
         __syncthreads();
         DfDev d = sweep == 0 ? desc->s.lo_df : desc->s.up_df;
         globalize(d);
-        df2_sweep<true>(d, glb(lucsr) + d.voff, xs, flags, wl[tid / kWave]);
+        df2_sweep<true>(d, glb(lucsr) + d.voff, XWhole{xs}, flags, wl[tid / kWave]);
         __syncthreads();
     }
     const uint32_t *rpi = glb(desc->s.row_perm_inv);
@@ -2364,11 +2389,11 @@ void copy_df(const MatDesc *desc, uint32_t nval, const float *lu, float *lucsr, 
 }
 
 // Both sweeps of the preconditioner in one workgroup over t32 (global).
-void sweeps_df(const MatDesc *desc, const float *lucsr, float *t32, int threads, size_t lds, int transpose,
-               hipStream_t st)
+void sweeps_df(const MatDesc *desc, const float *lucsr, float *t32, uint32_t r0, int threads, size_t lds,
+               int transpose, hipStream_t st)
 {
-    if (transpose) k_sweeps_df<true><<<1, threads, lds, st>>>(desc, lucsr, t32);
-    else k_sweeps_df<false><<<1, threads, lds, st>>>(desc, lucsr, t32);
+    if (transpose) k_sweeps_df<true><<<1, threads, lds, st>>>(desc, lucsr, t32, r0);
+    else k_sweeps_df<false><<<1, threads, lds, st>>>(desc, lucsr, t32, r0);
 }
 
 // glu_solve in one workgroup: the factors into the sweep layout, then the

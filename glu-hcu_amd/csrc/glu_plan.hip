@@ -72,8 +72,8 @@ void refine_one(const MatDesc *desc, uint32_t nval, const double *ax, const doub
                 const float *lu, float *lucsr, float *t32, double *r64, double *norms, double *kry, int32_t *iters,
                 double *eta, double tol, int maxit, int threads, size_t ref_lds, int transpose, hipStream_t st);
 void copy_df(const MatDesc *desc, uint32_t nval, const float *lu, float *lucsr, int transpose, hipStream_t st);
-void sweeps_df(const MatDesc *desc, const float *lucsr, float *t32, int threads, size_t lds, int transpose,
-               hipStream_t st);
+void sweeps_df(const MatDesc *desc, const float *lucsr, float *t32, uint32_t r0, int threads, size_t lds,
+               int transpose, hipStream_t st);
 void adjoint_values(const RefineDev &R, const double *lam, const double *x, double *grad, uint32_t n,
                     hipStream_t st);
 size_t matio_size();
@@ -1999,6 +1999,10 @@ glu_status refine_host_loop(glu_plan *p, const double *d_ax, const float *d_lu, 
 {
     const SolveDev &S = p->S;
     const uint32_t n = p->n;
+    // the sweeps keep the results of the last rows in the LDS their ready
+    // bits leave free (rows r0..n-1)
+    const size_t bits = launch::batch_lds_bytes(1, n), cap = launch::enable_lds();
+    const uint32_t r0 = n - (uint32_t)std::min<size_t>(n, cap > bits ? (cap - bits) / sizeof(float) : 0);
     HIP_TRY(hipMemsetAsync(d_x, 0, sizeof(double) * n, st));
     HIP_TRY(hipMemcpyAsync(r64, d_b, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
     double anorm = 0.0, bnorm = 0.0, eta = 0.0;
@@ -2009,7 +2013,8 @@ glu_status refine_host_loop(glu_plan *p, const double *d_ax, const float *d_lu, 
         else
             launch::perm_scale_f64(r64, t32, S.row_perm, S.row_scale, 1, n, st);
         if (lucsr)
-            launch::sweeps_df(p->d_desc, lucsr, t32, p->fused_threads, launch::batch_lds_bytes(1, n), transpose, st);
+            launch::sweeps_df(p->d_desc, lucsr, t32, r0, p->fused_threads, bits + sizeof(float) * (n - r0), transpose,
+                              st);
         else
             run_sweeps(p, d_lu, t32, transpose, st);
         if (!transpose)
