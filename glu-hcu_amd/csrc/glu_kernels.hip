@@ -2081,22 +2081,26 @@ __global__ void __launch_bounds__(1024) k_refine_one(const MatDesc *__restrict__
 // LDS: the FP64 vector work and the products with A run across the grid).
 // kT: A' x = b (U' then L').  Dynamic LDS: the ready bits, ceil(n/32) words,
 // then the results of rows r0..n-1 (XSplit), copied to t32 after each sweep.
-template <bool kT>
+template <bool kT, bool kSplit>
 __global__ void __launch_bounds__(1024) k_sweeps_df(const MatDesc *__restrict__ desc,
                                                     const float *__restrict__ lucsr, float *__restrict__ t32,
                                                     uint32_t r0)
 {
     const uint32_t n = desc->n, nwords = (n + 31) / 32, tid = threadIdx.x, nt = blockDim.x;
     uint32_t *flags = reinterpret_cast<uint32_t *>(g_lds_vec);
-    const XSplit xv{t32, g_lds_vec + nwords, r0};
     for (int sweep = 0; sweep < 2; ++sweep) {
         for (uint32_t i = tid; i < nwords; i += nt) flags[i] = 0u;
         __syncthreads();
         DfDev d = sweep == 0 ? (kT ? desc->s.tup_df : desc->s.lo_df) : (kT ? desc->s.tlo_df : desc->s.up_df);
         globalize(d);
-        df2_sweep(d, lucsr + d.voff, xv, flags);
-        __syncthreads();
-        for (uint32_t r = r0 + tid; r < n; r += nt) t32[r] = xv.xl[r - r0];   // the LDS rows back
+        if (kSplit) {
+            const XSplit xv{t32, g_lds_vec + nwords, r0};
+            df2_sweep(d, lucsr + d.voff, xv, flags);
+            __syncthreads();
+            for (uint32_t r = r0 + tid; r < n; r += nt) t32[r] = xv.xl[r - r0];   // the LDS rows back
+        } else {
+            df2_sweep(d, lucsr + d.voff, XWhole{t32}, flags);
+        }
         __syncthreads();
     }
 }
@@ -2106,10 +2110,10 @@ __global__ void __launch_bounds__(1024) k_sweeps_df(const MatDesc *__restrict__ 
 // copy of the factors, x = float(xs[row_perm_inv] * row_scale) — the
 // arithmetic of Symbolic_Matrix::solve (src/symbolic.cc:215-247).  Dynamic
 // LDS as k_batch_same_refine.
-template <int kMode>
+template <int kMode, bool kSplit = false>
 __global__ void __launch_bounds__(1024) k_solve_one(const MatDesc *__restrict__ desc,
                                                     const float *__restrict__ lucsr, const float *__restrict__ b,
-                                                    float *__restrict__ x, float *__restrict__ t32)
+                                                    float *__restrict__ x, float *__restrict__ t32, uint32_t r0)
 {
     __shared__ __attribute__((aligned(16))) float wl[1024 / kWave][kWave];
     const uint32_t n = desc->n, tid = threadIdx.x, nt = blockDim.x;
@@ -2127,7 +2131,14 @@ __global__ void __launch_bounds__(1024) k_solve_one(const MatDesc *__restrict__ 
         __syncthreads();
         DfDev d = sweep == 0 ? desc->s.lo_df : desc->s.up_df;
         globalize(d);
-        df2_sweep<true>(d, glb(lucsr) + d.voff, XWhole{xs}, flags, wl[tid / kWave]);
+        if (kMode == kSweepFlags && kSplit) {   // the vector in global memory but its last rows (r0..n-1) in LDS
+            const XSplit xv{xs, g_lds_vec + nwords, r0};
+            df2_sweep<true>(d, glb(lucsr) + d.voff, xv, flags, wl[tid / kWave]);
+            __syncthreads();
+            for (uint32_t r = r0 + tid; r < n; r += nt) xs[r] = xv.xl[r - r0];
+        } else {
+            df2_sweep<true>(d, glb(lucsr) + d.voff, XWhole{xs}, flags, wl[tid / kWave]);
+        }
         __syncthreads();
     }
     const uint32_t *rpi = glb(desc->s.row_perm_inv);
@@ -2392,18 +2403,25 @@ void copy_df(const MatDesc *desc, uint32_t nval, const float *lu, float *lucsr, 
 void sweeps_df(const MatDesc *desc, const float *lucsr, float *t32, uint32_t r0, int threads, size_t lds,
                int transpose, hipStream_t st)
 {
-    if (transpose) k_sweeps_df<true><<<1, threads, lds, st>>>(desc, lucsr, t32, r0);
-    else k_sweeps_df<false><<<1, threads, lds, st>>>(desc, lucsr, t32, r0);
+    // r0 = UINT32_MAX: the whole vector in global memory
+    if (r0 != UINT32_MAX) {
+        if (transpose) k_sweeps_df<true, true><<<1, threads, lds, st>>>(desc, lucsr, t32, r0);
+        else k_sweeps_df<false, true><<<1, threads, lds, st>>>(desc, lucsr, t32, r0);
+    } else {
+        if (transpose) k_sweeps_df<true, false><<<1, threads, lds, st>>>(desc, lucsr, t32, r0);
+        else k_sweeps_df<false, false><<<1, threads, lds, st>>>(desc, lucsr, t32, r0);
+    }
 }
 
 // glu_solve in one workgroup: the factors into the sweep layout, then the
 // exact sweeps (mode: kSweepLds or kSweepFlags).
 void solve_one(const MatDesc *desc, uint32_t nval, const float *lu, float *lucsr, const float *b, float *x,
-               float *t32, int threads, int mode, size_t lds, hipStream_t st)
+               float *t32, uint32_t r0, int threads, int mode, size_t lds, hipStream_t st)
 {
     k_copy_df<false><<<blocks_for(nval, 256), 256, 0, st>>>(desc, lu, lucsr);
-    if (mode == kSweepLds) k_solve_one<kSweepLds><<<1, threads, lds, st>>>(desc, lucsr, b, x, t32);
-    else k_solve_one<kSweepFlags><<<1, threads, lds, st>>>(desc, lucsr, b, x, t32);
+    if (mode == kSweepLds) k_solve_one<kSweepLds><<<1, threads, lds, st>>>(desc, lucsr, b, x, t32, r0);
+    else if (r0 != UINT32_MAX) k_solve_one<kSweepFlags, true><<<1, threads, lds, st>>>(desc, lucsr, b, x, t32, r0);
+    else k_solve_one<kSweepFlags><<<1, threads, lds, st>>>(desc, lucsr, b, x, t32, r0);
 }
 
 void abft_cca(const FactorDev &F, const float *val, float *cca, int batch, hipStream_t st)
@@ -2483,8 +2501,11 @@ size_t enable_lds()
         reinterpret_cast<const void *>(&k_solve_one<kSweepFlags>),
         reinterpret_cast<const void *>(&k_refine_one<kSweepLds, false>),
         reinterpret_cast<const void *>(&k_refine_one<kSweepLds, true>),
-        reinterpret_cast<const void *>(&k_sweeps_df<false>),
-        reinterpret_cast<const void *>(&k_sweeps_df<true>),
+        reinterpret_cast<const void *>(&k_sweeps_df<false, false>),
+        reinterpret_cast<const void *>(&k_sweeps_df<true, false>),
+        reinterpret_cast<const void *>(&k_sweeps_df<false, true>),
+        reinterpret_cast<const void *>(&k_sweeps_df<true, true>),
+        reinterpret_cast<const void *>(&k_solve_one<kSweepFlags, true>),
         reinterpret_cast<const void *>(&k_batch_same_refine<kSweepLds>),
         reinterpret_cast<const void *>(&k_batch_same_refine<kSweepFlags>),
         reinterpret_cast<const void *>(&k_batch_same_refine<kSweepLevels>),

@@ -67,7 +67,7 @@ void factor_window(const FactorDev &F, int w, uint32_t nslot, float *val, const 
                    int threads, hipStream_t st);
 void zero_u64(unsigned long long *p, int count, hipStream_t st);
 void solve_one(const MatDesc *desc, uint32_t nval, const float *lu, float *lucsr, const float *b, float *x,
-               float *t32, int threads, int mode, size_t lds, hipStream_t st);
+               float *t32, uint32_t r0, int threads, int mode, size_t lds, hipStream_t st);
 void refine_one(const MatDesc *desc, uint32_t nval, const double *ax, const double *b, double *x,
                 const float *lu, float *lucsr, float *t32, double *r64, double *norms, double *kry, int32_t *iters,
                 double *eta, double tol, int maxit, int threads, size_t ref_lds, int transpose, hipStream_t st);
@@ -1791,6 +1791,15 @@ void run_sweeps(glu_plan *p, const float *lu, float *t, int transpose, hipStream
     }
 }
 
+// Rows whose values a one-workgroup sweep with ready bits of `bits` bytes
+// keeps in the rest of `cap` bytes of LDS: the last ones, when they are at
+// least a quarter of the n rows, else none.
+uint32_t split_rows(uint32_t n, size_t bits, size_t cap)
+{
+    const size_t keep = cap > bits ? std::min<size_t>(n, (cap - bits) / sizeof(float)) : 0;
+    return 4 * keep >= n ? (uint32_t)keep : 0u;
+}
+
 // The richest sweep mode of the batch kernels whose LDS fits order n:
 // 2 = ready bits and correction vector in LDS, 1 = ready bits only,
 // 0 = level-synchronous sweeps.
@@ -1948,8 +1957,12 @@ extern "C" glu_status glu_solve(glu_plan *p, const float *d_lu, const float *d_b
     auto issue = [&](hipStream_t s) {
         const SolveDev &S = p->S;
         if (mode != 0) {
+            // mode 1: the LDS the ready bits leave free holds the last rows
+            // (r0..n-1) when they are a quarter of the rows or more
+            const size_t bits = launch::batch_lds_bytes(mode, p->n), cap = launch::enable_lds();
+            const uint32_t keep = mode == 1 ? split_rows(p->n, bits, cap) : 0u;
             launch::solve_one(p->d_desc, p->df_nval, d_lu, p->d_blucsr, d_b, d_x, p->d_bt32,
-                              p->fused_threads, mode, launch::batch_lds_bytes(mode, p->n), s);
+                              keep ? p->n - keep : UINT32_MAX, p->fused_threads, mode, bits + sizeof(float) * keep, s);
         } else if (!transpose) {
             launch::perm_scale_f32(d_b, p->d_t32, S.col_perm, S.col_scale_perm, 0, p->n, s);
             run_sweeps(p, d_lu, p->d_t32, 0, s);
@@ -2000,9 +2013,10 @@ glu_status refine_host_loop(glu_plan *p, const double *d_ax, const float *d_lu, 
     const SolveDev &S = p->S;
     const uint32_t n = p->n;
     // the sweeps keep the results of the last rows in the LDS their ready
-    // bits leave free (rows r0..n-1)
+    // bits leave free (rows r0..n-1), when that is a quarter of the rows or
+    // more (below that the test per access costs more than it saves)
     const size_t bits = launch::batch_lds_bytes(1, n), cap = launch::enable_lds();
-    const uint32_t r0 = n - (uint32_t)std::min<size_t>(n, cap > bits ? (cap - bits) / sizeof(float) : 0);
+    const uint32_t keep = split_rows(n, bits, cap), r0 = keep ? n - keep : UINT32_MAX;
     HIP_TRY(hipMemsetAsync(d_x, 0, sizeof(double) * n, st));
     HIP_TRY(hipMemcpyAsync(r64, d_b, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
     double anorm = 0.0, bnorm = 0.0, eta = 0.0;
@@ -2013,8 +2027,7 @@ glu_status refine_host_loop(glu_plan *p, const double *d_ax, const float *d_lu, 
         else
             launch::perm_scale_f64(r64, t32, S.row_perm, S.row_scale, 1, n, st);
         if (lucsr)
-            launch::sweeps_df(p->d_desc, lucsr, t32, r0, p->fused_threads, bits + sizeof(float) * (n - r0), transpose,
-                              st);
+            launch::sweeps_df(p->d_desc, lucsr, t32, r0, p->fused_threads, bits + sizeof(float) * keep, transpose, st);
         else
             run_sweeps(p, d_lu, t32, transpose, st);
         if (!transpose)
