@@ -288,7 +288,7 @@ __device__ __forceinline__ void factor_divide(float *__restrict__ val, uint2 it,
 }
 
 struct LevelSpan {
-    uint32_t c0, nc, m0, nm, ml, d0, nd;
+    uint32_t c0, nc, m0, nm, ml;
 };
 
 __device__ __forceinline__ LevelSpan level_span(const FactorDev &F, int lvl)
@@ -299,12 +299,6 @@ __device__ __forceinline__ LevelSpan level_span(const FactorDev &F, int lvl)
     s.m0 = F.ment_ptr[lvl];
     s.nm = F.ment_ptr[lvl + 1] - s.m0;
     s.ml = F.ment_long[lvl];
-    if (lvl > 0) {
-        s.d0 = F.dvd_ptr[lvl - 1];
-        s.nd = F.dvd_ptr[lvl] - s.d0;
-    } else {
-        s.d0 = s.nd = 0;
-    }
     return s;
 }
 
@@ -327,23 +321,18 @@ __device__ __forceinline__ void factor_level(const FactorDev &F, float *__restri
 // tasks software-pipelined: a wave loads the table data of its next task
 // (the next one of the level, or its first one of the next level, issued
 // before the barrier) while it runs the current one, so a task waits only
-// for its values.  Same items and order as factor_level.
-// kDeferDiv: no division items here (the caller divides after the last
-// step: no update reads a divided L value, each divides on the fly).
+// for its values.  Same items and order as factor_level (no division items:
+// the caller divides after the last step; no update reads a divided L
+// value, each divides on the fly).
 // kFullMerge: see factor_merge_wave.
-template <bool kDeferDiv = false, bool kFullMerge = true>
+template <bool kFullMerge = true>
 __device__ __forceinline__ void factor_levels_wg(const FactorDev &F, float *__restrict__ val, int l0, int l1,
                                                  float pert, int perturb, float *lds)
 {
     const uint32_t tid = threadIdx.x, nt = blockDim.x;
     const uint32_t lane = tid & (kWave - 1), w = tid / kWave, nw = nt / kWave;
     CTab cur;
-    uint2 dv = make_uint2(0u, 0u);   // this lane's first division item of the step
-    if (l0 < l1) {
-        if (F.ctask_ptr[l0] + w < F.ctask_ptr[l0 + 1]) ctab_load(F, F.ctask_ptr[l0] + w, lane, cur);
-        const LevelSpan s = level_span(F, l0);
-        if (!kDeferDiv && tid < s.nd) dv = F.dvd[s.d0 + tid];
-    }
+    if (l0 < l1 && F.ctask_ptr[l0] + w < F.ctask_ptr[l0 + 1]) ctab_load(F, F.ctask_ptr[l0] + w, lane, cur);
     for (int lvl = l0; lvl < l1; ++lvl) {
         const LevelSpan s = level_span(F, lvl);
         for (uint32_t e = w; e < s.ml; e += nw)
@@ -356,14 +345,9 @@ __device__ __forceinline__ void factor_levels_wg(const FactorDev &F, float *__re
             if (more) cur = nxt;
         }
         for (uint32_t t = s.ml + tid; t < s.nm; t += nt) factor_merge(F, val, s.m0 + t, pert, perturb);
-        if (!kDeferDiv) {
-            if (tid < s.nd) factor_divide(val, dv, pert, perturb);
-            for (uint32_t t = tid + nt; t < s.nd; t += nt) factor_divide(val, F.dvd[s.d0 + t], pert, perturb);
-        }
         if (lvl + 1 < l1) {   // the next step's table data, loaded across the barrier
             const LevelSpan q = level_span(F, lvl + 1);
             if (q.c0 + w < q.c0 + q.nc) ctab_load(F, q.c0 + w, lane, cur);
-            if (!kDeferDiv && tid < q.nd) dv = F.dvd[q.d0 + tid];
         }
         __syncthreads();
     }
@@ -410,7 +394,7 @@ __global__ void __launch_bounds__(1024) k_factor_fused(FactorDev F, float *__res
 {
     __shared__ __attribute__((aligned(16))) float lds[1024 / kWave][kWave];
     const float pert = perturb ? *d_pert : 0.f;
-    factor_levels_wg<true>(F, val, l0, l1, pert, perturb, lds[threadIdx.x / kWave]);
+    factor_levels_wg(F, val, l0, l1, pert, perturb, lds[threadIdx.x / kWave]);
 }
 
 // The divisions L(i,k) /= U(k,k) of every column, after the last step (no
@@ -456,7 +440,7 @@ __device__ __forceinline__ WinDev load_window(const FactorDev &F, int w)
 
 // Steps [W.l0, W.l1) inside one workgroup with the values they touch staged
 // in LDS (slots): load, one barrier per step, store back.
-template <bool kDeferDiv = false, bool kFullMerge = true>
+template <bool kFullMerge = true>
 __device__ __forceinline__ void factor_window(const FactorDev &F, const WinDev &W, float *__restrict__ val,
                                               float *slots, float pert, int perturb, float *wlds)
 {
@@ -464,7 +448,7 @@ __device__ __forceinline__ void factor_window(const FactorDev &F, const WinDev &
     for (uint32_t s = tid; s < W.nslot; s += nt) slots[s] = val[W.pos[s]];
     __syncthreads();
     const FactorDev T = window_view(F, W);
-    factor_levels_wg<kDeferDiv, kFullMerge>(T, slots, W.l0, W.l1, pert, perturb, wlds);
+    factor_levels_wg<kFullMerge>(T, slots, W.l0, W.l1, pert, perturb, wlds);
     for (uint32_t s = tid; s < W.nslot; s += nt) val[W.pos[s]] = slots[s];
     __syncthreads();
 }
@@ -481,11 +465,11 @@ __device__ __forceinline__ void factor_all_wg(const FactorDev &F, float *__restr
     int l = 0;
     for (int w = 0; w < F.nwin; ++w) {
         const WinDev W = load_window(F, w);
-        factor_levels_wg<true, false>(F, val, l, W.l0, pert, perturb, wlds);
-        factor_window<true, false>(F, W, val, slots, pert, perturb, wlds);
+        factor_levels_wg<false>(F, val, l, W.l0, pert, perturb, wlds);
+        factor_window<false>(F, W, val, slots, pert, perturb, wlds);
         l = W.l1;
     }
-    factor_levels_wg<true, false>(F, val, l, F.num_lev + 1, pert, perturb, wlds);
+    factor_levels_wg<false>(F, val, l, F.num_lev + 1, pert, perturb, wlds);
     // the divisions, eight items a lane at a time (their loads issued
     // together); items are independent: an item writes its L entry and, for
     // a perturbed pivot, the pivot, which the column's other items divide
@@ -524,7 +508,7 @@ __global__ void __launch_bounds__(1024) k_factor_window(FactorDev F, float *__re
     __shared__ __attribute__((aligned(16))) float lds[1024 / kWave][kWave];
     const float pert = perturb ? *d_pert : 0.f;
     const WinDev W = load_window(F, w);
-    factor_window<true>(F, W, val, g_lds_vec, pert, perturb, lds[threadIdx.x / kWave]);
+    factor_window(F, W, val, g_lds_vec, pert, perturb, lds[threadIdx.x / kWave]);
 }
 
 // ---------------------------------------------------------------------------

@@ -501,8 +501,8 @@ struct Tables {
     std::vector<uint32_t> ctask_ptr;   // [num_lev + 2]
     std::vector<uint32_t> ment, ment_ptr, ment_long, msrc_ptr;
     std::vector<MergeSrc> msrc;
-    std::vector<uint2> dvd;            // division items, by level
-    std::vector<uint32_t> dvd_ptr;     // [num_lev + 1]
+    std::vector<uint2> dvd;            // division items, in CSC position order
+    std::vector<uint32_t> dvd_ptr;     // [num_lev + 1]: items of the columns of levels < l (built by level)
 };
 
 // Runs work(worker, i) for i in [0, count) on up to `threads` host threads
@@ -781,6 +781,10 @@ bool build_factor_tables(const glu_pattern &P, Tables &T, int64_t &updates, std:
     T.ctask_ptr[nl] = T.ctask_ptr[nl + 1] = (uint32_t)(T.cln.size() / 64);
     T.ment_ptr[nl] = T.ment_ptr[nl + 1] = (uint32_t)T.ment.size();
     T.dvd_ptr[nl] = (uint32_t)T.dvd.size();
+    // The divisions all run after the last step, in one pass; in position
+    // order a wave's items are consecutive L entries of a column (each
+    // column's first entry first), so the pass reads and writes whole lines.
+    std::sort(T.dvd.begin(), T.dvd.end(), [](const uint2 &a, const uint2 &b) { return a.x < b.x; });
     if (T.ctgt.size() >= UINT32_MAX || T.msrc.size() >= UINT32_MAX) {
         err = "more than 2^32 update items";
         return false;
