@@ -388,6 +388,165 @@ __global__ void __launch_bounds__(256) k_factor_grid(FactorDev F, float *__restr
                                            lds[threadIdx.x / kWave]);
 }
 
+// The items of factor_level for K matrices of a same-pattern batch at once
+// (values lu + m stride, GESP value pert[m], m < K): a lane loads its table
+// data once and issues the K matrices' value loads together; each matrix
+// gets exactly the arithmetic, in the same order, of the one-matrix item.
+template <int K>
+__device__ __forceinline__ void ctab_run_k(float *__restrict__ lu, size_t stride, const CTab &c, const float *pert,
+                                           int perturb)
+{
+    if ((c.tt[0] & c.tt[1] & c.tt[2] & c.tt[3]) == ~0u) return;
+    float d[K], a[K], uv[K][4], tv[K][4];
+#pragma unroll
+    for (int m = 0; m < K; ++m) {
+        const float *val = lu + m * stride;
+        d[m] = val[c.dpos];
+        a[m] = val[c.lpos];
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u)
+            if (c.tt[u] != ~0u) {
+                uv[m][u] = val[c.up[u]];
+                tv[m][u] = val[c.tt[u]];
+            }
+    }
+#pragma unroll
+    for (int m = 0; m < K; ++m) {
+        float *val = lu + m * stride;
+        const float l = a[m] / gesp(d[m], pert[m], perturb);
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u)
+            if (c.tt[u] != ~0u) val[c.tt[u]] = tv[m][u] + (-(l * uv[m][u]));
+    }
+}
+
+template <int K>
+__device__ __forceinline__ void factor_merge_k(const FactorDev &F, float *__restrict__ lu, size_t stride, uint32_t e,
+                                               const float *pert, int perturb)
+{
+    const uint4 r0 = F.mrec[2 * e], r1 = F.mrec[2 * e + 1];
+    float v[K], la[K], da[K], ua[K], lb[K], db[K], ub[K];
+#pragma unroll
+    for (int m = 0; m < K; ++m) {
+        const float *val = lu + m * stride;
+        v[m] = val[r0.x];
+        la[m] = val[r0.y];
+        ua[m] = val[r0.z];
+        da[m] = val[r0.w];
+        lb[m] = val[r1.x];
+        ub[m] = val[r1.y];
+        db[m] = val[r1.z];
+    }
+#pragma unroll
+    for (int m = 0; m < K; ++m) {
+        const float p0 = (la[m] / gesp(da[m], pert[m], perturb)) * ua[m];
+        const float p1 = (lb[m] / gesp(db[m], pert[m], perturb)) * ub[m];
+        v[m] = v[m] + (-p0);
+        v[m] = v[m] + (-p1);
+    }
+    if (r1.w > 2) {   // sources 2.., four at a time (as factor_merge_rest)
+        const uint32_t s1 = F.msrc_ptr[e + 1];
+        uint32_t s = F.msrc_ptr[e] + 2;
+        for (; s + 4 <= s1; s += 4) {
+            MergeSrc q[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) q[u] = F.msrc[s + u];
+            float p[K][4];
+#pragma unroll
+            for (int m = 0; m < K; ++m) {
+                const float *val = lu + m * stride;
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    p[m][u] = (val[q[u].lpos] / gesp(val[q[u].dpos], pert[m], perturb)) * val[q[u].upos];
+            }
+#pragma unroll
+            for (int m = 0; m < K; ++m)
+#pragma unroll
+                for (int u = 0; u < 4; ++u) v[m] = v[m] + (-p[m][u]);
+        }
+        for (; s < s1; ++s) {
+            const MergeSrc q = F.msrc[s];
+#pragma unroll
+            for (int m = 0; m < K; ++m) {
+                const float *val = lu + m * stride;
+                v[m] = v[m] + (-((val[q.lpos] / gesp(val[q.dpos], pert[m], perturb)) * val[q.upos]));
+            }
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < K; ++m) lu[m * stride + r0.x] = v[m];
+}
+
+// lds: this wave's K x 64 floats (256-byte aligned slices).
+template <int K>
+__device__ __forceinline__ void factor_merge_wave_k(const FactorDev &F, float *__restrict__ lu, size_t stride,
+                                                    uint32_t e, const float *pert, int perturb, uint32_t lane,
+                                                    float *lds)
+{
+    const uint32_t t = F.ment[e];
+    const uint32_t s0 = F.msrc_ptr[e], s1 = F.msrc_ptr[e + 1];
+    float v[K];
+#pragma unroll
+    for (int m = 0; m < K; ++m) v[m] = lu[m * stride + t];
+    for (uint32_t base = s0; base < s1; base += kWave) {
+        const uint32_t s = base + lane;
+        float p[K];
+#pragma unroll
+        for (int m = 0; m < K; ++m) p[m] = 0.f;
+        if (s < s1) {
+            const MergeSrc q = F.msrc[s];
+#pragma unroll
+            for (int m = 0; m < K; ++m) {
+                const float *val = lu + m * stride;
+                p[m] = (val[q.lpos] / gesp(val[q.dpos], pert[m], perturb)) * val[q.upos];
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < K; ++m) lds[m * kWave + lane] = p[m];
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        if (lane == 0)
+#pragma unroll
+            for (int m = 0; m < K; ++m) v[m] = serial_sub(v[m], lds + m * kWave, kWave, false);
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+    if (lane == 0)
+#pragma unroll
+        for (int m = 0; m < K; ++m) lu[m * stride + t] = v[m];
+}
+
+// One level for every matrix of a same-pattern batch (matrix m: values
+// lu + m nnz, GESP value pertd[2 m] as a double).  Block b takes lane chunk
+// b / groups of the level for matrices [K g, K g + K), g = b % groups:
+// the blocks that share a chunk's table data are dispatched side by side,
+// so the tables come from L2 rather than once per matrix, and each lane
+// loads its table data once for its K matrices.
+template <int K>
+__global__ void __launch_bounds__(256) k_factor_grid_batch(FactorDev F, float *__restrict__ lu,
+                                                           const double *__restrict__ pertd, int lvl, int perturb,
+                                                           uint32_t groups)
+{
+    __shared__ __attribute__((aligned(256))) float lds[256 / kWave][K * kWave];
+    const uint32_t g = blockIdx.x % groups, chunk = blockIdx.x / groups;
+    const uint32_t tid = chunk * blockDim.x + threadIdx.x, nthreads = (gridDim.x / groups) * blockDim.x;
+    const uint32_t lane = tid & (kWave - 1), w = tid / kWave, nw = nthreads / kWave;
+    const size_t stride = F.nnz;
+    float *base = lu + (size_t)g * K * stride;
+    float pert[K];
+#pragma unroll
+    for (int m = 0; m < K; ++m) pert[m] = perturb ? (float)pertd[2 * ((size_t)g * K + m)] : 0.f;
+    const LevelSpan s = level_span(F, lvl);
+    for (uint32_t e = w; e < s.ml; e += nw)
+        factor_merge_wave_k<K>(F, base, stride, s.m0 + e, pert, perturb, lane, lds[threadIdx.x / kWave]);
+    for (uint32_t t = s.c0 + w; t < s.c0 + s.nc; t += nw) {
+        CTab c;
+        ctab_load(F, t, lane, c);
+        ctab_run_k<K>(base, stride, c, pert, perturb);
+    }
+    for (uint32_t t = s.ml + tid; t < s.nm; t += nthreads) factor_merge_k<K>(F, base, stride, s.m0 + t, pert, perturb);
+}
+
 // A run of narrow levels inside one workgroup, a barrier between levels.
 __global__ void __launch_bounds__(1024) k_factor_fused(FactorDev F, float *__restrict__ val,
                                                        const float *__restrict__ d_pert, int l0, int l1, int perturb)
@@ -453,16 +612,18 @@ __device__ __forceinline__ void factor_window(const FactorDev &F, const WinDev &
     __syncthreads();
 }
 
-// All steps of one factorization by one workgroup: windows in LDS, the
-// steps between them in global memory.  The divisions L(i,k) /= U(k,k) of
-// every column run after the last step, all at once: no step reads a
-// divided L value (updates divide on the fly) and no step writes L(:,k) or
-// U(k,k) after column k's step, so the values are those of the per-step
-// order, one barrier-bound pass per step less.
+// Steps l0.. of one factorization by one workgroup: windows in LDS, the
+// steps between them in global memory; l0 is 0 or lies at or before the
+// first window's first step (the steps before it ran as
+// k_factor_grid_batch launches).  The divisions L(i,k) /= U(k,k) of every
+// column run after the last step, all at once: no step reads a divided L
+// value (updates divide on the fly) and no step writes L(:,k) or U(k,k)
+// after column k's step, so the values are those of the per-step order,
+// one barrier-bound pass per step less.
 __device__ __forceinline__ void factor_all_wg(const FactorDev &F, float *__restrict__ val, float *slots, float pert,
-                                              int perturb, float *wlds)
+                                              int perturb, float *wlds, int l0)
 {
-    int l = 0;
+    int l = l0;
     for (int w = 0; w < F.nwin; ++w) {
         const WinDev W = load_window(F, w);
         factor_levels_wg<false>(F, val, l, W.l0, pert, perturb, wlds);
@@ -1556,8 +1717,11 @@ enum : int {
 // or maxit (wg_refine).  Md is the matrix's descriptor in device memory:
 // each phase takes the (globalized) tables it needs from it, so the
 // descriptor is not held in registers as a whole.
-template <bool kCopy>
-__device__ __forceinline__ void wg_factor(const MatDesc *Md, const MatIO &io, int perturb, double *sh, float *wlds)
+//
+// wg_values: steps 1-2 (the matrix's values, |b|_inf with x = 0 and
+// r64 = b, the GESP value).
+__device__ __forceinline__ void wg_values(const MatDesc *Md, const MatIO &io, int perturb, double *sh, double &bnorm,
+                                          float &pert)
 {
     const uint32_t n = Md->n, nnz = Md->nnz;
     const uint32_t tid = threadIdx.x, nt = blockDim.x;
@@ -1567,7 +1731,6 @@ __device__ __forceinline__ void wg_factor(const MatDesc *Md, const MatIO &io, in
     //    |b|_inf (|A|_inf comes with the refinement's first product with A)
     for (uint32_t p = tid; p < nnz; p += nt) io.lu[p] = 0.f;
     __syncthreads();
-    double bnorm;
     {
         RefineDev R = Md->r;
         globalize(R);
@@ -1601,7 +1764,7 @@ __device__ __forceinline__ void wg_factor(const MatDesc *Md, const MatIO &io, in
     }
 
     // 2. GESP value from the freshly placed values
-    float pert = 0.f;
+    pert = 0.f;
     if (perturb) {
         const uint32_t *col_ptr = glb(Md->f.col_ptr);
         double cm = 0.0;
@@ -1612,13 +1775,27 @@ __device__ __forceinline__ void wg_factor(const MatDesc *Md, const MatIO &io, in
         }
         pert = kPertFactor * (float)wg_max(cm, sh);
     }
+}
+
+// Steps 1-4 (l0 = 0), or steps l0.. of step 3 and step 4 after
+// k_batch_values and the k_factor_grid_batch launches of steps [0, l0)
+// (GESP value in io.norms[0], |b|_inf in io.norms[1]).
+template <bool kCopy>
+__device__ __forceinline__ void wg_factor(const MatDesc *Md, const MatIO &io, int perturb, double *sh, float *wlds,
+                                          int l0)
+{
+    const uint32_t tid = threadIdx.x, nt = blockDim.x;
+    double bnorm = 0.0;
+    float pert;
+    if (l0 == 0) wg_values(Md, io, perturb, sh, bnorm, pert);
+    else pert = perturb ? (float)io.norms[0] : 0.f;
 
     // 3. factorization, level by level; LDS windows with their values in LDS
     //    (the sweeps' LDS is free until step 5)
     {
         FactorDev F = Md->f;
         globalize(F);
-        factor_all_wg(F, io.lu, g_lds_vec, pert, perturb, wlds);
+        factor_all_wg(F, io.lu, g_lds_vec, pert, perturb, wlds, l0);
     }
 
     // 4. the factors copied into the sweeps' task-contiguous layout, built
@@ -1653,7 +1830,7 @@ __device__ __forceinline__ void wg_factor(const MatDesc *Md, const MatIO &io, in
             __syncthreads();
         }
     }
-    if (tid == 0 && io.norms) io.norms[1] = bnorm;
+    if (tid == 0 && io.norms && l0 == 0) io.norms[1] = bnorm;
 }
 
 // Workgroup sum in a fixed order (deterministic): each wave's butterfly,
@@ -1966,7 +2143,8 @@ __device__ __forceinline__ MatIO batch_io(const MatDesc *desc, const double *ax,
     return io;
 }
 
-// Dynamic LDS: the largest LDS window's values.
+// Dynamic LDS: the largest LDS window's values.  l0 > 0: steps [0, l0) ran
+// as k_batch_values + k_factor_grid_batch launches.
 template <bool kCopy>
 __global__ void __launch_bounds__(1024) k_batch_same_factor(const MatDesc *__restrict__ desc,
                                                             const double *__restrict__ ax, const double *__restrict__ b,
@@ -1974,27 +2152,56 @@ __global__ void __launch_bounds__(1024) k_batch_same_factor(const MatDesc *__res
                                                             float *__restrict__ lucsr, float *__restrict__ t32,
                                                             double *__restrict__ r64, double *__restrict__ norms,
                                                             double *__restrict__ kry, int32_t *__restrict__ iters,
-                                                            double *__restrict__ eta, int perturb)
+                                                            double *__restrict__ eta, int perturb, int l0)
 {
     __shared__ double sh[1024 / kWave];
     __shared__ __attribute__((aligned(16))) float wl[1024 / kWave][kWave];
     const MatIO io = batch_io(desc, ax, b, x, lu, lucsr, t32, r64, norms, kry, iters, eta);
-    wg_factor<kCopy>(desc, io, perturb, sh, wl[threadIdx.x / kWave]);
+    wg_factor<kCopy>(desc, io, perturb, sh, wl[threadIdx.x / kWave], l0);
+}
+
+// Steps 1-2 of wg_factor for matrix blockIdx.x of a same-pattern batch
+// ahead of k_factor_grid_batch: norms[2 m] = the GESP value (as a double),
+// norms[2 m + 1] = |b|_inf (b null: refactor only, no right-hand side).
+__global__ void __launch_bounds__(1024) k_batch_values(const MatDesc *__restrict__ desc,
+                                                       const double *__restrict__ ax, const double *__restrict__ b,
+                                                       double *__restrict__ x, float *__restrict__ lu,
+                                                       double *__restrict__ r64, double *__restrict__ norms, int perturb)
+{
+    __shared__ double sh[1024 / kWave];
+    const uint64_t m = blockIdx.x, n = desc->n;
+    MatIO io{};
+    io.ax = glb(ax + m * desc->r.nnz_a);
+    io.lu = glb(lu + m * desc->nnz);
+    if (b) {
+        io.b = glb(b + m * n);
+        io.x = glb(x + m * n);
+        io.r64 = glb(r64 + m * n);
+    }
+    double bnorm;
+    float pert;
+    wg_values(desc, io, perturb, sh, bnorm, pert);
+    if (threadIdx.x == 0) {
+        norms[2 * m] = (double)pert;
+        norms[2 * m + 1] = bnorm;
+    }
 }
 
 // Refresh + factor only (glu_batch_factor): matrix blockIdx.x of a
 // same-pattern batch, values ax + b nnz_a into lu + b nnz.  Dynamic LDS: the
-// largest LDS window's values.
+// largest LDS window's values.  l0 > 0: steps [0, l0) ran as
+// k_batch_values + k_factor_grid_batch launches (GESP values in norms).
 __global__ void __launch_bounds__(1024) k_batch_factor(const MatDesc *__restrict__ desc,
                                                        const double *__restrict__ ax, float *__restrict__ lu,
-                                                       int perturb)
+                                                       double *__restrict__ norms, int perturb, int l0)
 {
     __shared__ double sh[1024 / kWave];
     __shared__ __attribute__((aligned(16))) float wl[1024 / kWave][kWave];
     MatIO io{};
     io.ax = glb(ax + (uint64_t)blockIdx.x * desc->r.nnz_a);
     io.lu = glb(lu + (uint64_t)blockIdx.x * desc->nnz);
-    wg_factor<false>(desc, io, perturb, sh, wl[threadIdx.x / kWave]);
+    if (l0 > 0) io.norms = glb(norms + 2 * (uint64_t)blockIdx.x);
+    wg_factor<false>(desc, io, perturb, sh, wl[threadIdx.x / kWave], l0);
 }
 
 // Dynamic LDS: the sweeps' ready bits (and correction vector).
@@ -2141,7 +2348,7 @@ __global__ void __launch_bounds__(1024) k_batch_group_factor(const MatDesc *__re
     const uint32_t m = order[blockIdx.x];
     MatIO io = ios[m];
     globalize(io);
-    wg_factor<kCopy>(descs + m, io, perturb, sh, wl[threadIdx.x / kWave]);
+    wg_factor<kCopy>(descs + m, io, perturb, sh, wl[threadIdx.x / kWave], 0);
 }
 
 template <int kMode>
@@ -2339,14 +2546,14 @@ size_t batch_lds_bytes(int mode, uint32_t n)
 void batch_same(const MatDesc *desc, int batch, const double *ax, const double *b, double *x, float *lu,
                 float *lucsr, float *t32, double *r64, double *norms, double *kry, int32_t *iters, double *eta,
                 double tol, int maxit, int perturb, int threads, int mode, size_t fact_lds, size_t ref_lds,
-                hipStream_t st)
+                int l0, hipStream_t st)
 {
     if (mode == kSweepLevels)
         k_batch_same_factor<false><<<batch, threads, fact_lds, st>>>(desc, ax, b, x, lu, lucsr, t32, r64, norms,
-                                                                      kry, iters, eta, perturb);
+                                                                      kry, iters, eta, perturb, l0);
     else
         k_batch_same_factor<true><<<batch, threads, fact_lds, st>>>(desc, ax, b, x, lu, lucsr, t32, r64, norms,
-                                                                     kry, iters, eta, perturb);
+                                                                     kry, iters, eta, perturb, l0);
     if (mode == kSweepLds)
         k_batch_same_refine<kSweepLds><<<batch, threads, ref_lds, st>>>(desc, ax, b, x, lu, lucsr, t32, r64, norms,
                                                                          kry, iters, eta, tol, maxit);
@@ -2425,10 +2632,35 @@ void abft_check(const FactorDev &F, const uint32_t *row_idx, const uint32_t *dia
     k_abft_finish<<<blocks_for(batch, 256), 256, 0, st>>>(first_bad, batch);
 }
 
-void batch_factor(const MatDesc *desc, int batch, const double *ax, float *lu, int perturb, int threads,
-                  size_t fact_lds, hipStream_t st)
+void batch_factor(const MatDesc *desc, int batch, const double *ax, float *lu, double *norms, int perturb,
+                  int threads, size_t fact_lds, int l0, hipStream_t st)
 {
-    k_batch_factor<<<batch, threads, fact_lds, st>>>(desc, ax, lu, perturb);
+    k_batch_factor<<<batch, threads, fact_lds, st>>>(desc, ax, lu, norms, perturb, l0);
+}
+
+void batch_values(const MatDesc *desc, int batch, const double *ax, const double *b, double *x, float *lu,
+                  double *r64, double *norms, int perturb, int threads, hipStream_t st)
+{
+    k_batch_values<<<batch, threads, 0, st>>>(desc, ax, b, x, lu, r64, norms, perturb);
+}
+
+// Matrices per lane of k_factor_grid_batch: the most of 4, 2, 1 that
+// divides the batch.
+int grid_batch_k(int batch)
+{
+    return batch % 4 == 0 ? 4 : batch % 2 == 0 ? 2 : 1;
+}
+
+// lanes: the level's lanes for one matrix (column tasks and long merges a
+// wave each, short merges a lane each).
+void factor_grid_batch(const FactorDev &F, float *lu, const double *norms, int lvl, uint32_t lanes, int batch,
+                       int perturb, hipStream_t st)
+{
+    const int k = grid_batch_k(batch);
+    const uint32_t groups = (uint32_t)(batch / k), blocks = blocks_for(lanes, 256) * groups;
+    if (k == 4) k_factor_grid_batch<4><<<blocks, 256, 0, st>>>(F, lu, norms, lvl, perturb, groups);
+    else if (k == 2) k_factor_grid_batch<2><<<blocks, 256, 0, st>>>(F, lu, norms, lvl, perturb, groups);
+    else k_factor_grid_batch<1><<<blocks, 256, 0, st>>>(F, lu, norms, lvl, perturb, groups);
 }
 
 void batch_group(const MatDesc *descs, const void *ios, const uint32_t *order, int count, double tol,

@@ -51,9 +51,13 @@ size_t batch_lds_bytes(int mode, uint32_t n);
 void batch_same(const MatDesc *desc, int batch, const double *ax, const double *b, double *x, float *lu,
                 float *lucsr, float *t32, double *r64, double *norms, double *kry, int32_t *iters, double *eta,
                 double tol, int maxit, int perturb, int threads, int mode, size_t fact_lds, size_t ref_lds,
-                hipStream_t st);
-void batch_factor(const MatDesc *desc, int batch, const double *ax, float *lu, int perturb, int threads,
-                  size_t fact_lds, hipStream_t st);
+                int l0, hipStream_t st);
+void batch_factor(const MatDesc *desc, int batch, const double *ax, float *lu, double *norms, int perturb,
+                  int threads, size_t fact_lds, int l0, hipStream_t st);
+void batch_values(const MatDesc *desc, int batch, const double *ax, const double *b, double *x, float *lu,
+                  double *r64, double *norms, int perturb, int threads, hipStream_t st);
+void factor_grid_batch(const FactorDev &F, float *lu, const double *norms, int lvl, uint32_t lanes, int batch,
+                       int perturb, hipStream_t st);
 void abft_cca(const FactorDev &F, const float *val, float *cca, int batch, hipStream_t st);
 void abft_check(const FactorDev &F, const uint32_t *row_idx, const uint32_t *diag, const float *lu,
                 const float *cca, float *ccl, float tol, int32_t *first_bad, float *max_diff, int batch,
@@ -327,6 +331,11 @@ struct glu_plan {
     uint32_t win_nslot = 0;      // the largest window's values
     uint32_t copy_range = 0;     // slots the batch kernels' value copy stages in LDS at a time
     std::vector<uint32_t> win_slots;   // values of each window
+    // batched kernels: steps [0, batch_grid_lev) run as one grid launch per
+    // step over the batch (grid_lanes: each one's lanes for one matrix)
+    int batch_grid_opt = 0;      // glu_plan_opts.batch_grid_levels
+    int batch_grid_lev = 0;
+    std::vector<uint32_t> grid_lanes;
 
     // hipGraph replay of the launch sequences (use_graph): one executable
     // graph per entry point, keyed on the buffers it was captured with
@@ -365,6 +374,8 @@ struct glu_plan {
     char *d_rout = nullptr;      // glu_refine's iteration count and backward error (device)
     double *d_bkry = nullptr;    // Arnoldi vectors V, Z = M V, W = A Z: (3 krylov + 1) n per matrix
     int batch_cap = 0;
+    double *d_fnorms = nullptr;  // glu_batch_factor: GESP value per matrix (2 doubles each)
+    int fnorms_cap = 0;
 
     ~glu_plan()
     {
@@ -378,6 +389,7 @@ struct glu_plan {
         if (d_tlucsr) (void)hipFree(d_tlucsr);
         if (d_abft) (void)hipFree(d_abft);
         if (d_bnorms) (void)hipFree(d_bnorms);
+        if (d_fnorms) (void)hipFree(d_fnorms);
         for (GraphSlot *g : {&g_factor, &g_solve[0], &g_solve[1]})
             if (g->exec) (void)hipGraphExecDestroy(g->exec);
         if (cap_stream) (void)hipStreamDestroy(cap_stream);
@@ -1320,6 +1332,8 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
         if (opts->krylov) p->krylov = opts->krylov < 0 ? 0 : opts->krylov;
         if (opts->lds_window_values < 0) return fail(GLU_ERR_ARGUMENT, "lds_window_values must be >= 0");
         p->win_values = (uint32_t)opts->lds_window_values;
+        if (opts->batch_grid_levels < -1) return fail(GLU_ERR_ARGUMENT, "batch_grid_levels must be >= -1");
+        p->batch_grid_opt = opts->batch_grid_levels;
     }
     p->n = P.n;
     p->nnz = P.nnz;
@@ -1347,6 +1361,16 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
     }
     factor_segments(*p, T, wins);
     p->units = (int64_t)(T.cln.size() / 64);
+    // The batched kernels' leading grid steps: those ahead of the first LDS
+    // window with at least a workgroup's worth of lanes per matrix (a
+    // narrower step takes one pass of the per-matrix workgroup anyway).
+    {
+        const int lim = std::min(wins.empty() ? P.num_lev + 1 : wins[0].l0,
+                                 p->batch_grid_opt < 0 ? 0 : p->batch_grid_opt > 0 ? p->batch_grid_opt : INT32_MAX);
+        for (int l = 0; l < lim && level_lanes(T, l) >= (uint32_t)p->fused_threads; ++l)
+            p->grid_lanes.push_back(level_lanes(T, l));
+        p->batch_grid_lev = (int)p->grid_lanes.size();
+    }
 
     const uint32_t n = P.n;
     // Solve schedules.
@@ -1878,7 +1902,7 @@ extern "C" glu_status glu_plan_info(const glu_plan *p, int64_t info[12])
     info[8] = p->nwin;
     info[9] = p->win_steps;
     info[10] = p->win_nslot;
-    info[11] = 0;
+    info[11] = p->batch_grid_lev;
     return GLU_OK;
 }
 
@@ -2221,6 +2245,23 @@ extern "C" glu_status glu_adjoint_values(glu_plan *p, const double *d_lam, const
     return GLU_OK;
 }
 
+namespace {
+
+// The batched kernels' first steps: each matrix's values, |b|_inf and GESP
+// value (norms: 2 per matrix), then steps [0, batch_grid_lev) as one grid
+// launch each over the whole batch; the per-matrix workgroups go on from
+// there (b null: refactor only).
+void batch_grid_steps(glu_plan *p, int batch, const double *d_ax, const double *d_b, double *d_x, float *d_lu,
+                      double *r64, double *norms, int perturb, hipStream_t st)
+{
+    if (p->batch_grid_lev == 0) return;
+    launch::batch_values(p->d_desc, batch, d_ax, d_b, d_x, d_lu, r64, norms, perturb, p->fused_threads, st);
+    for (int l = 0; l < p->batch_grid_lev; ++l)
+        launch::factor_grid_batch(p->F, d_lu, norms, l, p->grid_lanes[l], batch, perturb, st);
+}
+
+}  // namespace
+
 extern "C" glu_status glu_batch_solve(glu_plan *p, int batch, const double *d_ax, const double *d_b, double *d_x,
                                       float *d_lu, double tol, int maxit, int32_t *d_iters, double *d_eta,
                                       unsigned flags, void *stream)
@@ -2232,10 +2273,13 @@ extern "C" glu_status glu_batch_solve(glu_plan *p, int batch, const double *d_ax
     if (s != GLU_OK) return s;
     const size_t cap = launch::enable_lds();
     const int mode = sweep_mode(p->n, cap);
+    const int perturb = (int)(flags & 1u);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    batch_grid_steps(p, batch, d_ax, d_b, d_x, d_lu, p->d_br64, p->d_bnorms, perturb, st);
     launch::batch_same(p->d_desc, batch, d_ax, d_b, d_x, d_lu, p->d_blucsr, p->d_bt32, p->d_br64, p->d_bnorms,
-                       p->d_bkry, d_iters, d_eta, tol, maxit, (int)(flags & 1u), p->fused_threads, mode,
+                       p->d_bkry, d_iters, d_eta, tol, maxit, perturb, p->fused_threads, mode,
                        sizeof(float) * std::max(p->win_nslot, p->copy_range), launch::batch_lds_bytes(mode, p->n),
-                       static_cast<hipStream_t>(stream));
+                       p->batch_grid_lev, st);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(GLU_ERR_DEVICE, std::string("batch launch: ") + hipGetErrorString(e));
     return GLU_OK;
@@ -2247,8 +2291,18 @@ extern "C" glu_status glu_batch_factor(glu_plan *p, int batch, const double *d_a
     if (!p || batch <= 0 || !d_ax || !d_lu) return fail(GLU_ERR_ARGUMENT, "bad argument");
     if (!p->has_refine) return fail(GLU_ERR_ARGUMENT, "plan has no analysis");
     (void)launch::enable_lds();
-    launch::batch_factor(p->d_desc, batch, d_ax, d_lu, (int)(flags & 1u), p->fused_threads,
-                         sizeof(float) * p->win_nslot, static_cast<hipStream_t>(stream));
+    const int perturb = (int)(flags & 1u);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (p->batch_grid_lev > 0 && batch > p->fnorms_cap) {
+        if (p->d_fnorms) (void)hipFree(p->d_fnorms);
+        p->d_fnorms = nullptr;
+        p->fnorms_cap = 0;
+        HIP_TRY(hipMalloc(&p->d_fnorms, 2 * sizeof(double) * (size_t)batch));
+        p->fnorms_cap = batch;
+    }
+    batch_grid_steps(p, batch, d_ax, nullptr, nullptr, d_lu, nullptr, p->d_fnorms, perturb, st);
+    launch::batch_factor(p->d_desc, batch, d_ax, d_lu, p->d_fnorms, perturb, p->fused_threads,
+                         sizeof(float) * p->win_nslot, p->batch_grid_lev, st);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(GLU_ERR_DEVICE, std::string("batch factor launch: ") + hipGetErrorString(e));
     return GLU_OK;

@@ -54,7 +54,7 @@ class Pattern(C.Structure):
 class PlanOpts(C.Structure):
     _fields_ = [("fuse_units", C.c_int32), ("block_threads", C.c_int32), ("use_graph", C.c_int32),
                 ("lds_windows", C.c_int32), ("solve_dataflow", C.c_int32), ("krylov", C.c_int32),
-                ("lds_window_values", C.c_int32), ("reserved", C.c_int32 * 1)]
+                ("lds_window_values", C.c_int32), ("batch_grid_levels", C.c_int32)]
 
 
 _lib = None

@@ -56,18 +56,23 @@ class Plan:
     levels of the single-matrix solves barrier-free (0 = default, 1 = on,
     -1 = off).  ``krylov``: refinement of the batched and grouped solves
     after the first solve (0 = default, restarted FGMRES(2); -1 = plain
-    refinement; 1..4 = FGMRES(m)).
+    refinement; 1..4 = FGMRES(m)).  ``batch_grid_levels``: leading
+    factorization steps of the batched kernels run as one grid launch per
+    step over the whole batch (0 = default, the wide steps ahead of the
+    first LDS window; -1 = none; k > 0 = at most k); the same bits either way.
 
     After creation, ``lds_window_count``, ``lds_window_steps`` and
-    ``lds_window_max_values`` describe the factorization's LDS windows.
+    ``lds_window_max_values`` describe the factorization's LDS windows and
+    ``batch_grid_levels`` the batched kernels' leading grid steps.
     """
 
     def __init__(self, source, fuse_units: int = 0, block_threads: int = 0, use_graph: bool = False,
-                 lds_windows: int = 0, solve_dataflow: int = 0, krylov: int = 0, lds_window_values: int = 0):
+                 lds_windows: int = 0, solve_dataflow: int = 0, krylov: int = 0, lds_window_values: int = 0,
+                 batch_grid_levels: int = 0):
         _require_device()
         L = capi.lib()
         opts = capi.PlanOpts(fuse_units, block_threads, int(bool(use_graph)), int(lds_windows),
-                             int(solve_dataflow), int(krylov), int(lds_window_values))
+                             int(solve_dataflow), int(krylov), int(lds_window_values), int(batch_grid_levels))
         h = C.c_void_p()
         if isinstance(source, Analysis):
             capi.check(L.glu_plan_create_from_analysis(source.handle, C.byref(opts), C.byref(h)),
@@ -86,7 +91,7 @@ class Plan:
         capi.check(L.glu_plan_info(h, capi.ptr(info)), "glu_plan_info")
         self.n, self.nnz_lu, self.nnz_a, self.num_lev, self.updates, self.segments, \
             self.device_bytes, self.units, self.lds_window_count, self.lds_window_steps, \
-            self.lds_window_max_values, _ = (int(v) for v in info)
+            self.lds_window_max_values, self.batch_grid_levels = (int(v) for v in info)
 
     @property
     def handle(self):
@@ -154,7 +159,8 @@ class Plan:
 
     # -- same-pattern batch --------------------------------------------
     def batch_solve(self, ax, b, tol=1e-10, maxit=10, perturb=False, lu=None, x=None):
-        """One persistent launch: refresh + factor + solve + refine per matrix.
+        """Refresh + factor + solve + refine of every matrix: a workgroup per
+        matrix (after the wide leading steps as grid launches over the batch).
 
         ax: (B, nnz_a) float64, b: (B, n) float64.  Returns (x, lu, iters, eta).
         """
@@ -199,7 +205,8 @@ class Plan:
         return first, mx
 
     def batch_factor(self, ax, lu=None, perturb=False):
-        """One launch: refresh + factor per matrix (no solve).
+        """Refresh + factor of every matrix (no solve), launched as batch_solve's
+        factorization.
 
         ax: (B, nnz_a) float64.  Returns lu (B, nnz_lu) float32, bit-identical
         to refresh + factor of each matrix.

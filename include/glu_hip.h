@@ -137,7 +137,14 @@ typedef struct glu_plan_opts {
                                as the workgroup's LDS holds (gfx950: ~39.8k),
                                k > 0 = at most k (smaller windows, more of
                                them: a schedule knob and a test hook) */
-    int32_t reserved[1];
+    int32_t batch_grid_levels; /* glu_batch_solve/glu_batch_factor: the leading
+                               factorization steps run as one grid launch per
+                               step over the whole batch (a lane's table data
+                               serves up to 4 matrices) before the
+                               workgroup-per-matrix kernel: 0 = default (the
+                               steps ahead of the first LDS window with at
+                               least block_threads lanes), -1 = none, k > 0 =
+                               at most k of those.  Same bits either way. */
 } glu_plan_opts;
 
 glu_status glu_plan_create(const glu_pattern *pat, const glu_plan_opts *opts, glu_plan **out);
@@ -151,7 +158,7 @@ void glu_plan_destroy(glu_plan *p);
  * info[5]=segments in the factor schedule info[6]=device bytes held
  * info[7]=64-lane column tasks info[8]=LDS windows of the factorization
  * info[9]=steps inside them info[10]=values the largest window stages in
- * LDS info[11]=reserved (0) */
+ * LDS info[11]=leading steps the batched kernels run as grid launches */
 glu_status glu_plan_info(const glu_plan *p, int64_t info[12]);
 
 /* ------------------------------------------------------------------------

@@ -380,6 +380,42 @@ def test_batch_factor_matches_single(torch_mod, perturb):
         assert torch.equal(one.view(torch.int32), lu[b].view(torch.int32)), b
 
 
+@pytest.mark.parametrize("B", [8, 6, 5])
+@pytest.mark.parametrize("perturb", [False, True])
+def test_batch_grid_steps_same_bits(torch_mod, B, perturb):
+    """The batched kernels' leading steps as grid launches over the batch
+    (4, 2 or 1 matrices per lane as the batch size allows) give the same
+    bits as the workgroup-per-matrix kernel alone: factors, solutions,
+    iteration counts and backward errors of glu_batch_solve, factors of
+    glu_batch_factor, and the single path's factors."""
+    from pyglu import mna
+    from pyglu.device import Plan
+    torch = torch_mod
+    data, indices, indptr, n, an, plan, pat = _setup("batch")
+    axd = torch.from_numpy(np.stack([mna.perturb_values(data, 5100 + b) for b in range(B)])).cuda()
+    bd = torch.from_numpy(np.stack([mna.rhs(n, 5300 + b) for b in range(B)])).cuda()
+    assert plan.batch_grid_levels > 0
+    runs = {}
+    for opt in (-1, 0, 3):
+        pl = plan if opt == 0 else Plan(an, batch_grid_levels=opt)
+        assert pl.batch_grid_levels == {-1: 0, 0: plan.batch_grid_levels, 3: 3}[opt]
+        print(f"  B={B} perturb={perturb} batch_grid_levels={pl.batch_grid_levels}", flush=True)
+        x, lu, iters, eta = pl.batch_solve(axd, bd, tol=1e-10, maxit=20, perturb=perturb)
+        luf = pl.batch_factor(axd, perturb=perturb)
+        torch.cuda.synchronize()
+        runs[opt] = [t.clone() for t in (x, lu, iters, eta, luf)]
+        if pl is not plan:
+            pl.close()
+    for opt in (0, 3):
+        for a, b in zip(runs[-1], runs[opt]):
+            assert torch.equal(a.view(torch.uint8), b.view(torch.uint8)), opt
+    assert torch.equal(runs[0][1].view(torch.int32), runs[0][4].view(torch.int32))
+    one = plan.refresh(axd[B - 1]).reshape(-1)
+    plan.factor(one, perturb=perturb)
+    torch.cuda.synchronize()
+    assert torch.equal(one.view(torch.int32), runs[0][1][B - 1].view(torch.int32))
+
+
 def test_abft_device_matches_oracle(torch_mod):
     """glu_abft_sums / glu_abft_check (src/symbolic.cc:163-197) on a batch of
     three value sets: checksums bit-identical to the oracle's, the same
