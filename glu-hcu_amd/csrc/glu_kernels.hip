@@ -388,163 +388,212 @@ __global__ void __launch_bounds__(256) k_factor_grid(FactorDev F, float *__restr
                                            lds[threadIdx.x / kWave]);
 }
 
-// The items of factor_level for K matrices of a same-pattern batch at once
-// (values lu + m stride, GESP value pert[m], m < K): a lane loads its table
-// data once and issues the K matrices' value loads together; each matrix
-// gets exactly the arithmetic, in the same order, of the one-matrix item.
-template <int K>
-__device__ __forceinline__ void ctab_run_k(float *__restrict__ lu, size_t stride, const CTab &c, const float *pert,
-                                           int perturb)
+// ---- interleaved batch steps ----------------------------------------------
+// The wide leading steps of a same-pattern batch run over a scratch copy of
+// the values with kIl matrices interleaved position by position: matrix
+// m = kIl g + j keeps position p at S[(g nnz + p) kIl + j], so the kIl
+// lanes that run one item for the kIl matrices of a group touch one 64-byte
+// line per value (and share its table data), where matrix-contiguous
+// values cost a line per lane.  A wave runs 64 / kIl items.
+constexpr uint32_t kIl = 16;
+constexpr uint32_t kIlItems = kWave / kIl;   // items per wave
+
+// Column-task item `q` (lane q % 64 of task c0 + q / 64) of one matrix,
+// values V[p kIl]: factor_ctasks' arithmetic.
+__device__ __forceinline__ void il_ctask(const FactorDev &F, float *__restrict__ V, uint32_t c0, uint32_t q,
+                                         float pert, int perturb)
 {
+    CTab c;
+    ctab_load(F, c0 + q / kWave, q % kWave, c);
     if ((c.tt[0] & c.tt[1] & c.tt[2] & c.tt[3]) == ~0u) return;
-    float d[K], a[K], uv[K][4], tv[K][4];
+    const float d = V[(size_t)c.dpos * kIl], a = V[(size_t)c.lpos * kIl];
+    float uv[4], tv[4];
 #pragma unroll
-    for (int m = 0; m < K; ++m) {
-        const float *val = lu + m * stride;
-        d[m] = val[c.dpos];
-        a[m] = val[c.lpos];
+    for (uint32_t u = 0; u < 4; ++u)
+        if (c.tt[u] != ~0u) {
+            uv[u] = V[(size_t)c.up[u] * kIl];
+            tv[u] = V[(size_t)c.tt[u] * kIl];
+        }
+    const float l = a / gesp(d, pert, perturb);
 #pragma unroll
-        for (uint32_t u = 0; u < 4; ++u)
-            if (c.tt[u] != ~0u) {
-                uv[m][u] = val[c.up[u]];
-                tv[m][u] = val[c.tt[u]];
-            }
-    }
-#pragma unroll
-    for (int m = 0; m < K; ++m) {
-        float *val = lu + m * stride;
-        const float l = a[m] / gesp(d[m], pert[m], perturb);
-#pragma unroll
-        for (uint32_t u = 0; u < 4; ++u)
-            if (c.tt[u] != ~0u) val[c.tt[u]] = tv[m][u] + (-(l * uv[m][u]));
-    }
+    for (uint32_t u = 0; u < 4; ++u)
+        if (c.tt[u] != ~0u) V[(size_t)c.tt[u] * kIl] = tv[u] + (-(l * uv[u]));
 }
 
-template <int K>
-__device__ __forceinline__ void factor_merge_k(const FactorDev &F, float *__restrict__ lu, size_t stride, uint32_t e,
-                                               const float *pert, int perturb)
+// Merge entry e of one matrix, values V[p kIl]: factor_merge's arithmetic.
+__device__ __forceinline__ void il_merge(const FactorDev &F, float *__restrict__ V, uint32_t e, float pert,
+                                         int perturb)
 {
     const uint4 r0 = F.mrec[2 * e], r1 = F.mrec[2 * e + 1];
-    float v[K], la[K], da[K], ua[K], lb[K], db[K], ub[K];
-#pragma unroll
-    for (int m = 0; m < K; ++m) {
-        const float *val = lu + m * stride;
-        v[m] = val[r0.x];
-        la[m] = val[r0.y];
-        ua[m] = val[r0.z];
-        da[m] = val[r0.w];
-        lb[m] = val[r1.x];
-        ub[m] = val[r1.y];
-        db[m] = val[r1.z];
-    }
-#pragma unroll
-    for (int m = 0; m < K; ++m) {
-        const float p0 = (la[m] / gesp(da[m], pert[m], perturb)) * ua[m];
-        const float p1 = (lb[m] / gesp(db[m], pert[m], perturb)) * ub[m];
-        v[m] = v[m] + (-p0);
-        v[m] = v[m] + (-p1);
-    }
-    if (r1.w > 2) {   // sources 2.., four at a time (as factor_merge_rest)
+    float v = V[(size_t)r0.x * kIl];
+    const float p0 = (V[(size_t)r0.y * kIl] / gesp(V[(size_t)r0.w * kIl], pert, perturb)) * V[(size_t)r0.z * kIl];
+    const float p1 = (V[(size_t)r1.x * kIl] / gesp(V[(size_t)r1.z * kIl], pert, perturb)) * V[(size_t)r1.y * kIl];
+    v = v + (-p0);
+    v = v + (-p1);
+    if (r1.w > 2) {
         const uint32_t s1 = F.msrc_ptr[e + 1];
         uint32_t s = F.msrc_ptr[e] + 2;
         for (; s + 4 <= s1; s += 4) {
-            MergeSrc q[4];
+            float p[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) q[u] = F.msrc[s + u];
-            float p[K][4];
-#pragma unroll
-            for (int m = 0; m < K; ++m) {
-                const float *val = lu + m * stride;
-#pragma unroll
-                for (int u = 0; u < 4; ++u)
-                    p[m][u] = (val[q[u].lpos] / gesp(val[q[u].dpos], pert[m], perturb)) * val[q[u].upos];
+            for (int u = 0; u < 4; ++u) {
+                const MergeSrc m = F.msrc[s + u];
+                p[u] = (V[(size_t)m.lpos * kIl] / gesp(V[(size_t)m.dpos * kIl], pert, perturb)) *
+                       V[(size_t)m.upos * kIl];
             }
 #pragma unroll
-            for (int m = 0; m < K; ++m)
-#pragma unroll
-                for (int u = 0; u < 4; ++u) v[m] = v[m] + (-p[m][u]);
+            for (int u = 0; u < 4; ++u) v = v + (-p[u]);
         }
         for (; s < s1; ++s) {
-            const MergeSrc q = F.msrc[s];
-#pragma unroll
-            for (int m = 0; m < K; ++m) {
-                const float *val = lu + m * stride;
-                v[m] = v[m] + (-((val[q.lpos] / gesp(val[q.dpos], pert[m], perturb)) * val[q.upos]));
-            }
+            const MergeSrc m = F.msrc[s];
+            v = v + (-((V[(size_t)m.lpos * kIl] / gesp(V[(size_t)m.dpos * kIl], pert, perturb)) *
+                       V[(size_t)m.upos * kIl]));
         }
     }
-#pragma unroll
-    for (int m = 0; m < K; ++m) lu[m * stride + r0.x] = v[m];
+    V[(size_t)r0.x * kIl] = v;
 }
 
-// lds: this wave's K x 64 floats (256-byte aligned slices).
-template <int K>
-__device__ __forceinline__ void factor_merge_wave_k(const FactorDev &F, float *__restrict__ lu, size_t stride,
-                                                    uint32_t e, const float *pert, int perturb, uint32_t lane,
-                                                    float *lds)
+// Long merge entry e by one wave for the kIl matrices of a group: lane
+// (slot, j) = (lane / kIl, lane % kIl) computes matrix j's products of
+// sources base + 4 u + slot (kIlItems sources a pass, kU passes issued
+// together), and every lane subtracts matrix j's products in source order,
+// taking them from the slot lanes with a permute: factor_merge_wave's
+// arithmetic (sources past the end give +0, and v - 0 is v).
+__device__ __forceinline__ void il_merge_long(const FactorDev &F, float *__restrict__ V, uint32_t e, float pert,
+                                              int perturb, bool valid, uint32_t lane)
 {
+    constexpr uint32_t kU = 4;
     const uint32_t t = F.ment[e];
     const uint32_t s0 = F.msrc_ptr[e], s1 = F.msrc_ptr[e + 1];
-    float v[K];
+    const uint32_t slot = lane / kIl, j = lane % kIl;
+    float v = valid ? V[(size_t)t * kIl] : 0.f;
+    for (uint32_t base = s0; base < s1; base += kIlItems * kU) {
+        float p[kU];
 #pragma unroll
-    for (int m = 0; m < K; ++m) v[m] = lu[m * stride + t];
-    for (uint32_t base = s0; base < s1; base += kWave) {
-        const uint32_t s = base + lane;
-        float p[K];
-#pragma unroll
-        for (int m = 0; m < K; ++m) p[m] = 0.f;
-        if (s < s1) {
-            const MergeSrc q = F.msrc[s];
-#pragma unroll
-            for (int m = 0; m < K; ++m) {
-                const float *val = lu + m * stride;
-                p[m] = (val[q.lpos] / gesp(val[q.dpos], pert[m], perturb)) * val[q.upos];
+        for (uint32_t u = 0; u < kU; ++u) {
+            const uint32_t s = base + u * kIlItems + slot;
+            p[u] = 0.f;
+            if (valid && s < s1) {
+                const MergeSrc m = F.msrc[s];
+                p[u] = (V[(size_t)m.lpos * kIl] / gesp(V[(size_t)m.dpos * kIl], pert, perturb)) *
+                       V[(size_t)m.upos * kIl];
             }
         }
 #pragma unroll
-        for (int m = 0; m < K; ++m) lds[m * kWave + lane] = p[m];
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        if (lane == 0)
+        for (uint32_t u = 0; u < kU; ++u)
 #pragma unroll
-            for (int m = 0; m < K; ++m) v[m] = serial_sub(v[m], lds + m * kWave, kWave, false);
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            for (uint32_t k = 0; k < kIlItems; ++k) {
+                const int src = (int)((k * kIl + j) * 4);
+                v = v - __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(p[u])));
+            }
     }
-    if (lane == 0)
-#pragma unroll
-        for (int m = 0; m < K; ++m) lu[m * stride + t] = v[m];
+    if (valid && slot == 0) V[(size_t)t * kIl] = v;
 }
 
-// One level for every matrix of a same-pattern batch (matrix m: values
-// lu + m nnz, GESP value pertd[2 m] as a double).  Block b takes lane chunk
-// b / groups of the level for matrices [K g, K g + K), g = b % groups:
-// the blocks that share a chunk's table data are dispatched side by side,
-// so the tables come from L2 rather than once per matrix, and each lane
-// loads its table data once for its K matrices.
-template <int K>
-__global__ void __launch_bounds__(256) k_factor_grid_batch(FactorDev F, float *__restrict__ lu,
-                                                           const double *__restrict__ pertd, int lvl, int perturb,
-                                                           uint32_t groups)
+// One step for every matrix of a same-pattern batch over the interleaved
+// copy S (GESP value of matrix m in pertd[2 m] as a double).  Block b takes
+// wave chunk b / groups of the step for group g = b % groups, so the blocks
+// that share a chunk's table data are dispatched side by side (the tables
+// come from L2, not once per group).  The step's first ml waves take its
+// long merges, the rest its column-task items, then its short merges,
+// kIlItems a wave.
+__global__ void __launch_bounds__(256) k_factor_grid_il(FactorDev F, float *__restrict__ S,
+                                                        const double *__restrict__ pertd, int lvl, int perturb,
+                                                        uint32_t groups, uint32_t batch)
 {
-    __shared__ __attribute__((aligned(256))) float lds[256 / kWave][K * kWave];
     const uint32_t g = blockIdx.x % groups, chunk = blockIdx.x / groups;
-    const uint32_t tid = chunk * blockDim.x + threadIdx.x, nthreads = (gridDim.x / groups) * blockDim.x;
-    const uint32_t lane = tid & (kWave - 1), w = tid / kWave, nw = nthreads / kWave;
-    const size_t stride = F.nnz;
-    float *base = lu + (size_t)g * K * stride;
-    float pert[K];
-#pragma unroll
-    for (int m = 0; m < K; ++m) pert[m] = perturb ? (float)pertd[2 * ((size_t)g * K + m)] : 0.f;
+    const uint32_t lane = threadIdx.x % kWave, j = lane % kIl;
+    const uint32_t gw = chunk * (blockDim.x / kWave) + threadIdx.x / kWave;
+    const uint32_t m = g * kIl + j;
+    const bool valid = m < batch;
+    float *V = S + (size_t)g * F.nnz * kIl + j;
+    const float pert = perturb && valid ? (float)pertd[2 * m] : 0.f;
     const LevelSpan s = level_span(F, lvl);
-    for (uint32_t e = w; e < s.ml; e += nw)
-        factor_merge_wave_k<K>(F, base, stride, s.m0 + e, pert, perturb, lane, lds[threadIdx.x / kWave]);
-    for (uint32_t t = s.c0 + w; t < s.c0 + s.nc; t += nw) {
-        CTab c;
-        ctab_load(F, t, lane, c);
-        ctab_run_k<K>(base, stride, c, pert, perturb);
+    if (gw < s.ml) {
+        il_merge_long(F, V, s.m0 + gw, pert, perturb, valid, lane);
+        return;
     }
-    for (uint32_t t = s.ml + tid; t < s.nm; t += nthreads) factor_merge_k<K>(F, base, stride, s.m0 + t, pert, perturb);
+    if (!valid) return;
+    const uint32_t q = (gw - s.ml) * kIlItems + lane / kIl, nct = s.nc * kWave;
+    if (q < nct) il_ctask(F, V, s.c0, q, pert, perturb);
+    else if (q - nct < s.nm - s.ml) il_merge(F, V, s.m0 + s.ml + (q - nct), pert, perturb);
+}
+
+// S = 0 (float4 stores; count4 = floats / 4).
+__global__ void k_il_zero(float4 *__restrict__ S, size_t count4)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count4; i += (size_t)gridDim.x * blockDim.x)
+        S[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// The scaled entries of A into S (wg_values' step 1, interleaved): thread
+// (k, j) of group blockIdx.y places entry k of matrix kIl g + j.
+__global__ void k_il_scatter(RefineDev R, const double *__restrict__ ax, float *__restrict__ S, uint32_t nnz,
+                             uint32_t batch)
+{
+    const uint32_t g = blockIdx.y, i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t k = i / kIl, j = i % kIl, m = g * kIl + j;
+    if (k >= R.nnz_a || m >= batch) return;
+    S[((size_t)g * nnz + R.a_dst[k]) * kIl + j] = (float)((ax[(size_t)m * R.nnz_a + k] * R.a_rs[k]) * R.a_cs[k]);
+}
+
+// GESP: bits[m] = max over columns of sum_p |val[p]| (FP32, column order)
+// for the matrices of group blockIdx.y, then k_il_pert_finish.
+__global__ void k_il_pert(const uint32_t *__restrict__ col_ptr, const float *__restrict__ S, uint32_t n,
+                          uint32_t nnz, uint32_t batch, unsigned int *__restrict__ bits)
+{
+    const uint32_t g = blockIdx.y, i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t c = i / kIl, j = i % kIl, m = g * kIl + j;
+    if (c >= n || m >= batch) return;
+    const float *V = S + (size_t)g * nnz * kIl + j;
+    float s = 0.f;
+    for (uint32_t p = col_ptr[c]; p < col_ptr[c + 1]; ++p) s += fabsf(V[(size_t)p * kIl]);
+    if (s == s) atomicMax(bits + m, __float_as_uint(s));
+}
+
+// pertd[2 m] = the GESP value; re-arms bits for the next call.
+__global__ void k_il_pert_finish(unsigned int *__restrict__ bits, double *__restrict__ pertd, uint32_t batch)
+{
+    const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= batch) return;
+    pertd[2 * m] = (double)(kPertFactor * __uint_as_float(bits[m]));
+    bits[m] = 0u;
+}
+
+// S (interleaved) -> lu (matrix-contiguous), 256 positions of one group a
+// block through LDS: whole-line reads and writes.
+__global__ void __launch_bounds__(256) k_il_to_rows(const float *__restrict__ S, float *__restrict__ lu,
+                                                    uint32_t nnz, uint32_t batch)
+{
+    __shared__ float tile[kIl][256 + 1];
+    const uint32_t g = blockIdx.y, p0 = blockIdx.x * 256, tid = threadIdx.x;
+    const float *src = S + ((size_t)g * nnz + p0) * kIl;
+    for (uint32_t i = tid; i < 256 * kIl; i += 256) {
+        const uint32_t p = i / kIl;
+        if (p0 + p < nnz) tile[i % kIl][p] = src[i];
+    }
+    __syncthreads();
+    if (p0 + tid >= nnz) return;
+    for (uint32_t j = 0; j < kIl && g * kIl + j < batch; ++j)
+        lu[(size_t)(g * kIl + j) * nnz + p0 + tid] = tile[j][tid];
+}
+
+// |b|_inf of matrix blockIdx.x with x = 0 and r64 = b (wg_values' step 1
+// for the right-hand side): norms[2 m + 1].
+__global__ void __launch_bounds__(1024) k_batch_rhs(const double *__restrict__ b, double *__restrict__ x,
+                                                    double *__restrict__ r64, double *__restrict__ norms, uint32_t n)
+{
+    __shared__ double sh[1024 / kWave];
+    const size_t m = blockIdx.x;
+    double bmax = 0.0;
+    for (uint32_t c = threadIdx.x; c < n; c += blockDim.x) {
+        const double v = b[m * n + c];
+        bmax = nmax(bmax, fabs(v));
+        x[m * n + c] = 0.0;
+        r64[m * n + c] = v;
+    }
+    const double bnorm = wg_max(bmax, sh);
+    if (threadIdx.x == 0) norms[2 * m + 1] = bnorm;
 }
 
 // A run of narrow levels inside one workgroup, a barrier between levels.
@@ -615,7 +664,7 @@ __device__ __forceinline__ void factor_window(const FactorDev &F, const WinDev &
 // Steps l0.. of one factorization by one workgroup: windows in LDS, the
 // steps between them in global memory; l0 is 0 or lies at or before the
 // first window's first step (the steps before it ran as
-// k_factor_grid_batch launches).  The divisions L(i,k) /= U(k,k) of every
+// k_factor_grid_il launches).  The divisions L(i,k) /= U(k,k) of every
 // column run after the last step, all at once: no step reads a divided L
 // value (updates divide on the fly) and no step writes L(:,k) or U(k,k)
 // after column k's step, so the values are those of the per-step order,
@@ -1777,9 +1826,9 @@ __device__ __forceinline__ void wg_values(const MatDesc *Md, const MatIO &io, in
     }
 }
 
-// Steps 1-4 (l0 = 0), or steps l0.. of step 3 and step 4 after
-// k_batch_values and the k_factor_grid_batch launches of steps [0, l0)
-// (GESP value in io.norms[0], |b|_inf in io.norms[1]).
+// Steps 1-4 (l0 = 0), or steps l0.. of step 3 and step 4 after the
+// interleaved launches of steps [0, l0) (k_factor_grid_il; GESP value in
+// io.norms[0], |b|_inf in io.norms[1]).
 template <bool kCopy>
 __device__ __forceinline__ void wg_factor(const MatDesc *Md, const MatIO &io, int perturb, double *sh, float *wlds,
                                           int l0)
@@ -2144,7 +2193,7 @@ __device__ __forceinline__ MatIO batch_io(const MatDesc *desc, const double *ax,
 }
 
 // Dynamic LDS: the largest LDS window's values.  l0 > 0: steps [0, l0) ran
-// as k_batch_values + k_factor_grid_batch launches.
+// as interleaved grid launches (k_factor_grid_il).
 template <bool kCopy>
 __global__ void __launch_bounds__(1024) k_batch_same_factor(const MatDesc *__restrict__ desc,
                                                             const double *__restrict__ ax, const double *__restrict__ b,
@@ -2160,37 +2209,10 @@ __global__ void __launch_bounds__(1024) k_batch_same_factor(const MatDesc *__res
     wg_factor<kCopy>(desc, io, perturb, sh, wl[threadIdx.x / kWave], l0);
 }
 
-// Steps 1-2 of wg_factor for matrix blockIdx.x of a same-pattern batch
-// ahead of k_factor_grid_batch: norms[2 m] = the GESP value (as a double),
-// norms[2 m + 1] = |b|_inf (b null: refactor only, no right-hand side).
-__global__ void __launch_bounds__(1024) k_batch_values(const MatDesc *__restrict__ desc,
-                                                       const double *__restrict__ ax, const double *__restrict__ b,
-                                                       double *__restrict__ x, float *__restrict__ lu,
-                                                       double *__restrict__ r64, double *__restrict__ norms, int perturb)
-{
-    __shared__ double sh[1024 / kWave];
-    const uint64_t m = blockIdx.x, n = desc->n;
-    MatIO io{};
-    io.ax = glb(ax + m * desc->r.nnz_a);
-    io.lu = glb(lu + m * desc->nnz);
-    if (b) {
-        io.b = glb(b + m * n);
-        io.x = glb(x + m * n);
-        io.r64 = glb(r64 + m * n);
-    }
-    double bnorm;
-    float pert;
-    wg_values(desc, io, perturb, sh, bnorm, pert);
-    if (threadIdx.x == 0) {
-        norms[2 * m] = (double)pert;
-        norms[2 * m + 1] = bnorm;
-    }
-}
-
 // Refresh + factor only (glu_batch_factor): matrix blockIdx.x of a
 // same-pattern batch, values ax + b nnz_a into lu + b nnz.  Dynamic LDS: the
-// largest LDS window's values.  l0 > 0: steps [0, l0) ran as
-// k_batch_values + k_factor_grid_batch launches (GESP values in norms).
+// largest LDS window's values.  l0 > 0: steps [0, l0) ran as interleaved
+// grid launches (k_factor_grid_il; GESP values in norms).
 __global__ void __launch_bounds__(1024) k_batch_factor(const MatDesc *__restrict__ desc,
                                                        const double *__restrict__ ax, float *__restrict__ lu,
                                                        double *__restrict__ norms, int perturb, int l0)
@@ -2638,30 +2660,50 @@ void batch_factor(const MatDesc *desc, int batch, const double *ax, float *lu, d
     k_batch_factor<<<batch, threads, fact_lds, st>>>(desc, ax, lu, norms, perturb, l0);
 }
 
-void batch_values(const MatDesc *desc, int batch, const double *ax, const double *b, double *x, float *lu,
-                  double *r64, double *norms, int perturb, int threads, hipStream_t st)
+// The interleaved copy of a same-pattern batch (groups of kIl matrices,
+// S: groups kIl nnz floats): zero, the scaled entries of A and, with
+// perturb, each matrix's GESP value into pertd[2 m] (bits: batch words,
+// zero on entry and on return).
+void il_values(const RefineDev &R, const FactorDev &F, const double *ax, float *S, unsigned int *bits,
+               double *pertd, uint32_t nnz, uint32_t n, int batch, int perturb, hipStream_t st)
 {
-    k_batch_values<<<batch, threads, 0, st>>>(desc, ax, b, x, lu, r64, norms, perturb);
+    const uint32_t groups = (uint32_t)((batch + kIl - 1) / kIl);
+    const size_t count4 = (size_t)groups * nnz * kIl / 4;
+    k_il_zero<<<(unsigned)std::min<size_t>(blocks_for(count4, 256), 8192), 256, 0, st>>>(
+        reinterpret_cast<float4 *>(S), count4);
+    k_il_scatter<<<dim3(blocks_for((uint64_t)R.nnz_a * kIl, 256), groups), 256, 0, st>>>(R, ax, S, nnz,
+                                                                                           (uint32_t)batch);
+    if (perturb) {
+        k_il_pert<<<dim3(blocks_for((uint64_t)n * kIl, 256), groups), 256, 0, st>>>(F.col_ptr, S, n, nnz,
+                                                                                      (uint32_t)batch, bits);
+        k_il_pert_finish<<<blocks_for(batch, 256), 256, 0, st>>>(bits, pertd, (uint32_t)batch);
+    }
 }
 
-// Matrices per lane of k_factor_grid_batch: the most of 4, 2, 1 that
-// divides the batch.
-int grid_batch_k(int batch)
+// waves: the step's waves for one group (long merges, then the column-task
+// items and short merges kIlItems a wave).
+void factor_grid_il(const FactorDev &F, float *S, const double *pertd, int lvl, uint32_t waves, int batch,
+                    int perturb, hipStream_t st)
 {
-    return batch % 4 == 0 ? 4 : batch % 2 == 0 ? 2 : 1;
+    const uint32_t groups = (uint32_t)((batch + kIl - 1) / kIl);
+    k_factor_grid_il<<<blocks_for(waves, 256 / kWave) * groups, 256, 0, st>>>(F, S, pertd, lvl, perturb, groups,
+                                                                               (uint32_t)batch);
 }
 
-// lanes: the level's lanes for one matrix (column tasks and long merges a
-// wave each, short merges a lane each).
-void factor_grid_batch(const FactorDev &F, float *lu, const double *norms, int lvl, uint32_t lanes, int batch,
-                       int perturb, hipStream_t st)
+void il_to_rows(const float *S, float *lu, uint32_t nnz, int batch, hipStream_t st)
 {
-    const int k = grid_batch_k(batch);
-    const uint32_t groups = (uint32_t)(batch / k), blocks = blocks_for(lanes, 256) * groups;
-    if (k == 4) k_factor_grid_batch<4><<<blocks, 256, 0, st>>>(F, lu, norms, lvl, perturb, groups);
-    else if (k == 2) k_factor_grid_batch<2><<<blocks, 256, 0, st>>>(F, lu, norms, lvl, perturb, groups);
-    else k_factor_grid_batch<1><<<blocks, 256, 0, st>>>(F, lu, norms, lvl, perturb, groups);
+    const uint32_t groups = (uint32_t)((batch + kIl - 1) / kIl);
+    k_il_to_rows<<<dim3(blocks_for(nnz, 256), groups), 256, 0, st>>>(S, lu, nnz, (uint32_t)batch);
 }
+
+void batch_rhs(const double *b, double *x, double *r64, double *norms, uint32_t n, int batch, int threads,
+               hipStream_t st)
+{
+    k_batch_rhs<<<batch, threads, 0, st>>>(b, x, r64, norms, n);
+}
+
+uint32_t il_items_per_wave() { return kIlItems; }
+uint32_t il_group() { return kIl; }
 
 void batch_group(const MatDesc *descs, const void *ios, const uint32_t *order, int count, double tol,
                  int maxit, int perturb, int threads, int mode, size_t fact_lds, size_t ref_lds, hipStream_t st)

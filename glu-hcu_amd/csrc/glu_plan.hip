@@ -54,10 +54,15 @@ void batch_same(const MatDesc *desc, int batch, const double *ax, const double *
                 int l0, hipStream_t st);
 void batch_factor(const MatDesc *desc, int batch, const double *ax, float *lu, double *norms, int perturb,
                   int threads, size_t fact_lds, int l0, hipStream_t st);
-void batch_values(const MatDesc *desc, int batch, const double *ax, const double *b, double *x, float *lu,
-                  double *r64, double *norms, int perturb, int threads, hipStream_t st);
-void factor_grid_batch(const FactorDev &F, float *lu, const double *norms, int lvl, uint32_t lanes, int batch,
-                       int perturb, hipStream_t st);
+void il_values(const RefineDev &R, const FactorDev &F, const double *ax, float *S, unsigned int *bits,
+               double *pertd, uint32_t nnz, uint32_t n, int batch, int perturb, hipStream_t st);
+void factor_grid_il(const FactorDev &F, float *S, const double *pertd, int lvl, uint32_t waves, int batch,
+                    int perturb, hipStream_t st);
+void il_to_rows(const float *S, float *lu, uint32_t nnz, int batch, hipStream_t st);
+void batch_rhs(const double *b, double *x, double *r64, double *norms, uint32_t n, int batch, int threads,
+               hipStream_t st);
+uint32_t il_items_per_wave();
+uint32_t il_group();
 void abft_cca(const FactorDev &F, const float *val, float *cca, int batch, hipStream_t st);
 void abft_check(const FactorDev &F, const uint32_t *row_idx, const uint32_t *diag, const float *lu,
                 const float *cca, float *ccl, float tol, int32_t *first_bad, float *max_diff, int batch,
@@ -332,10 +337,11 @@ struct glu_plan {
     uint32_t copy_range = 0;     // slots the batch kernels' value copy stages in LDS at a time
     std::vector<uint32_t> win_slots;   // values of each window
     // batched kernels: steps [0, batch_grid_lev) run as one grid launch per
-    // step over the batch (grid_lanes: each one's lanes for one matrix)
+    // step over an interleaved copy of the batch (grid_waves: each one's
+    // waves for one group of interleaved matrices)
     int batch_grid_opt = 0;      // glu_plan_opts.batch_grid_levels
     int batch_grid_lev = 0;
-    std::vector<uint32_t> grid_lanes;
+    std::vector<uint32_t> grid_waves;
 
     // hipGraph replay of the launch sequences (use_graph): one executable
     // graph per entry point, keyed on the buffers it was captured with
@@ -375,7 +381,9 @@ struct glu_plan {
     double *d_bkry = nullptr;    // Arnoldi vectors V, Z = M V, W = A Z: (3 krylov + 1) n per matrix
     int batch_cap = 0;
     double *d_fnorms = nullptr;  // glu_batch_factor: GESP value per matrix (2 doubles each)
-    int fnorms_cap = 0;
+    float *d_il = nullptr;       // interleaved batch values (groups of 16 matrices)
+    unsigned int *d_il_bits = nullptr;   // per-matrix GESP maxima, zero between calls
+    int il_cap = 0;              // matrices d_il, d_il_bits and d_fnorms hold
 
     ~glu_plan()
     {
@@ -390,6 +398,8 @@ struct glu_plan {
         if (d_abft) (void)hipFree(d_abft);
         if (d_bnorms) (void)hipFree(d_bnorms);
         if (d_fnorms) (void)hipFree(d_fnorms);
+        if (d_il) (void)hipFree(d_il);
+        if (d_il_bits) (void)hipFree(d_il_bits);
         for (GraphSlot *g : {&g_factor, &g_solve[0], &g_solve[1]})
             if (g->exec) (void)hipGraphExecDestroy(g->exec);
         if (cap_stream) (void)hipStreamDestroy(cap_stream);
@@ -1367,9 +1377,13 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
     {
         const int lim = std::min(wins.empty() ? P.num_lev + 1 : wins[0].l0,
                                  p->batch_grid_opt < 0 ? 0 : p->batch_grid_opt > 0 ? p->batch_grid_opt : INT32_MAX);
-        for (int l = 0; l < lim && level_lanes(T, l) >= (uint32_t)p->fused_threads; ++l)
-            p->grid_lanes.push_back(level_lanes(T, l));
-        p->batch_grid_lev = (int)p->grid_lanes.size();
+        const uint32_t per_wave = launch::il_items_per_wave();
+        for (int l = 0; l < lim && level_lanes(T, l) >= (uint32_t)p->fused_threads; ++l) {
+            const uint32_t ml = T.ment_long[l], nm = T.ment_ptr[l + 1] - T.ment_ptr[l];
+            const uint32_t items = (T.ctask_ptr[l + 1] - T.ctask_ptr[l]) * 64 + (nm - ml);
+            p->grid_waves.push_back(ml + (items + per_wave - 1) / per_wave);
+        }
+        p->batch_grid_lev = (int)p->grid_waves.size();
     }
 
     const uint32_t n = P.n;
@@ -2247,17 +2261,40 @@ extern "C" glu_status glu_adjoint_values(glu_plan *p, const double *d_lam, const
 
 namespace {
 
-// The batched kernels' first steps: each matrix's values, |b|_inf and GESP
-// value (norms: 2 per matrix), then steps [0, batch_grid_lev) as one grid
-// launch each over the whole batch; the per-matrix workgroups go on from
-// there (b null: refactor only).
+// Scratch of the interleaved steps for `batch` matrices (grown on demand;
+// the GESP maxima start at zero and every call leaves them so).
+glu_status ensure_il(glu_plan *p, int batch)
+{
+    if (p->batch_grid_lev == 0 || batch <= p->il_cap) return GLU_OK;
+    for (void *q : {(void *)p->d_il, (void *)p->d_il_bits, (void *)p->d_fnorms})
+        if (q) (void)hipFree(q);
+    p->d_il = nullptr;
+    p->d_il_bits = nullptr;
+    p->d_fnorms = nullptr;
+    p->il_cap = 0;
+    const size_t padded = ((size_t)batch + launch::il_group() - 1) / launch::il_group() * launch::il_group();
+    HIP_TRY(hipMalloc(&p->d_il, sizeof(float) * padded * p->nnz));
+    HIP_TRY(hipMalloc(&p->d_il_bits, sizeof(unsigned int) * (size_t)batch));
+    HIP_TRY(hipMalloc(&p->d_fnorms, 2 * sizeof(double) * (size_t)batch));
+    HIP_TRY(hipMemset(p->d_il_bits, 0, sizeof(unsigned int) * (size_t)batch));
+    p->il_cap = batch;
+    return GLU_OK;
+}
+
+// The batched kernels' first steps: the values of every matrix into the
+// interleaved copy (with each GESP value in norms[2 m]) and, with a
+// right-hand side, |b|_inf into norms[2 m + 1] with x = 0 and r64 = b; steps
+// [0, batch_grid_lev) as one grid launch each; the factors so far into
+// d_lu.  The per-matrix workgroups go on from there.
 void batch_grid_steps(glu_plan *p, int batch, const double *d_ax, const double *d_b, double *d_x, float *d_lu,
                       double *r64, double *norms, int perturb, hipStream_t st)
 {
     if (p->batch_grid_lev == 0) return;
-    launch::batch_values(p->d_desc, batch, d_ax, d_b, d_x, d_lu, r64, norms, perturb, p->fused_threads, st);
+    launch::il_values(p->R, p->F, d_ax, p->d_il, p->d_il_bits, norms, p->nnz, p->n, batch, perturb, st);
+    if (d_b) launch::batch_rhs(d_b, d_x, r64, norms, p->n, batch, p->fused_threads, st);
     for (int l = 0; l < p->batch_grid_lev; ++l)
-        launch::factor_grid_batch(p->F, d_lu, norms, l, p->grid_lanes[l], batch, perturb, st);
+        launch::factor_grid_il(p->F, p->d_il, norms, l, p->grid_waves[l], batch, perturb, st);
+    launch::il_to_rows(p->d_il, d_lu, p->nnz, batch, st);
 }
 
 }  // namespace
@@ -2270,6 +2307,7 @@ extern "C" glu_status glu_batch_solve(glu_plan *p, int batch, const double *d_ax
         return fail(GLU_ERR_ARGUMENT, "bad argument");
     if (!p->has_refine) return fail(GLU_ERR_ARGUMENT, "plan has no analysis");
     glu_status s = ensure_batch(p, batch);
+    if (s == GLU_OK) s = ensure_il(p, batch);
     if (s != GLU_OK) return s;
     const size_t cap = launch::enable_lds();
     const int mode = sweep_mode(p->n, cap);
@@ -2293,13 +2331,8 @@ extern "C" glu_status glu_batch_factor(glu_plan *p, int batch, const double *d_a
     (void)launch::enable_lds();
     const int perturb = (int)(flags & 1u);
     hipStream_t st = static_cast<hipStream_t>(stream);
-    if (p->batch_grid_lev > 0 && batch > p->fnorms_cap) {
-        if (p->d_fnorms) (void)hipFree(p->d_fnorms);
-        p->d_fnorms = nullptr;
-        p->fnorms_cap = 0;
-        HIP_TRY(hipMalloc(&p->d_fnorms, 2 * sizeof(double) * (size_t)batch));
-        p->fnorms_cap = batch;
-    }
+    glu_status s = ensure_il(p, batch);
+    if (s != GLU_OK) return s;
     batch_grid_steps(p, batch, d_ax, nullptr, nullptr, d_lu, nullptr, p->d_fnorms, perturb, st);
     launch::batch_factor(p->d_desc, batch, d_ax, d_lu, p->d_fnorms, perturb, p->fused_threads,
                          sizeof(float) * p->win_nslot, p->batch_grid_lev, st);
