@@ -92,8 +92,11 @@ struct FactorDev {
     // source 0}, mrec[2e + 1] = {L, U, pivot position of source 1, source
     // count}; sources from the third on are read from msrc
     const uint4 *mrec;
-    const uint2 *dvd;          // division items {L position, pivot position | kFirstBit}, in position order
-    const uint32_t *dvd_ptr;   // [num_lev + 1]: dvd_ptr[num_lev] is their count
+    // division items {L position, pivot position | kFirstBit}: those of the
+    // columns of each step before the batched kernels' last grid step, step
+    // by step, then the rest; each step's, and the rest, in position order
+    const uint2 *dvd;
+    const uint32_t *dvd_ptr;   // [num_lev + 1]: items of the columns of steps < l
     int32_t num_lev;
     uint32_t n, nnz;
     // LDS windows: runs of steps that work on values staged in LDS

@@ -420,6 +420,37 @@ __device__ __forceinline__ void il_ctask(const FactorDev &F, float *__restrict__
         if (c.tt[u] != ~0u) V[(size_t)c.tt[u] * kIl] = tv[u] + (-(l * uv[u]));
 }
 
+// Column-task items qa and qb of one matrix, their loads issued together
+// (items of one step are independent: distinct targets, none read by
+// another item).  An unused lane's positions are 0, a valid address.
+__device__ __forceinline__ void il_ctask2(const FactorDev &F, float *__restrict__ V, uint32_t c0, uint32_t qa,
+                                          uint32_t qb, float pert, int perturb)
+{
+    CTab c[2];
+    ctab_load(F, c0 + qa / kWave, qa % kWave, c[0]);
+    ctab_load(F, c0 + qb / kWave, qb % kWave, c[1]);
+    float d[2], a[2], uv[2][4], tv[2][4];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        d[i] = V[(size_t)c[i].dpos * kIl];
+        a[i] = V[(size_t)c[i].lpos * kIl];
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u)
+            if (c[i].tt[u] != ~0u) {
+                uv[i][u] = V[(size_t)c[i].up[u] * kIl];
+                tv[i][u] = V[(size_t)c[i].tt[u] * kIl];
+            }
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        if ((c[i].tt[0] & c[i].tt[1] & c[i].tt[2] & c[i].tt[3]) == ~0u) continue;
+        const float l = a[i] / gesp(d[i], pert, perturb);
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u)
+            if (c[i].tt[u] != ~0u) V[(size_t)c[i].tt[u] * kIl] = tv[i][u] + (-(l * uv[i][u]));
+    }
+}
+
 // Merge entry e of one matrix, values V[p kIl]: factor_merge's arithmetic.
 __device__ __forceinline__ void il_merge(const FactorDev &F, float *__restrict__ V, uint32_t e, float pert,
                                          int perturb)
@@ -490,13 +521,27 @@ __device__ __forceinline__ void il_merge_long(const FactorDev &F, float *__restr
     if (valid && slot == 0) V[(size_t)t * kIl] = v;
 }
 
+// Division item `it` of one matrix, values V[p kIl]: factor_divide's
+// arithmetic.
+__device__ __forceinline__ void il_divide(float *__restrict__ V, uint2 it, float pert, int perturb)
+{
+    const uint32_t dpos = it.y & ~kFirstBit;
+    const float d0 = V[(size_t)dpos * kIl];
+    const bool pt = perturb && fabsf(d0) < pert;
+    V[(size_t)it.x * kIl] = V[(size_t)it.x * kIl] / (pt ? pert : d0);
+    if (pt && (it.y & kFirstBit)) V[(size_t)dpos * kIl] = pert;
+}
+
 // One step for every matrix of a same-pattern batch over the interleaved
 // copy S (GESP value of matrix m in pertd[2 m] as a double).  Block b takes
 // wave chunk b / groups of the step for group g = b % groups, so the blocks
 // that share a chunk's table data are dispatched side by side (the tables
 // come from L2, not once per group).  The step's first ml waves take its
-// long merges, the rest its column-task items, then its short merges,
-// kIlItems a wave.
+// long merges, the rest its column-task items, its short merges, then the
+// division items of the previous step's columns, kIlItems a wave: those
+// columns are final once that step ran, and this step neither reads nor
+// writes them.  A wave takes 2 kIlItems items, the second kIlItems after
+// the first (two column-task items: their loads issued together).
 __global__ void __launch_bounds__(256) k_factor_grid_il(FactorDev F, float *__restrict__ S,
                                                         const double *__restrict__ pertd, int lvl, int perturb,
                                                         uint32_t groups, uint32_t batch)
@@ -514,9 +559,23 @@ __global__ void __launch_bounds__(256) k_factor_grid_il(FactorDev F, float *__re
         return;
     }
     if (!valid) return;
-    const uint32_t q = (gw - s.ml) * kIlItems + lane / kIl, nct = s.nc * kWave;
-    if (q < nct) il_ctask(F, V, s.c0, q, pert, perturb);
-    else if (q - nct < s.nm - s.ml) il_merge(F, V, s.m0 + s.ml + (q - nct), pert, perturb);
+    const uint32_t nct = s.nc * kWave, nsm = s.nm - s.ml;
+    const uint32_t qa = (gw - s.ml) * 2 * kIlItems + lane / kIl, qb = qa + kIlItems;
+    if (qb < nct) {
+        il_ctask2(F, V, s.c0, qa, qb, pert, perturb);
+        return;
+    }
+#pragma unroll 1
+    for (uint32_t q = qa; q <= qb; q += kIlItems) {
+        if (q < nct) {
+            il_ctask(F, V, s.c0, q, pert, perturb);
+        } else if (q - nct < nsm) {
+            il_merge(F, V, s.m0 + s.ml + (q - nct), pert, perturb);
+        } else if (lvl > 0) {
+            const uint32_t d = F.dvd_ptr[lvl - 1] + (q - nct - nsm);
+            if (d < F.dvd_ptr[lvl]) il_divide(V, F.dvd[d], pert, perturb);
+        }
+    }
 }
 
 // S = 0 (float4 stores; count4 = floats / 4).
@@ -526,15 +585,26 @@ __global__ void k_il_zero(float4 *__restrict__ S, size_t count4)
         S[i] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
-// The scaled entries of A into S (wg_values' step 1, interleaved): thread
-// (k, j) of group blockIdx.y places entry k of matrix kIl g + j.
-__global__ void k_il_scatter(RefineDev R, const double *__restrict__ ax, float *__restrict__ S, uint32_t nnz,
-                             uint32_t batch)
+// The scaled entries of A into S (wg_values' step 1, interleaved): block
+// (x, g) takes entries [64 x, 64 x + 64) of the kIl matrices of group g,
+// read matrix by matrix (whole lines of ax), scaled, and turned in LDS so
+// that each entry's kIl values go out as one line.
+__global__ void __launch_bounds__(256) k_il_scatter(RefineDev R, const double *__restrict__ ax,
+                                                    float *__restrict__ S, uint32_t nnz, uint32_t batch)
 {
-    const uint32_t g = blockIdx.y, i = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t k = i / kIl, j = i % kIl, m = g * kIl + j;
-    if (k >= R.nnz_a || m >= batch) return;
-    S[((size_t)g * nnz + R.a_dst[k]) * kIl + j] = (float)((ax[(size_t)m * R.nnz_a + k] * R.a_rs[k]) * R.a_cs[k]);
+    constexpr uint32_t kK = 64;
+    __shared__ float tile[kK][kIl + 1];
+    const uint32_t g = blockIdx.y, k0 = blockIdx.x * kK, tid = threadIdx.x;
+    for (uint32_t i = tid; i < kK * kIl; i += blockDim.x) {
+        const uint32_t j = i / kK, kk = i % kK, k = k0 + kk, m = g * kIl + j;
+        if (k < R.nnz_a && m < batch)
+            tile[kk][j] = (float)((ax[(size_t)m * R.nnz_a + k] * R.a_rs[k]) * R.a_cs[k]);
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < kK * kIl; i += blockDim.x) {
+        const uint32_t kk = i / kIl, j = i % kIl, k = k0 + kk;
+        if (k < R.nnz_a && g * kIl + j < batch) S[((size_t)g * nnz + R.a_dst[k]) * kIl + j] = tile[kk][j];
+    }
 }
 
 // GESP: bits[m] = max over columns of sum_p |val[p]| (FP32, column order)
@@ -567,10 +637,16 @@ __global__ void __launch_bounds__(256) k_il_to_rows(const float *__restrict__ S,
 {
     __shared__ float tile[kIl][256 + 1];
     const uint32_t g = blockIdx.y, p0 = blockIdx.x * 256, tid = threadIdx.x;
-    const float *src = S + ((size_t)g * nnz + p0) * kIl;
-    for (uint32_t i = tid; i < 256 * kIl; i += 256) {
-        const uint32_t p = i / kIl;
-        if (p0 + p < nnz) tile[i % kIl][p] = src[i];
+    const float4 *src = reinterpret_cast<const float4 *>(S + ((size_t)g * nnz + p0) * kIl);
+    for (uint32_t f = tid; f < 256 * kIl / 4; f += 256) {   // a position's kIl values: kIl / 4 float4
+        const uint32_t p = f / (kIl / 4), j = (f % (kIl / 4)) * 4;
+        if (p0 + p < nnz) {
+            const float4 v = src[f];
+            tile[j][p] = v.x;
+            tile[j + 1][p] = v.y;
+            tile[j + 2][p] = v.z;
+            tile[j + 3][p] = v.w;
+        }
     }
     __syncthreads();
     if (p0 + tid >= nnz) return;
@@ -683,10 +759,11 @@ __device__ __forceinline__ void factor_all_wg(const FactorDev &F, float *__restr
     // the divisions, eight items a lane at a time (their loads issued
     // together); items are independent: an item writes its L entry and, for
     // a perturbed pivot, the pivot, which the column's other items divide
-    // by either way (see factor_divide)
+    // by either way (see factor_divide).  Those of the columns of steps
+    // before l0 - 1 ran with the interleaved steps (k_factor_grid_il).
     const uint32_t nd = F.dvd_ptr[F.num_lev], tid = threadIdx.x, nt = blockDim.x;
     constexpr int U = 8;
-    for (uint32_t t0 = tid; t0 < nd; t0 += U * nt) {
+    for (uint32_t t0 = (l0 > 0 ? F.dvd_ptr[l0 - 1] : 0u) + tid; t0 < nd; t0 += U * nt) {
         uint2 it[U];
         float d[U], x[U];
 #pragma unroll
@@ -2671,8 +2748,7 @@ void il_values(const RefineDev &R, const FactorDev &F, const double *ax, float *
     const size_t count4 = (size_t)groups * nnz * kIl / 4;
     k_il_zero<<<(unsigned)std::min<size_t>(blocks_for(count4, 256), 8192), 256, 0, st>>>(
         reinterpret_cast<float4 *>(S), count4);
-    k_il_scatter<<<dim3(blocks_for((uint64_t)R.nnz_a * kIl, 256), groups), 256, 0, st>>>(R, ax, S, nnz,
-                                                                                           (uint32_t)batch);
+    k_il_scatter<<<dim3(blocks_for(R.nnz_a, 64), groups), 256, 0, st>>>(R, ax, S, nnz, (uint32_t)batch);
     if (perturb) {
         k_il_pert<<<dim3(blocks_for((uint64_t)n * kIl, 256), groups), 256, 0, st>>>(F.col_ptr, S, n, nnz,
                                                                                       (uint32_t)batch, bits);

@@ -338,7 +338,8 @@ struct glu_plan {
     std::vector<uint32_t> win_slots;   // values of each window
     // batched kernels: steps [0, batch_grid_lev) run as one grid launch per
     // step over an interleaved copy of the batch (grid_waves: each one's
-    // waves for one group of interleaved matrices)
+    // waves for one group of interleaved matrices: long merges one each,
+    // other items two per lane)
     int batch_grid_opt = 0;      // glu_plan_opts.batch_grid_levels
     int batch_grid_lev = 0;
     std::vector<uint32_t> grid_waves;
@@ -803,10 +804,6 @@ bool build_factor_tables(const glu_pattern &P, Tables &T, int64_t &updates, std:
     T.ctask_ptr[nl] = T.ctask_ptr[nl + 1] = (uint32_t)(T.cln.size() / 64);
     T.ment_ptr[nl] = T.ment_ptr[nl + 1] = (uint32_t)T.ment.size();
     T.dvd_ptr[nl] = (uint32_t)T.dvd.size();
-    // The divisions all run after the last step, in one pass; in position
-    // order a wave's items are consecutive L entries of a column (each
-    // column's first entry first), so the pass reads and writes whole lines.
-    std::sort(T.dvd.begin(), T.dvd.end(), [](const uint2 &a, const uint2 &b) { return a.x < b.x; });
     if (T.ctgt.size() >= UINT32_MAX || T.msrc.size() >= UINT32_MAX) {
         err = "more than 2^32 update items";
         return false;
@@ -1375,15 +1372,29 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
     // window with at least a workgroup's worth of lanes per matrix (a
     // narrower step takes one pass of the per-matrix workgroup anyway).
     {
-        const int lim = std::min(wins.empty() ? P.num_lev + 1 : wins[0].l0,
-                                 p->batch_grid_opt < 0 ? 0 : p->batch_grid_opt > 0 ? p->batch_grid_opt : INT32_MAX);
+        const int lim = std::min({wins.empty() ? P.num_lev : std::min(wins[0].l0, P.num_lev),
+                                  p->batch_grid_opt < 0 ? 0 : p->batch_grid_opt > 0 ? p->batch_grid_opt : INT32_MAX});
         const uint32_t per_wave = launch::il_items_per_wave();
         for (int l = 0; l < lim && level_lanes(T, l) >= (uint32_t)p->fused_threads; ++l) {
             const uint32_t ml = T.ment_long[l], nm = T.ment_ptr[l + 1] - T.ment_ptr[l];
-            const uint32_t items = (T.ctask_ptr[l + 1] - T.ctask_ptr[l]) * 64 + (nm - ml);
-            p->grid_waves.push_back(ml + (items + per_wave - 1) / per_wave);
+            const uint32_t items = (T.ctask_ptr[l + 1] - T.ctask_ptr[l]) * 64 + (nm - ml) +
+                                   (l > 0 ? T.dvd_ptr[l] - T.dvd_ptr[l - 1] : 0u);   // + previous step's divisions
+            p->grid_waves.push_back(ml + (items + 2 * per_wave - 1) / (2 * per_wave));
         }
         p->batch_grid_lev = (int)p->grid_waves.size();
+    }
+    // Division items: those of a column whose step is one of the batched
+    // kernels' grid steps but the last run with the next grid step, the
+    // rest in one pass after the last step (the single path does all at
+    // the end).  In position order a wave's items are consecutive L entries
+    // of a column (each column's first entry first), so they read and write
+    // whole lines.
+    {
+        const auto by_pos = [](const uint2 &a, const uint2 &b) { return a.x < b.x; };
+        const int early = std::max(p->batch_grid_lev - 1, 0);
+        for (int l = 0; l < early; ++l)
+            std::sort(T.dvd.begin() + T.dvd_ptr[l], T.dvd.begin() + T.dvd_ptr[l + 1], by_pos);
+        std::sort(T.dvd.begin() + T.dvd_ptr[early], T.dvd.end(), by_pos);
     }
 
     const uint32_t n = P.n;
