@@ -3,22 +3,24 @@
 
 Workload: 256 MNA matrices sharing one n=20,000 (~112k nnz) pattern, values
 = base * (1 + U[-0.1, 0.1]) per entry, one RHS each; a step is one batched
-solve (two dependent launches, one workgroup per matrix): refresh the values
-into the LU pattern (frozen MC64 scales and permutation), factor in FP32,
-then solve in FP32 and refine in FP64 (FGMRES preconditioned by the FP32
-factors) until the normwise backward error is <= 1e-10.  Inputs are
-resident on the GPU before the timed region.
+solve: refresh the values into the LU pattern (frozen MC64 scales and
+permutation), factor in FP32 (the wide leading steps as grid launches over
+an interleaved copy of the batch, 16 matrices per 64-byte line, the rest in
+one workgroup per matrix), then solve in FP32 and refine in FP64 (FGMRES
+preconditioned by the FP32 factors, one workgroup per matrix) until the
+normwise backward error is <= 1e-10.  Inputs are resident on the GPU before
+the timed region.
 
 Reported beside it (same JSON line):
 * the rate of looping the plain FP32 drop-in path (single-matrix
   level-scheduled factor + refinement) over the same matrices;
-* the batched refactor alone (refresh + factor, one launch);
+* the batched refactor alone (refresh + factor, no solve);
 * the batch rate including the host-to-device copy of values and RHS;
 * the plain FP32 factor + solve of the n=4,960 MNA matrix;
 * the single-matrix numeric refactor time of the n=500,000 matrix;
 * the uneven group (64 distinct patterns, n in [1k, 50k]) in one launch;
 * forward + backward of the differentiable solve at n=50,000;
-* the batch kernel's HBM roofline fraction;
+* the batch step's HBM roofline fraction;
 * the reference NICSLU CPU refactor rate on the host cores.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W]
@@ -577,7 +579,7 @@ def main():
     fb = _events(lambda: plan.batch_factor(ax, lu=lu), 11)
     log(f"[bench] batched refactor (refresh + factor, no solve): {fb:.3f} ms -> {B / fb * 1e3:.1f} refactors/s")
 
-    # ---------------- roofline of the batch kernel ----------------------
+    # ---------------- roofline of the batch step ------------------------
     per_mat = [alg_bytes_factor(an.nnz_lu, n, B) + int(it) * alg_bytes_refine(data.size, an.nnz_lu, n, B)
                for it in iters_h]
     traffic_alg = float(np.sum(per_mat))
@@ -623,7 +625,8 @@ def main():
         "config": {"workload": f"batched same-pattern refactor + FP64 refine: {B} MNA matrices, n={n}, "
                                f"nnz(A)={data.size}, nnz(L+U)={an.nnz_lu}, {an.num_lev} levels, tol {args.tol:g}",
                    "batch": B, "n": n, "nnz": int(data.size), "nnz_lu": an.nnz_lu, "levels": an.num_lev,
-                   "parallelism": "one workgroup per matrix"},
+                   "parallelism": f"{plan.batch_grid_levels} wide steps as grid launches over 16-matrix "
+                                  f"interleaved groups, then one workgroup per matrix"},
         "correct": ok,
         "refine_iters_max": int(iters_h.max()),
         "eta_max": float(eta_h.max()),
@@ -639,7 +642,8 @@ def main():
         "autograd": grad,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
-                     "kernel": "k_batch_same_factor + k_batch_same_refine (one batch step)",
+                     "kernel": "one batch step: k_il_zero, k_il_scatter, k_batch_rhs, k_factor_grid_il per wide "
+                               "step, k_il_to_rows, k_batch_same_factor, k_batch_same_refine",
                      "kernel_ms": round(kernel_ms, 4), "alg_bytes": round(traffic_alg)},
         "cpu_baseline": cpu,
     }

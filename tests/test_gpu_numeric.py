@@ -386,7 +386,8 @@ def test_batch_grid_steps_same_bits(torch_mod, B, perturb):
     """The batched kernels' leading steps as grid launches over an
     interleaved copy of the batch (groups of 16 matrices; 40 = two full
     groups and a partial one) give the same bits as the workgroup-per-matrix
-    kernel alone: factors, solutions, iteration counts and backward errors of
+    kernel alone (one, three and the default number of grid steps):
+    factors, solutions, iteration counts and backward errors of
     glu_batch_solve, factors of glu_batch_factor, and the single path's
     factors."""
     from pyglu import mna
@@ -397,9 +398,9 @@ def test_batch_grid_steps_same_bits(torch_mod, B, perturb):
     bd = torch.from_numpy(np.stack([mna.rhs(n, 5300 + b) for b in range(B)])).cuda()
     assert plan.batch_grid_levels > 0
     runs = {}
-    for opt in (-1, 0, 3):
+    for opt in (-1, 0, 1, 3):
         pl = plan if opt == 0 else Plan(an, batch_grid_levels=opt)
-        assert pl.batch_grid_levels == {-1: 0, 0: plan.batch_grid_levels, 3: 3}[opt]
+        assert pl.batch_grid_levels == {-1: 0, 0: plan.batch_grid_levels, 1: 1, 3: 3}[opt]
         print(f"  B={B} perturb={perturb} batch_grid_levels={pl.batch_grid_levels}", flush=True)
         x, lu, iters, eta = pl.batch_solve(axd, bd, tol=1e-10, maxit=20, perturb=perturb)
         luf = pl.batch_factor(axd, perturb=perturb)
@@ -407,7 +408,7 @@ def test_batch_grid_steps_same_bits(torch_mod, B, perturb):
         runs[opt] = [t.clone() for t in (x, lu, iters, eta, luf)]
         if pl is not plan:
             pl.close()
-    for opt in (0, 3):
+    for opt in (0, 1, 3):
         for a, b in zip(runs[-1], runs[opt]):
             assert torch.equal(a.view(torch.uint8), b.view(torch.uint8)), opt
     assert torch.equal(runs[0][1].view(torch.int32), runs[0][4].view(torch.int32))
