@@ -429,17 +429,18 @@ __device__ __forceinline__ void il_ctask2(const FactorDev &F, float *__restrict_
     CTab c[2];
     ctab_load(F, c0 + qa / kWave, qa % kWave, c[0]);
     ctab_load(F, c0 + qb / kWave, qb % kWave, c[1]);
+    // every load without a branch (an empty slot's U position is 0 and its
+    // target is read at 0, valid addresses), so all are issued together
     float d[2], a[2], uv[2][4], tv[2][4];
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
         d[i] = V[(size_t)c[i].dpos * kIl];
         a[i] = V[(size_t)c[i].lpos * kIl];
 #pragma unroll
-        for (uint32_t u = 0; u < 4; ++u)
-            if (c[i].tt[u] != ~0u) {
-                uv[i][u] = V[(size_t)c[i].up[u] * kIl];
-                tv[i][u] = V[(size_t)c[i].tt[u] * kIl];
-            }
+        for (uint32_t u = 0; u < 4; ++u) {
+            uv[i][u] = V[(size_t)c[i].up[u] * kIl];
+            tv[i][u] = V[(size_t)(c[i].tt[u] != ~0u ? c[i].tt[u] : 0u) * kIl];
+        }
     }
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
@@ -486,29 +487,36 @@ __device__ __forceinline__ void il_merge(const FactorDev &F, float *__restrict__
 
 // Long merge entry e by one wave for the kIl matrices of a group: lane
 // (slot, j) = (lane / kIl, lane % kIl) computes matrix j's products of
-// sources base + 4 u + slot (kIlItems sources a pass, kU passes issued
-// together), and every lane subtracts matrix j's products in source order,
-// taking them from the slot lanes with a permute: factor_merge_wave's
-// arithmetic (sources past the end give +0, and v - 0 is v).
+// sources base + 4 u + slot (kIlItems sources a pass, kU passes), and every
+// lane subtracts matrix j's products in source order, taking them from the
+// slot lanes with a permute: factor_merge_wave's arithmetic (sources past
+// the end give +0, and v - 0 is v).  The loads take no branch (past the
+// end: the last source again; a padding matrix's lanes read its padding
+// values), so the kU passes' loads are issued together.
 __device__ __forceinline__ void il_merge_long(const FactorDev &F, float *__restrict__ V, uint32_t e, float pert,
                                               int perturb, bool valid, uint32_t lane)
 {
     constexpr uint32_t kU = 4;
     const uint32_t t = F.ment[e];
-    const uint32_t s0 = F.msrc_ptr[e], s1 = F.msrc_ptr[e + 1];
+    const uint32_t s0 = F.msrc_ptr[e], s1 = F.msrc_ptr[e + 1];   // s1 > s0 + kLongMerge
     const uint32_t slot = lane / kIl, j = lane % kIl;
-    float v = valid ? V[(size_t)t * kIl] : 0.f;
+    float v = V[(size_t)t * kIl];
     for (uint32_t base = s0; base < s1; base += kIlItems * kU) {
+        MergeSrc m[kU];
+#pragma unroll
+        for (uint32_t u = 0; u < kU; ++u) m[u] = F.msrc[min(base + u * kIlItems + slot, s1 - 1)];
+        float lv[kU], dv[kU], uv[kU];
+#pragma unroll
+        for (uint32_t u = 0; u < kU; ++u) {
+            lv[u] = V[(size_t)m[u].lpos * kIl];
+            dv[u] = V[(size_t)m[u].dpos * kIl];
+            uv[u] = V[(size_t)m[u].upos * kIl];
+        }
         float p[kU];
 #pragma unroll
         for (uint32_t u = 0; u < kU; ++u) {
-            const uint32_t s = base + u * kIlItems + slot;
-            p[u] = 0.f;
-            if (valid && s < s1) {
-                const MergeSrc m = F.msrc[s];
-                p[u] = (V[(size_t)m.lpos * kIl] / gesp(V[(size_t)m.dpos * kIl], pert, perturb)) *
-                       V[(size_t)m.upos * kIl];
-            }
+            const float q = (lv[u] / gesp(dv[u], pert, perturb)) * uv[u];
+            p[u] = base + u * kIlItems + slot < s1 ? q : 0.f;
         }
 #pragma unroll
         for (uint32_t u = 0; u < kU; ++u)
