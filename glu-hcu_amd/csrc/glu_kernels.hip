@@ -128,14 +128,15 @@ __device__ __forceinline__ void ctab_run(const FactorDev &F, float *__restrict__
                                          int perturb)
 {
     if ((c.tt[0] & c.tt[1] & c.tt[2] & c.tt[3]) == ~0u) return;   // nothing to update
+    // every load without a branch (an empty slot's U position is 0 and its
+    // target is read at 0, valid positions), so that they issue together
     const float d = val[c.dpos], a = val[c.lpos];
     float uv[4], tv[4];
 #pragma unroll
-    for (uint32_t u = 0; u < 4; ++u)
-        if (c.tt[u] != ~0u) {
-            uv[u] = val[c.up[u]];
-            tv[u] = val[c.tt[u]];
-        }
+    for (uint32_t u = 0; u < 4; ++u) {
+        uv[u] = val[c.up[u]];
+        tv[u] = val[c.tt[u] != ~0u ? c.tt[u] : 0u];
+    }
     const float l = a / gesp(d, pert, perturb);
 #pragma unroll
     for (uint32_t u = 0; u < 4; ++u)
@@ -163,12 +164,13 @@ __device__ __forceinline__ void factor_merge_rest(const FactorDev &F, float *__r
     const uint32_t s1 = F.msrc_ptr[e + 1];
     uint32_t s = F.msrc_ptr[e] + 2;
     for (; s + 4 <= s1; s += 4) {
+        MergeSrc m[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) m[u] = F.msrc[s + u];
+        __builtin_amdgcn_sched_barrier(0);   // the four table loads issue before any value load
         float p[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const MergeSrc m = F.msrc[s + u];
-            p[u] = (val[m.lpos] / gesp(val[m.dpos], pert, perturb)) * val[m.upos];
-        }
+        for (int u = 0; u < 4; ++u) p[u] = (val[m[u].lpos] / gesp(val[m[u].dpos], pert, perturb)) * val[m[u].upos];
 #pragma unroll
         for (int u = 0; u < 4; ++u) v = v + (-p[u]);
     }
@@ -409,11 +411,10 @@ __device__ __forceinline__ void il_ctask(const FactorDev &F, float *__restrict__
     const float d = V[(size_t)c.dpos * kIl], a = V[(size_t)c.lpos * kIl];
     float uv[4], tv[4];
 #pragma unroll
-    for (uint32_t u = 0; u < 4; ++u)
-        if (c.tt[u] != ~0u) {
-            uv[u] = V[(size_t)c.up[u] * kIl];
-            tv[u] = V[(size_t)c.tt[u] * kIl];
-        }
+    for (uint32_t u = 0; u < 4; ++u) {   // branch-free loads, as in ctab_run
+        uv[u] = V[(size_t)c.up[u] * kIl];
+        tv[u] = V[(size_t)(c.tt[u] != ~0u ? c.tt[u] : 0u) * kIl];
+    }
     const float l = a / gesp(d, pert, perturb);
 #pragma unroll
     for (uint32_t u = 0; u < 4; ++u)
@@ -466,13 +467,15 @@ __device__ __forceinline__ void il_merge(const FactorDev &F, float *__restrict__
         const uint32_t s1 = F.msrc_ptr[e + 1];
         uint32_t s = F.msrc_ptr[e] + 2;
         for (; s + 4 <= s1; s += 4) {
+            MergeSrc m[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) m[u] = F.msrc[s + u];
+            __builtin_amdgcn_sched_barrier(0);   // the four table loads issue before any value load
             float p[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const MergeSrc m = F.msrc[s + u];
-                p[u] = (V[(size_t)m.lpos * kIl] / gesp(V[(size_t)m.dpos * kIl], pert, perturb)) *
-                       V[(size_t)m.upos * kIl];
-            }
+            for (int u = 0; u < 4; ++u)
+                p[u] = (V[(size_t)m[u].lpos * kIl] / gesp(V[(size_t)m[u].dpos * kIl], pert, perturb)) *
+                       V[(size_t)m[u].upos * kIl];
 #pragma unroll
             for (int u = 0; u < 4; ++u) v = v + (-p[u]);
         }
