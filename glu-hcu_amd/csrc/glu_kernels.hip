@@ -775,19 +775,20 @@ __device__ __forceinline__ void factor_all_wg(const FactorDev &F, float *__restr
     const uint32_t nd = F.dvd_ptr[F.num_lev], tid = threadIdx.x, nt = blockDim.x;
     constexpr int U = 8;
     for (uint32_t t0 = (l0 > 0 ? F.dvd_ptr[l0 - 1] : 0u) + tid; t0 < nd; t0 += U * nt) {
+        // loads without branches (past the end: the last item again), so
+        // that the eight items' table loads, then value loads, issue together
         uint2 it[U];
         float d[U], x[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) it[u] = t0 + u * nt < nd ? F.dvd[t0 + u * nt] : make_uint2(~0u, 0u);
+        for (int u = 0; u < U; ++u) it[u] = F.dvd[min(t0 + u * nt, nd - 1)];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            d[u] = val[it[u].y & ~kFirstBit];
+            x[u] = val[it[u].x];
+        }
 #pragma unroll
         for (int u = 0; u < U; ++u)
-            if (it[u].x != ~0u) {
-                d[u] = val[it[u].y & ~kFirstBit];
-                x[u] = val[it[u].x];
-            }
-#pragma unroll
-        for (int u = 0; u < U; ++u)
-            if (it[u].x != ~0u) {
+            if (t0 + u * nt < nd) {
                 const uint32_t dpos = it[u].y & ~kFirstBit;
                 const bool pt = perturb && fabsf(d[u]) < pert;
                 val[it[u].x] = x[u] / (pt ? pert : d[u]);
@@ -1954,13 +1955,16 @@ __device__ __forceinline__ void wg_factor(const MatDesc *Md, const MatIO &io, in
             for (uint32_t k0 = cptr[r] + tid; k0 < e1; k0 += U * nt) {
                 uint2 e[U];
                 float v[U];
+                // loads without branches (past the end: the last entry
+                // again), so that the eight table loads, then the eight
+                // value loads, issue together
 #pragma unroll
-                for (int u = 0; u < U; ++u) e[u] = k0 + u * nt < e1 ? cpy[k0 + u * nt] : make_uint2(0u, ~0u);
+                for (int u = 0; u < U; ++u) e[u] = cpy[min(k0 + u * nt, e1 - 1)];
 #pragma unroll
-                for (int u = 0; u < U; ++u) v[u] = e[u].y != ~0u ? io.lu[e[u].x] : 0.f;
+                for (int u = 0; u < U; ++u) v[u] = io.lu[e[u].x];
 #pragma unroll
                 for (int u = 0; u < U; ++u)
-                    if (e[u].y != ~0u) buf[e[u].y] = v[u];
+                    if (k0 + u * nt < e1) buf[e[u].y] = v[u];
             }
             __syncthreads();
             for (uint32_t i = tid; i < len; i += nt) io.lucsr[base + i] = buf[i];
