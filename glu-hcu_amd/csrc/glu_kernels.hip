@@ -606,10 +606,19 @@ __global__ void __launch_bounds__(256) k_il_scatter(RefineDev R, const double *_
     constexpr uint32_t kK = 64;
     __shared__ float tile[kK][kIl + 1];
     const uint32_t g = blockIdx.y, k0 = blockIdx.x * kK, tid = threadIdx.x;
-    for (uint32_t i = tid; i < kK * kIl; i += blockDim.x) {
-        const uint32_t j = i / kK, kk = i % kK, k = k0 + kk, m = g * kIl + j;
-        if (k < R.nnz_a && m < batch)
-            tile[kk][j] = (float)((ax[(size_t)m * R.nnz_a + k] * R.a_rs[k]) * R.a_cs[k]);
+    // loads without branches (past the last entry or matrix: the last one
+    // again, never stored), so that a thread's four issue together
+    float v[kK * kIl / 256];
+#pragma unroll
+    for (uint32_t u = 0; u < kK * kIl / 256; ++u) {
+        const uint32_t i = tid + 256 * u, j = i / kK, k = min(k0 + i % kK, R.nnz_a - 1);
+        const uint32_t m = min(g * kIl + j, batch - 1);
+        v[u] = (float)((ax[(size_t)m * R.nnz_a + k] * R.a_rs[k]) * R.a_cs[k]);
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < kK * kIl / 256; ++u) {
+        const uint32_t i = tid + 256 * u;
+        tile[i % kK][i / kK] = v[u];
     }
     __syncthreads();
     for (uint32_t i = tid; i < kK * kIl; i += blockDim.x) {
@@ -648,16 +657,24 @@ __global__ void __launch_bounds__(256) k_il_to_rows(const float *__restrict__ S,
 {
     __shared__ float tile[kIl][256 + 1];
     const uint32_t g = blockIdx.y, p0 = blockIdx.x * 256, tid = threadIdx.x;
-    const float4 *src = reinterpret_cast<const float4 *>(S + ((size_t)g * nnz + p0) * kIl);
-    for (uint32_t f = tid; f < 256 * kIl / 4; f += 256) {   // a position's kIl values: kIl / 4 float4
-        const uint32_t p = f / (kIl / 4), j = (f % (kIl / 4)) * 4;
-        if (p0 + p < nnz) {
-            const float4 v = src[f];
-            tile[j][p] = v.x;
-            tile[j + 1][p] = v.y;
-            tile[j + 2][p] = v.z;
-            tile[j + 3][p] = v.w;
-        }
+    const float4 *src = reinterpret_cast<const float4 *>(S + (size_t)g * nnz * kIl);
+    // a position's kIl values are kIl / 4 float4; the loads take no branch
+    // (past the last position: the last one again), so a thread's four
+    // issue together
+    constexpr uint32_t kF = 256 * kIl / 4 / 256;
+    float4 v[kF];
+#pragma unroll
+    for (uint32_t u = 0; u < kF; ++u) {
+        const uint32_t f = tid + 256 * u, p = min(p0 + f / (kIl / 4), nnz - 1);
+        v[u] = src[(size_t)p * (kIl / 4) + f % (kIl / 4)];
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < kF; ++u) {
+        const uint32_t f = tid + 256 * u, p = f / (kIl / 4), j = (f % (kIl / 4)) * 4;
+        tile[j][p] = v[u].x;
+        tile[j + 1][p] = v[u].y;
+        tile[j + 2][p] = v[u].z;
+        tile[j + 3][p] = v[u].w;
     }
     __syncthreads();
     if (p0 + tid >= nnz) return;
