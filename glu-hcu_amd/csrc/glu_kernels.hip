@@ -1852,6 +1852,11 @@ __device__ __forceinline__ void df2_sweep(const DfDev &D, const float *__restric
     df2_issue(D, sv, dn, lane, nxt);
     for (uint32_t j = 0; j < mine; ++j) {
         cur = nxt;
+        // this task's loads (issued a task ago) complete before the next
+        // task's are issued: the task's waits inside its polling loop then
+        // cover nothing in flight, where a wait there would also wait for
+        // the next task's loads
+        __builtin_amdgcn_s_waitcnt(0xF70);   // vmcnt(0); expcnt, lgkmcnt untouched
         dn = dn2;
         dn2 = desc(j + 2);
         df2_issue(D, sv, dn, lane, nxt);
