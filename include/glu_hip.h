@@ -221,18 +221,20 @@ glu_status glu_refine_noalloc(glu_plan *p, const double *d_ax, const float *d_lu
 glu_status glu_adjoint_values(glu_plan *p, const double *d_lam, const double *d_x, double *d_grad_ax,
                               void *stream);
 
-/* Same-pattern batch, one persistent launch: for every b < batch,
- * refresh the values d_ax + b*nnz_a into d_lu + b*nnz_lu, factor, solve and
- * refine d_b + b*n into d_x + b*n.  d_iters[b], d_eta[b] (device) receive the
- * per-matrix iteration count and backward error.  flags bit0: perturb. */
+/* Same-pattern batch: for every b < batch, refresh the values
+ * d_ax + b*nnz_a into d_lu + b*nnz_lu, factor, solve and refine d_b + b*n
+ * into d_x + b*n (the wide leading steps as grid launches over an
+ * interleaved copy of the batch, the rest one workgroup per matrix).
+ * d_iters[b], d_eta[b] (device) receive the per-matrix iteration count and
+ * backward error.  flags bit0: perturb. */
 glu_status glu_batch_solve(glu_plan *p, int batch, const double *d_ax, const double *d_b,
                            double *d_x, float *d_lu, double tol, int maxit, int32_t *d_iters,
                            double *d_eta, unsigned flags, void *stream);
 
-/* Same-pattern batched refactor, one launch: for every b < batch, refresh
- * the values d_ax + b*nnz_a into d_lu + b*nnz_lu (frozen scaling, as
- * glu_refresh) and factor them (as glu_factor: bit-identical factors), one
- * workgroup per matrix.  This is LUonDevice (include/numeric.h:6,
+/* Same-pattern batched refactor: for every b < batch, refresh the values
+ * d_ax + b*nnz_a into d_lu + b*nnz_lu (frozen scaling, as glu_refresh) and
+ * factor them (as glu_factor: bit-identical factors), scheduled as
+ * glu_batch_solve's factorization.  This is LUonDevice (include/numeric.h:6,
  * src/numeric.cu:347-639) over many value sets of one pattern, with the
  * values staged from A instead of from Symbolic_Matrix::predictLU
  * (src/symbolic.cc:82-103).  flags bit0: perturb. */
