@@ -421,20 +421,26 @@ __device__ __forceinline__ void il_ctask(const FactorDev &F, float *__restrict__
         if (c.tt[u] != ~0u) V[(size_t)c.tt[u] * kIl] = tv[u] + (-(l * uv[u]));
 }
 
-// Column-task items qa and qb of one matrix, their loads issued together
-// (items of one step are independent: distinct targets, none read by
-// another item).  An unused lane's positions are 0, a valid address.
-__device__ __forceinline__ void il_ctask2(const FactorDev &F, float *__restrict__ V, uint32_t c0, uint32_t qa,
-                                          uint32_t qb, float pert, int perturb)
+// Column-task items q0 + i kIlItems (i < kIlPer) of one matrix, their loads
+// issued together (items of one step are independent: distinct targets,
+// none read by another item).  An unused lane's positions are 0, a valid
+// address.
+constexpr uint32_t kIlPer = 2;   // column-task items per lane
+
+__device__ __forceinline__ void il_ctasks(const FactorDev &F, float *__restrict__ V, uint32_t c0, uint32_t q0,
+                                          float pert, int perturb)
 {
-    CTab c[2];
-    ctab_load(F, c0 + qa / kWave, qa % kWave, c[0]);
-    ctab_load(F, c0 + qb / kWave, qb % kWave, c[1]);
+    CTab c[kIlPer];
+#pragma unroll
+    for (uint32_t i = 0; i < kIlPer; ++i) {
+        const uint32_t q = q0 + i * kIlItems;
+        ctab_load(F, c0 + q / kWave, q % kWave, c[i]);
+    }
     // every load without a branch (an empty slot's U position is 0 and its
     // target is read at 0, valid addresses), so all are issued together
-    float d[2], a[2], uv[2][4], tv[2][4];
+    float d[kIlPer], a[kIlPer], uv[kIlPer][4], tv[kIlPer][4];
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
+    for (uint32_t i = 0; i < kIlPer; ++i) {
         d[i] = V[(size_t)c[i].dpos * kIl];
         a[i] = V[(size_t)c[i].lpos * kIl];
 #pragma unroll
@@ -444,7 +450,7 @@ __device__ __forceinline__ void il_ctask2(const FactorDev &F, float *__restrict_
         }
     }
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
+    for (uint32_t i = 0; i < kIlPer; ++i) {
         if ((c[i].tt[0] & c[i].tt[1] & c[i].tt[2] & c[i].tt[3]) == ~0u) continue;
         const float l = a[i] / gesp(d[i], pert, perturb);
 #pragma unroll
@@ -551,8 +557,8 @@ __device__ __forceinline__ void il_divide(float *__restrict__ V, uint2 it, float
 // long merges, the rest its column-task items, its short merges, then the
 // division items of the previous step's columns, kIlItems a wave: those
 // columns are final once that step ran, and this step neither reads nor
-// writes them.  A wave takes 2 kIlItems items, the second kIlItems after
-// the first (two column-task items: their loads issued together).
+// writes them.  A wave takes kIlPer kIlItems items, kIlItems apart (column-
+// task items: their loads issued together).
 __global__ void __launch_bounds__(256) k_factor_grid_il(FactorDev F, float *__restrict__ S,
                                                         const double *__restrict__ pertd, int lvl, int perturb,
                                                         uint32_t groups, uint32_t batch)
@@ -571,13 +577,13 @@ __global__ void __launch_bounds__(256) k_factor_grid_il(FactorDev F, float *__re
     }
     if (!valid) return;
     const uint32_t nct = s.nc * kWave, nsm = s.nm - s.ml;
-    const uint32_t qa = (gw - s.ml) * 2 * kIlItems + lane / kIl, qb = qa + kIlItems;
-    if (qb < nct) {
-        il_ctask2(F, V, s.c0, qa, qb, pert, perturb);
+    const uint32_t qa = (gw - s.ml) * kIlPer * kIlItems + lane / kIl, qz = qa + (kIlPer - 1) * kIlItems;
+    if (qz < nct) {
+        il_ctasks(F, V, s.c0, qa, pert, perturb);
         return;
     }
 #pragma unroll 1
-    for (uint32_t q = qa; q <= qb; q += kIlItems) {
+    for (uint32_t q = qa; q <= qz; q += kIlItems) {
         if (q < nct) {
             il_ctask(F, V, s.c0, q, pert, perturb);
         } else if (q - nct < nsm) {
@@ -2815,7 +2821,7 @@ void batch_rhs(const double *b, double *x, double *r64, double *norms, uint32_t 
     k_batch_rhs<<<batch, threads, 0, st>>>(b, x, r64, norms, n);
 }
 
-uint32_t il_items_per_wave() { return kIlItems; }
+uint32_t il_items_per_wave() { return kIlItems * kIlPer; }
 uint32_t il_group() { return kIl; }
 
 void batch_group(const MatDesc *descs, const void *ios, const uint32_t *order, int count, double tol,

@@ -339,7 +339,7 @@ struct glu_plan {
     // batched kernels: steps [0, batch_grid_lev) run as one grid launch per
     // step over an interleaved copy of the batch (grid_waves: each one's
     // waves for one group of interleaved matrices: long merges one each,
-    // other items two per lane)
+    // other items il_items_per_wave() a wave)
     int batch_grid_opt = 0;      // glu_plan_opts.batch_grid_levels
     int batch_grid_lev = 0;
     std::vector<uint32_t> grid_waves;
@@ -1379,7 +1379,7 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
             const uint32_t ml = T.ment_long[l], nm = T.ment_ptr[l + 1] - T.ment_ptr[l];
             const uint32_t items = (T.ctask_ptr[l + 1] - T.ctask_ptr[l]) * 64 + (nm - ml) +
                                    (l > 0 ? T.dvd_ptr[l] - T.dvd_ptr[l - 1] : 0u);   // + previous step's divisions
-            p->grid_waves.push_back(ml + (items + 2 * per_wave - 1) / (2 * per_wave));
+            p->grid_waves.push_back(ml + (items + per_wave - 1) / per_wave);
         }
         p->batch_grid_lev = (int)p->grid_waves.size();
     }
