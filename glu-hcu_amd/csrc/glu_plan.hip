@@ -1383,12 +1383,13 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
         }
         p->batch_grid_lev = (int)p->grid_waves.size();
     }
-    // Division items: those of a column whose step is one of the batched
-    // kernels' grid steps but the last run with the next grid step, the
-    // rest in one pass after the last step (the single path does all at
-    // the end).  In position order a wave's items are consecutive L entries
-    // of a column (each column's first entry first), so they read and write
-    // whole lines.
+    // Division items: those of the columns of each of the batched kernels'
+    // grid steps but the last run with the next grid step (kept step by
+    // step); the rest run in one pass after the last step (the single path
+    // runs them all there).  Each step's items, and the rest, are in
+    // position order: a wave's items are then consecutive L entries of a
+    // column (each column's first entry first) and read and write whole
+    // lines.
     {
         const auto by_pos = [](const uint2 &a, const uint2 &b) { return a.x < b.x; };
         const int early = std::max(p->batch_grid_lev - 1, 0);
