@@ -356,6 +356,8 @@ def test_plan_rejects_bad_krylov(torch_mod):
     for bad in (-2, 5):
         with pytest.raises(RuntimeError):
             Plan(an, krylov=bad)
+    with pytest.raises(RuntimeError):
+        Plan(an, batch_grid_levels=-2)
 
 
 @pytest.mark.parametrize("perturb", [False, True])
