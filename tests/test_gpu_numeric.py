@@ -421,6 +421,46 @@ def test_batch_grid_steps_same_bits(torch_mod, B, perturb):
     assert torch.equal(one.view(torch.int32), runs[0][1][B - 1].view(torch.int32))
 
 
+def test_batch_grid_steps_replaced_pivots(torch_mod):
+    """GESP pivot replacement inside the grid steps: value sets whose
+    columns are in part scaled by 1e-12 (the frozen analysis scales leave
+    those pivots below the GESP threshold, early columns among them) give
+    the same bits with and without the grid steps, the oracle's perturbed
+    factorization bit for bit, and not the oracle's unperturbed one."""
+    from oracle import oracle as O
+    from pyglu import mna
+    from pyglu.device import Plan
+    torch = torch_mod
+    data, indices, indptr, n, an, plan, pat = _setup("batch")
+    rng = np.random.default_rng(77)
+    B = 18
+    sets = []
+    for b in range(B):
+        v = mna.perturb_values(data, 5600 + b).copy()
+        for c in rng.choice(n, 600, replace=False):
+            v[indptr[c]:indptr[c + 1]] *= 1e-12
+        sets.append(v)
+    axd = torch.from_numpy(np.stack(sets)).cuda()
+    bd = torch.from_numpy(np.stack([mna.rhs(n, 5700 + b) for b in range(B)])).cuda()
+    assert plan.batch_grid_levels > 1
+    none = Plan(an, batch_grid_levels=-1)
+    runs = []
+    for pl in (none, plan):
+        x, lu, iters, eta = pl.batch_solve(axd, bd, tol=1e-10, maxit=5, perturb=True)
+        luf = pl.batch_factor(axd, perturb=True)
+        torch.cuda.synchronize()
+        runs.append([t.clone() for t in (x, lu, iters, eta, luf)])
+    none.close()
+    for a, b in zip(runs[0], runs[1]):
+        assert torch.equal(a.view(torch.uint8), b.view(torch.uint8))
+    _, ref = O.refresh(pat, an, indices, indptr, sets[0])
+    want, _ = O.factor_f32(pat, ref, perturb=True)
+    plain, _ = O.factor_f32(pat, ref, perturb=False)
+    got = runs[1][4][0].cpu().numpy()
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    assert not np.array_equal(plain.view(np.uint32), want.view(np.uint32))
+
+
 def test_abft_device_matches_oracle(torch_mod):
     """glu_abft_sums / glu_abft_check (src/symbolic.cc:163-197) on a batch of
     three value sets: checksums bit-identical to the oracle's, the same
