@@ -2365,6 +2365,34 @@ __global__ void __launch_bounds__(1024) k_batch_same_refine(const MatDesc *__res
     wg_refine<kMode>(desc, io, tol, maxit, sh, hs, xs, flags);
 }
 
+// k_batch_same_factor<true> and k_batch_same_refine<kMode> in one
+// workgroup per matrix (each matrix's refinement starts as soon as its
+// factors are copied).  Dynamic LDS: the larger of the two.
+template <int kMode>
+__global__ void __launch_bounds__(1024) k_batch_same_fused(const MatDesc *__restrict__ desc,
+                                                           const double *__restrict__ ax, const double *__restrict__ b,
+                                                           double *__restrict__ x, float *__restrict__ lu,
+                                                           float *__restrict__ lucsr, float *__restrict__ t32,
+                                                           double *__restrict__ r64, double *__restrict__ norms,
+                                                           double *__restrict__ kry, int32_t *__restrict__ iters,
+                                                           double *__restrict__ eta, int perturb, int l0, double tol,
+                                                           int maxit)
+{
+    __shared__ double sh[1024 / kWave];
+    __shared__ __attribute__((aligned(16))) float wl[1024 / kWave][kWave];
+    static_assert(sizeof(double) * kKrylovLds <= sizeof(wl), "Arnoldi scalars exceed the wave slices");
+    const MatIO io = batch_io(desc, ax, b, x, lu, lucsr, t32, r64, norms, kry, iters, eta);
+    wg_factor<true>(desc, io, perturb, sh, wl[threadIdx.x / kWave], l0);
+    __syncthreads();
+    // the refinement's Arnoldi scalars take the factorization's wave slices
+    // (static LDS no larger than k_batch_same_factor's)
+    double *hs = reinterpret_cast<double *>(&wl[0][0]);
+    uint32_t *flags;
+    float *xs;
+    batch_lds<kMode>(desc->n, io.t32, flags, xs);
+    wg_refine<kMode>(desc, io, tol, maxit, sh, hs, xs, flags);
+}
+
 // One matrix's factors gathered into the sweeps' layout (grid, a thread per
 // slot): lucsr[q] = lu[df_src[q]], 0 for padding slots.
 template <bool kT>
@@ -2683,23 +2711,29 @@ size_t batch_lds_bytes(int mode, uint32_t n)
     return mode == kSweepLevels ? 0 : 4 * words + (mode == kSweepLds ? 4 * (size_t)n : 0);
 }
 
-// Two launches: factor (dynamic LDS fact_lds: the largest window's values), then
-// refine in sweep mode `mode` (dynamic LDS ref_lds).
+// Sweep mode kSweepLds: one launch, factor then refine in each workgroup
+// (dynamic LDS: the larger of fact_lds and ref_lds; 4.67 -> 4.65 ms per
+// step of 256 n=20k matrices against the two launches, the refinements no
+// longer waiting for the slowest factorization).  Other modes: two
+// launches, factor (dynamic LDS fact_lds: the largest window's values),
+// then refine (dynamic LDS ref_lds).
 void batch_same(const MatDesc *desc, int batch, const double *ax, const double *b, double *x, float *lu,
                 float *lucsr, float *t32, double *r64, double *norms, double *kry, int32_t *iters, double *eta,
                 double tol, int maxit, int perturb, int threads, int mode, size_t fact_lds, size_t ref_lds,
                 int l0, hipStream_t st)
 {
+    if (mode == kSweepLds) {
+        k_batch_same_fused<kSweepLds><<<batch, threads, std::max(fact_lds, ref_lds), st>>>(
+            desc, ax, b, x, lu, lucsr, t32, r64, norms, kry, iters, eta, perturb, l0, tol, maxit);
+        return;
+    }
     if (mode == kSweepLevels)
         k_batch_same_factor<false><<<batch, threads, fact_lds, st>>>(desc, ax, b, x, lu, lucsr, t32, r64, norms,
                                                                       kry, iters, eta, perturb, l0);
     else
         k_batch_same_factor<true><<<batch, threads, fact_lds, st>>>(desc, ax, b, x, lu, lucsr, t32, r64, norms,
                                                                      kry, iters, eta, perturb, l0);
-    if (mode == kSweepLds)
-        k_batch_same_refine<kSweepLds><<<batch, threads, ref_lds, st>>>(desc, ax, b, x, lu, lucsr, t32, r64, norms,
-                                                                         kry, iters, eta, tol, maxit);
-    else if (mode == kSweepFlags)
+    if (mode == kSweepFlags)
         k_batch_same_refine<kSweepFlags><<<batch, threads, ref_lds, st>>>(desc, ax, b, x, lu, lucsr, t32, r64,
                                                                            norms, kry, iters, eta, tol, maxit);
     else
@@ -2883,7 +2917,7 @@ size_t enable_lds()
         reinterpret_cast<const void *>(&k_sweeps_df<false, true>),
         reinterpret_cast<const void *>(&k_sweeps_df<true, true>),
         reinterpret_cast<const void *>(&k_solve_one<kSweepFlags, true>),
-        reinterpret_cast<const void *>(&k_batch_same_refine<kSweepLds>),
+        reinterpret_cast<const void *>(&k_batch_same_fused<kSweepLds>),
         reinterpret_cast<const void *>(&k_batch_same_refine<kSweepFlags>),
         reinterpret_cast<const void *>(&k_batch_same_refine<kSweepLevels>),
         reinterpret_cast<const void *>(&k_batch_group_factor<true>),
