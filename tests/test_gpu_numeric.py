@@ -421,6 +421,54 @@ def test_batch_grid_steps_same_bits(torch_mod, B, perturb):
     assert torch.equal(one.view(torch.int32), runs[0][1][B - 1].view(torch.int32))
 
 
+def test_batch_vector_beyond_lds(torch_mod):
+    """A same-pattern batch whose order's correction vector does not fit in
+    LDS (n = 50,000: 200 KB of FP32 against 160 KB per workgroup), so the
+    refinement keeps its vector in global memory and only the ready bits in
+    LDS (factor and refinement as two launches): the same bits with and
+    without the grid steps, each matrix's factors equal to the single path's
+    and matrix 0's to the oracle's factorization of the oracle's refreshed
+    values, every backward error within 1e-10."""
+    from oracle import oracle as O
+    from pyglu import mna
+    from pyglu.analysis import Analysis
+    from pyglu.device import Plan
+    torch = torch_mod
+    data, indices, indptr, n = mna.mna_csc(mna.SPEC_GRAD, mna.SEED_GRAD)
+    assert 4 * n > 160 * 1024
+    an = Analysis(data, indices, indptr, n)
+    pat = O.SymPattern(an)
+    plan, none = Plan(an), Plan(an, batch_grid_levels=-1)
+    assert plan.batch_grid_levels > 1
+    B = 3
+    ax = np.stack([mna.perturb_values(data, 6100 + b) for b in range(B)])
+    bb = np.stack([mna.rhs(n, 6300 + b) for b in range(B)])
+    axd, bd = torch.from_numpy(ax).cuda(), torch.from_numpy(bb).cuda()
+    runs = []
+    for pl in (none, plan):
+        x, lu, iters, eta = pl.batch_solve(axd, bd, tol=1e-10, maxit=20)
+        luf = pl.batch_factor(axd)
+        torch.cuda.synchronize()
+        runs.append([t.clone() for t in (x, lu, iters, eta, luf)])
+    none.close()
+    for a, b in zip(runs[0], runs[1]):
+        assert torch.equal(a.view(torch.uint8), b.view(torch.uint8))
+    x, lu, iters, eta, luf = runs[1]
+    assert torch.equal(lu.view(torch.int32), luf.view(torch.int32))
+    for b in range(B):
+        one = plan.refresh(axd[b]).reshape(-1)
+        plan.factor(one)
+        torch.cuda.synchronize()
+        assert torch.equal(one.view(torch.int32), lu[b].view(torch.int32)), b
+        e = backward_error(ax[b], indices, indptr, n, x[b].cpu().numpy(), bb[b])
+        print(f"  n={n} matrix {b}: iters={int(iters[b])} eta={float(eta[b]):.2e} host={e:.2e}", flush=True)
+        assert float(eta[b]) <= 1e-10 and e <= 2e-10
+    _, ref = O.refresh(pat, an, indices, indptr, ax[0])
+    want, _ = O.factor_f32(pat, ref, perturb=False)
+    assert np.array_equal(lu[0].cpu().numpy().view(np.uint32), want.view(np.uint32))
+    plan.close()
+
+
 def test_batch_grid_steps_replaced_pivots(torch_mod):
     """GESP pivot replacement inside the grid steps: value sets whose
     columns are in part scaled by 1e-12 (the frozen analysis scales leave
