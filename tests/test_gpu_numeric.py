@@ -421,14 +421,17 @@ def test_batch_grid_steps_same_bits(torch_mod, B, perturb):
     assert torch.equal(one.view(torch.int32), runs[0][1][B - 1].view(torch.int32))
 
 
-def test_batch_vector_beyond_lds(torch_mod):
+@pytest.mark.parametrize("perturb", [False, True])
+def test_batch_vector_beyond_lds(torch_mod, perturb):
     """A same-pattern batch whose order's correction vector does not fit in
     LDS (n = 50,000: 200 KB of FP32 against 160 KB per workgroup), so the
     refinement keeps its vector in global memory and only the ready bits in
     LDS (factor and refinement as two launches): the same bits with and
     without the grid steps, each matrix's factors equal to the single path's
     and matrix 0's to the oracle's factorization of the oracle's refreshed
-    values, every backward error within 1e-10."""
+    values, every backward error within 1e-10 (with GESP, whose replaced
+    pivots can slow the refinement: within 1e-10 or the true residual's
+    backward error reported at maxit)."""
     from oracle import oracle as O
     from pyglu import mna
     from pyglu.analysis import Analysis
@@ -446,8 +449,8 @@ def test_batch_vector_beyond_lds(torch_mod):
     axd, bd = torch.from_numpy(ax).cuda(), torch.from_numpy(bb).cuda()
     runs = []
     for pl in (none, plan):
-        x, lu, iters, eta = pl.batch_solve(axd, bd, tol=1e-10, maxit=20)
-        luf = pl.batch_factor(axd)
+        x, lu, iters, eta = pl.batch_solve(axd, bd, tol=1e-10, maxit=20, perturb=perturb)
+        luf = pl.batch_factor(axd, perturb=perturb)
         torch.cuda.synchronize()
         runs.append([t.clone() for t in (x, lu, iters, eta, luf)])
     none.close()
@@ -457,14 +460,19 @@ def test_batch_vector_beyond_lds(torch_mod):
     assert torch.equal(lu.view(torch.int32), luf.view(torch.int32))
     for b in range(B):
         one = plan.refresh(axd[b]).reshape(-1)
-        plan.factor(one)
+        plan.factor(one, perturb=perturb)
         torch.cuda.synchronize()
         assert torch.equal(one.view(torch.int32), lu[b].view(torch.int32)), b
         e = backward_error(ax[b], indices, indptr, n, x[b].cpu().numpy(), bb[b])
         print(f"  n={n} matrix {b}: iters={int(iters[b])} eta={float(eta[b]):.2e} host={e:.2e}", flush=True)
-        assert float(eta[b]) <= 1e-10 and e <= 2e-10
+        if perturb:
+            # replaced pivots make the factors a weaker preconditioner: the
+            # refinement may stop at maxit, reporting the true residual's eta
+            assert abs(float(eta[b]) - e) <= 0.01 * e and (float(eta[b]) <= 1e-10 or int(iters[b]) == 20)
+        else:
+            assert float(eta[b]) <= 1e-10 and e <= 2e-10
     _, ref = O.refresh(pat, an, indices, indptr, ax[0])
-    want, _ = O.factor_f32(pat, ref, perturb=False)
+    want, _ = O.factor_f32(pat, ref, perturb=perturb)
     assert np.array_equal(lu[0].cpu().numpy().view(np.uint32), want.view(np.uint32))
     plan.close()
 
