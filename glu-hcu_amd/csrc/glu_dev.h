@@ -199,7 +199,8 @@ struct RefineDev {
     const uint32_t *a_dst;     // [nnz_a]
     const double *a_rs;        // rs_col of the entry's LU column [nnz_a]
     const double *a_cs;        // csp of the entry's LU row [nnz_a]
-    uint32_t nnz_a;
+    const uint32_t *lu_fill;   // LU positions no caller value lands on (fill-in), ascending [nfill]
+    uint32_t nnz_a, nfill;
 };
 
 struct MatDesc {

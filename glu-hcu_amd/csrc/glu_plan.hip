@@ -1493,7 +1493,7 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
     // Refinement / refresh tables from the analysis.
     std::vector<uint32_t> a_rptr, a_cidx, a_src, lu_src, lu_col, a_rows, a_cols;
     std::vector<double> rs_col, a_rs, a_cs;
-    std::vector<uint32_t> a_dst;
+    std::vector<uint32_t> a_dst, lu_fill;
     const Analysis *an = a ? &a->an : nullptr;
     if (an) {
         const uint32_t nnz_a = an->nnz;
@@ -1536,6 +1536,8 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
         }
         for (uint32_t k = 0; k < nnz_a; ++k)
             if (a_dst[k] == UINT32_MAX) return fail(GLU_ERR_INVALID, "a value of A has no LU position");
+        for (uint32_t q = 0; q < P.nnz; ++q)
+            if (lu_src[q] == UINT32_MAX) lu_fill.push_back(q);
         // rows / columns of A with more than kLongRow entries first
         auto long_first = [&](const uint32_t *ptr, std::vector<uint32_t> &order, uint32_t &nlong) {
             order.clear();
@@ -1607,7 +1609,7 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
     }
     size_t o_rp = 0, o_rpi = 0, o_cp = 0, o_cpi = 0, o_rs = 0, o_csp = 0;
     size_t o_arptr = 0, o_acidx = 0, o_asrc = 0, o_acptr = 0, o_aridx = 0, o_lusrc = 0, o_rscol = 0;
-    size_t o_lucol = 0, o_arows = 0, o_acols = 0, o_adst = 0, o_ars = 0, o_acs = 0;
+    size_t o_lucol = 0, o_arows = 0, o_acols = 0, o_adst = 0, o_ars = 0, o_acs = 0, o_lufill = 0;
     if (an) {
         o_rp = U.put(an->row_perm);
         o_rpi = U.put(an->row_perm_inv);
@@ -1626,6 +1628,7 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
         o_arows = U.put(a_rows);
         o_acols = U.put(a_cols);
         o_adst = U.put(a_dst);
+        o_lufill = U.put(lu_fill);
         o_ars = U.put(a_rs);
         o_acs = U.put(a_cs);
     }
@@ -1763,6 +1766,8 @@ glu_status build_plan(const glu_pattern &P, const glu_analysis *a, const glu_pla
         R.a_nlong_t = p->a_nlong_t;
         R.rs_col = reinterpret_cast<const double *>(A + o_rscol);
         R.a_dst = u32p(o_adst);
+        R.lu_fill = u32p(o_lufill);
+        R.nfill = (uint32_t)lu_fill.size();
         R.a_rs = reinterpret_cast<const double *>(A + o_ars);
         R.a_cs = reinterpret_cast<const double *>(A + o_acs);
         R.csp = S.col_scale_perm;

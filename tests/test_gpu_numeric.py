@@ -382,12 +382,13 @@ def test_batch_factor_matches_single(torch_mod, perturb):
         assert torch.equal(one.view(torch.int32), lu[b].view(torch.int32)), b
 
 
-@pytest.mark.parametrize("B", [40, 17, 5, 1])
+@pytest.mark.parametrize("B", [40, 32, 17, 5, 1])
 @pytest.mark.parametrize("perturb", [False, True])
 def test_batch_grid_steps_same_bits(torch_mod, B, perturb):
     """The batched kernels' leading steps as grid launches over an
     interleaved copy of the batch (groups of 16 matrices; 40 = two full
-    groups and a partial one, 17 = one full group and a single matrix) give
+    groups and a partial one, 32 = two full groups, whose copy is zeroed on
+    the fill-in lines only, 17 = one full group and a single matrix) give
     the same bits as the workgroup-per-matrix
     kernel alone (one, three and the default number of grid steps):
     factors, solutions, iteration counts and backward errors of
