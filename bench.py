@@ -642,7 +642,7 @@ def main():
         "autograd": grad,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
-                     "kernel": "one batch step: k_il_zero_fill, k_il_scatter, k_batch_rhs, k_factor_grid_il per wide "
+                     "kernel": "one batch step: k_il_zero_fill, k_il_scatter, k_factor_grid_il per wide "
                                "step, k_il_to_rows, k_batch_same_fused (per-matrix factor tail, sweep copy, refinement)",
                      "kernel_ms": round(kernel_ms, 4), "alg_bytes": round(traffic_alg)},
         "cpu_baseline": cpu,

@@ -698,24 +698,6 @@ __global__ void __launch_bounds__(256) k_il_to_rows(const float *__restrict__ S,
         lu[(size_t)(g * kIl + j) * nnz + p0 + tid] = tile[j][tid];
 }
 
-// |b|_inf of matrix blockIdx.x with x = 0 and r64 = b (wg_values' step 1
-// for the right-hand side): norms[2 m + 1].
-__global__ void __launch_bounds__(1024) k_batch_rhs(const double *__restrict__ b, double *__restrict__ x,
-                                                    double *__restrict__ r64, double *__restrict__ norms, uint32_t n)
-{
-    __shared__ double sh[1024 / kWave];
-    const size_t m = blockIdx.x;
-    double bmax = 0.0;
-    for (uint32_t c = threadIdx.x; c < n; c += blockDim.x) {
-        const double v = b[m * n + c];
-        bmax = nmax(bmax, fabs(v));
-        x[m * n + c] = 0.0;
-        r64[m * n + c] = v;
-    }
-    const double bnorm = wg_max(bmax, sh);
-    if (threadIdx.x == 0) norms[2 * m + 1] = bnorm;
-}
-
 // A run of narrow levels inside one workgroup, a barrier between levels.
 __global__ void __launch_bounds__(1024) k_factor_fused(FactorDev F, float *__restrict__ val,
                                                        const float *__restrict__ d_pert, int l0, int l1, int perturb)
@@ -1953,9 +1935,9 @@ __device__ __forceinline__ void wg_values(const MatDesc *Md, const MatIO &io, in
     }
 }
 
-// Steps 1-4 (l0 = 0), or steps l0.. of step 3 and step 4 after the
-// interleaved launches of steps [0, l0) (k_factor_grid_il; GESP value in
-// io.norms[0], |b|_inf in io.norms[1]).
+// Steps 1-4 (l0 = 0), or, after the interleaved launches of steps [0, l0)
+// (k_factor_grid_il; GESP value in io.norms[0]), |b|_inf into io.norms[1]
+// with x = 0 and r = b, steps l0.. of step 3 and step 4.
 template <bool kCopy>
 __device__ __forceinline__ void wg_factor(const MatDesc *Md, const MatIO &io, int perturb, double *sh, float *wlds,
                                           int l0)
@@ -1963,8 +1945,21 @@ __device__ __forceinline__ void wg_factor(const MatDesc *Md, const MatIO &io, in
     const uint32_t tid = threadIdx.x, nt = blockDim.x;
     double bnorm = 0.0;
     float pert;
-    if (l0 == 0) wg_values(Md, io, perturb, sh, bnorm, pert);
-    else pert = perturb ? (float)io.norms[0] : 0.f;
+    if (l0 == 0) {
+        wg_values(Md, io, perturb, sh, bnorm, pert);
+    } else {
+        pert = perturb ? (float)io.norms[0] : 0.f;
+        double bmax = 0.0;
+        if (io.b) {   // null: refactor only (k_batch_factor)
+            for (uint32_t c = tid; c < Md->n; c += nt) {
+                const double v = io.b[c];
+                bmax = nmax(bmax, fabs(v));
+                io.x[c] = 0.0;
+                io.r64[c] = v;
+            }
+            bnorm = wg_max(bmax, sh);
+        }
+    }
 
     // 3. factorization, level by level; LDS windows with their values in LDS
     //    (the sweeps' LDS is free until step 5)
@@ -2009,7 +2004,7 @@ __device__ __forceinline__ void wg_factor(const MatDesc *Md, const MatIO &io, in
             __syncthreads();
         }
     }
-    if (tid == 0 && io.norms && l0 == 0) io.norms[1] = bnorm;
+    if (tid == 0 && io.norms && (l0 == 0 || io.b)) io.norms[1] = bnorm;
 }
 
 // Workgroup sum in a fixed order (deterministic): each wave's butterfly,
@@ -2867,12 +2862,6 @@ void il_to_rows(const float *S, float *lu, uint32_t nnz, int batch, hipStream_t 
 {
     const uint32_t groups = (uint32_t)((batch + kIl - 1) / kIl);
     k_il_to_rows<<<dim3(blocks_for(nnz, 256), groups), 256, 0, st>>>(S, lu, nnz, (uint32_t)batch);
-}
-
-void batch_rhs(const double *b, double *x, double *r64, double *norms, uint32_t n, int batch, int threads,
-               hipStream_t st)
-{
-    k_batch_rhs<<<batch, threads, 0, st>>>(b, x, r64, norms, n);
 }
 
 uint32_t il_items_per_wave() { return kIlItems * kIlPer; }

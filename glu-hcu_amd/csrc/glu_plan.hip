@@ -59,8 +59,6 @@ void il_values(const RefineDev &R, const FactorDev &F, const double *ax, float *
 void factor_grid_il(const FactorDev &F, float *S, const double *pertd, int lvl, uint32_t waves, int batch,
                     int perturb, hipStream_t st);
 void il_to_rows(const float *S, float *lu, uint32_t nnz, int batch, hipStream_t st);
-void batch_rhs(const double *b, double *x, double *r64, double *norms, uint32_t n, int batch, int threads,
-               hipStream_t st);
 uint32_t il_items_per_wave();
 uint32_t il_group();
 void abft_cca(const FactorDev &F, const float *val, float *cca, int batch, hipStream_t st);
@@ -2299,16 +2297,15 @@ glu_status ensure_il(glu_plan *p, int batch)
 }
 
 // The batched kernels' first steps: the values of every matrix into the
-// interleaved copy (with each GESP value in norms[2 m]) and, with a
-// right-hand side, |b|_inf into norms[2 m + 1] with x = 0 and r64 = b; steps
+// interleaved copy (with each GESP value in norms[2 m]); steps
 // [0, batch_grid_lev) as one grid launch each; the factors so far into
-// d_lu.  The per-matrix workgroups go on from there.
-void batch_grid_steps(glu_plan *p, int batch, const double *d_ax, const double *d_b, double *d_x, float *d_lu,
-                      double *r64, double *norms, int perturb, hipStream_t st)
+// d_lu.  The per-matrix workgroups go on from there (with a right-hand
+// side, from |b|_inf into norms[2 m + 1], x = 0 and r = b).
+void batch_grid_steps(glu_plan *p, int batch, const double *d_ax, float *d_lu, double *norms, int perturb,
+                      hipStream_t st)
 {
     if (p->batch_grid_lev == 0) return;
     launch::il_values(p->R, p->F, d_ax, p->d_il, p->d_il_bits, norms, p->nnz, p->n, batch, perturb, st);
-    if (d_b) launch::batch_rhs(d_b, d_x, r64, norms, p->n, batch, p->fused_threads, st);
     for (int l = 0; l < p->batch_grid_lev; ++l)
         launch::factor_grid_il(p->F, p->d_il, norms, l, p->grid_waves[l], batch, perturb, st);
     launch::il_to_rows(p->d_il, d_lu, p->nnz, batch, st);
@@ -2330,7 +2327,7 @@ extern "C" glu_status glu_batch_solve(glu_plan *p, int batch, const double *d_ax
     const int mode = sweep_mode(p->n, cap);
     const int perturb = (int)(flags & 1u);
     hipStream_t st = static_cast<hipStream_t>(stream);
-    batch_grid_steps(p, batch, d_ax, d_b, d_x, d_lu, p->d_br64, p->d_bnorms, perturb, st);
+    batch_grid_steps(p, batch, d_ax, d_lu, p->d_bnorms, perturb, st);
     launch::batch_same(p->d_desc, batch, d_ax, d_b, d_x, d_lu, p->d_blucsr, p->d_bt32, p->d_br64, p->d_bnorms,
                        p->d_bkry, d_iters, d_eta, tol, maxit, perturb, p->fused_threads, mode,
                        sizeof(float) * std::max(p->win_nslot, p->copy_range), launch::batch_lds_bytes(mode, p->n),
@@ -2350,7 +2347,7 @@ extern "C" glu_status glu_batch_factor(glu_plan *p, int batch, const double *d_a
     hipStream_t st = static_cast<hipStream_t>(stream);
     glu_status s = ensure_il(p, batch);
     if (s != GLU_OK) return s;
-    batch_grid_steps(p, batch, d_ax, nullptr, nullptr, d_lu, nullptr, p->d_fnorms, perturb, st);
+    batch_grid_steps(p, batch, d_ax, d_lu, p->d_fnorms, perturb, st);
     launch::batch_factor(p->d_desc, batch, d_ax, d_lu, p->d_fnorms, perturb, p->fused_threads,
                          sizeof(float) * p->win_nslot, p->batch_grid_lev, st);
     hipError_t e = hipGetLastError();
