@@ -595,7 +595,6 @@ __global__ void __launch_bounds__(256) k_factor_grid_il(FactorDev F, float *__re
     }
 }
 
-// S = 0 (float4 stores; count4 = floats / 4).
 // Zeros on the fill-in lines of group blockIdx.y (a line: one position's 16
 // matrices, four float4 a line).
 __global__ void __launch_bounds__(256) k_il_zero_fill(const uint32_t *__restrict__ fill, uint32_t nfill,
@@ -606,6 +605,7 @@ __global__ void __launch_bounds__(256) k_il_zero_fill(const uint32_t *__restrict
     S[((size_t)blockIdx.y * nnz + fill[i / 4]) * 4 + i % 4] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
+// S = 0 (float4 stores; count4 = floats / 4).
 __global__ void k_il_zero(float4 *__restrict__ S, size_t count4)
 {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count4; i += (size_t)gridDim.x * blockDim.x)
